@@ -1,0 +1,897 @@
+// C-ABI of libedlat_hip (include/edlat_gpu.h): context, Hamiltonian and
+// sector set-up, the operator, the Lanczos drivers and the batched seeds.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/edlat_gpu.h"
+#include "chains.hpp"
+#include "seeds.hpp"
+#include "tridiag.hpp"
+
+struct edl_ctx {
+  edl::Ctx c;
+};
+
+namespace edl {
+
+InterParams& inter_of(Ctx& c) { return c.model.inter; }
+
+Sector& get_sector(Ctx& c, int isector) {
+  auto it = c.sectors.find(isector);
+  if (it != c.sectors.end()) return *it->second;
+  EDL_CHECK(c.model.set, "Hamiltonian not set (call edl_set_hamiltonian first)");
+  const int Ns = c.model.Ns;
+  EDL_CHECK(isector >= 1 && isector <= (Ns + 1) * (Ns + 1), "sector index out of range");
+  auto s = std::make_unique<Sector>();
+  s->isector = isector;
+  // get_Nup / get_Ndw (SECTOR.f90:401-424)
+  s->Ndw = (isector - 1) % (Ns + 1);
+  s->Nup = ((isector - 1) / (Ns + 1)) % (Ns + 1);
+  build_sector_tables(c, *s);
+  Sector& ref = *s;
+  c.sectors[isector] = std::move(s);
+  return ref;
+}
+
+static void release_sectors(Ctx& c) {
+  for (auto& kv : c.sectors) free_sector(*kv.second);
+  c.sectors.clear();
+}
+
+// ------------------------------------------------------------ utilities
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { EDL_HIP(hipMalloc(&p, bytes ? bytes : 4)); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct EventTimer {
+  hipEvent_t a{}, b{};
+  hipStream_t st;
+  explicit EventTimer(hipStream_t s) : st(s) {
+    EDL_HIP(hipEventCreate(&a));
+    EDL_HIP(hipEventCreate(&b));
+    EDL_HIP(hipEventRecord(a, st));
+  }
+  double stop() {
+    EDL_HIP(hipEventRecord(b, st));
+    EDL_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    EDL_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+  ~EventTimer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+template <typename T>
+double device_sumsq(const T* x, int64_t n, hipStream_t st);
+
+// ----------------------------------------------------------- chain runs
+struct ChainOut {
+  int nsteps = 0;
+  std::vector<double> alpha, beta;
+  double norm2 = 0.0;
+};
+
+// Per-chain device state allocated for a run.
+template <typename T>
+struct ChainAlloc {
+  DevBuf q0, q1, sc, alpha, beta, counter;
+};
+
+template <typename T>
+static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, const double* d_start, double norm2,
+                       int nmax, hipStream_t st) {
+  const int64_t D = S.dim;
+  fill_sector_view(S, sizeof(T) == 4, cd);
+  A.q0 = DevBuf(sizeof(T) * D);
+  A.q1 = DevBuf(sizeof(T) * D);
+  A.sc = DevBuf(sizeof(LzScalars));
+  A.alpha = DevBuf(sizeof(double) * nmax);
+  A.beta = DevBuf(sizeof(double) * nmax);
+  A.counter = DevBuf(sizeof(unsigned));
+  launch_convert<T>(d_start, A.q0.template as<T>(), D, st);
+  launch_zero_bytes(A.q1.p, sizeof(T) * D, st);
+  launch_zero_bytes(A.alpha.p, sizeof(double) * nmax, st);
+  launch_zero_bytes(A.beta.p, sizeof(double) * nmax, st);
+  launch_zero_bytes(A.counter.p, 4, st);
+  const double ss = device_sumsq<T>(A.q0.template as<T>(), D, st);
+  LzScalars s{};
+  s.k = 0;
+  s.stopped = (norm2 == 0.0 || ss == 0.0) ? 1 : 0;
+  s.nsteps = 0;
+  s.s_cur = ss > 0.0 ? std::sqrt(ss) : 1.0;
+  s.s_old = 1.0;
+  s.beta_k = 0.0;
+  s.alpha_k = 0.0;
+  s.norm2 = norm2;
+  EDL_HIP(hipMemcpyAsync(A.sc.p, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  cd.q0 = A.q0.p;
+  cd.q1 = A.q1.p;
+  cd.x = nullptr;
+  cd.ccoef = nullptr;
+  cd.counter = A.counter.template as<unsigned>();
+  cd.sc = A.sc.template as<LzScalars>();
+  cd.alpha = A.alpha.template as<double>();
+  cd.beta = A.beta.template as<double>();
+  cd.nmax = nmax;
+  cd.brk_tol = 1e-14;
+}
+
+// Autotuned schedule cache: key = (precision, sum of dims, #chains).
+struct TuneKey {
+  int prec;
+  int64_t dims;
+  int n;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, dims, n) < std::tie(o.prec, o.dims, o.n);
+  }
+};
+static std::map<TuneKey, Schedule>& tune_cache() {
+  static std::map<TuneKey, Schedule> m;
+  return m;
+}
+
+// Pick the tile/wave schedule of a chain set by timing phase A at step 0,
+// which is idempotent there (beta_0 = 0, q_{-1} = 0).
+template <typename T>
+static Schedule autotune(Ctx& c, const std::vector<ChainDev>& chains, hipStream_t st) {
+  TuneKey key{(int)sizeof(T), 0, (int)chains.size()};
+  for (auto& cd : chains) key.dims += (int64_t)cd.dimup * cd.dimdw;
+  auto it = tune_cache().find(key);
+  if (it != tune_cache().end()) return it->second;
+  const int threads[] = {256, 512, 1024};
+  const int rpt[] = {1, 2};
+  const int gmul[] = {4, 8, 16};
+  Schedule best;
+  best.block_threads = 256;
+  best.rows_per_thread = 1;
+  best.grid = c.num_cu * 8;
+  double best_ms = 1e300;
+  for (int th : threads)
+    for (int r : rpt)
+      for (int gm : gmul) {
+        Schedule w;
+        w.block_threads = th;
+        w.rows_per_thread = r;
+        w.grid = c.num_cu * gm;
+        ChainSet<T> cs;
+        cs.h = chains;
+        chainset_finalize(c, cs, w);
+        lz_phase_a(c, cs, st);  // warm-up
+        EventTimer tm(st);
+        for (int rep = 0; rep < 3; ++rep) lz_phase_a(c, cs, st);
+        const double ms = tm.stop();
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = cs.sch;
+          best.grid = w.grid;
+        }
+      }
+  tune_cache()[key] = best;
+  return best;
+}
+
+// Run the chain set until every chain has stopped or `steps` steps, with the
+// two phases of a step captured once in a hipGraph and replayed.  With
+// check > 0 and a single chain, the lowest Ritz value is checked every
+// `check` steps and the run ends when it changes by <= tol*|E|.
+template <typename T>
+static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, hipStream_t st) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  lz_phase_a(c, cs, st);
+  lz_phase_b(c, cs, st);
+  EDL_HIP(hipStreamEndCapture(st, &g));
+  EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  double e_prev = 0.0;
+  bool have_prev = false;
+  int done = 0;
+  for (int k = 0; k < steps; ++k) {
+    EDL_HIP(hipGraphLaunch(ge, st));
+    ++done;
+    if (check > 0 && tol > 0.0 && (k + 1) % check == 0 && cs.h.size() == 1) {
+      LzScalars s{};
+      EDL_HIP(hipMemcpyAsync(&s, cs.h[0].sc, sizeof(s), hipMemcpyDeviceToHost, st));
+      EDL_HIP(hipStreamSynchronize(st));
+      const int m = s.nsteps;
+      if (s.stopped) break;
+      std::vector<double> a(m), b(m), w(m);
+      EDL_HIP(hipMemcpy(a.data(), cs.h[0].alpha, sizeof(double) * m, hipMemcpyDeviceToHost));
+      EDL_HIP(hipMemcpy(b.data(), cs.h[0].beta, sizeof(double) * m, hipMemcpyDeviceToHost));
+      std::vector<double> e(m, 0.0);
+      for (int i = 0; i + 1 < m; ++i) e[i] = b[i + 1];
+      tridiag_eig(m, a.data(), e.data(), w.data(), nullptr, nullptr);
+      if (have_prev && std::fabs(w[0] - e_prev) <= tol * std::fabs(w[0])) break;
+      e_prev = w[0];
+      have_prev = true;
+    }
+  }
+  EDL_HIP(hipStreamSynchronize(st));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return done;
+}
+
+template <typename T>
+static ChainOut read_chain(const ChainDev& cd, int nmax, hipStream_t st) {
+  ChainOut o;
+  LzScalars s{};
+  EDL_HIP(hipMemcpyAsync(&s, cd.sc, sizeof(s), hipMemcpyDeviceToHost, st));
+  o.alpha.assign(nmax, 0.0);
+  o.beta.assign(nmax, 0.0);
+  EDL_HIP(hipMemcpyAsync(o.alpha.data(), cd.alpha, sizeof(double) * nmax, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(o.beta.data(), cd.beta, sizeof(double) * nmax, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  o.nsteps = s.nsteps;
+  o.norm2 = s.norm2;
+  return o;
+}
+
+static double lowest_ritz(const ChainOut& o, std::vector<double>* vec) {
+  const int m = o.nsteps;
+  EDL_CHECK(m > 0, "Lanczos produced no step");
+  std::vector<double> e(m, 0.0), w(m), Z;
+  for (int i = 0; i + 1 < m; ++i) e[i] = o.beta[i + 1];
+  if (vec) Z.resize((size_t)m * m);
+  tridiag_eig(m, o.alpha.data(), e.data(), w.data(), vec ? Z.data() : nullptr, nullptr);
+  if (vec) vec->assign(Z.begin(), Z.begin() + m);
+  return w[0];
+}
+
+struct GsRun {
+  double e0 = 0.0;
+  ChainOut out;
+  int64_t hxv = 0;
+};
+
+// One ground-state Lanczos run from d_start; the normalised Ritz vector of
+// the lowest eigenvalue is rebuilt into d_x (fp64) by replaying the same
+// deterministic recurrence with the tridiagonal eigenvector as weights.
+template <typename T>
+static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double tol, int check, double* d_x) {
+  hipStream_t st = c.stream;
+  const int nmax = (int)std::min<int64_t>(niter, S.dim);
+  GsRun r;
+  Schedule sch;
+  {
+    ChainSet<T> cs;
+    ChainAlloc<T> A;
+    ChainDev cd{};
+    init_chain<T>(c, cd, A, S, d_start, 1.0, nmax, st);
+    sch = autotune<T>(c, {cd}, st);
+    cs.h.push_back(cd);
+    chainset_finalize(c, cs, sch);
+    r.hxv += run_steps(c, cs, nmax, check, tol, st);
+    r.out = read_chain<T>(cs.h[0], nmax, st);
+  }
+  std::vector<double> cvec;
+  r.e0 = lowest_ritz(r.out, d_x ? &cvec : nullptr);
+  if (d_x) {
+    const int m = r.out.nsteps;
+    DevBuf dc(sizeof(double) * m);
+    EDL_HIP(hipMemcpyAsync(dc.p, cvec.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+    launch_zero_bytes(d_x, sizeof(double) * S.dim, st);
+    ChainSet<T> cs;
+    ChainAlloc<T> A;
+    ChainDev cd{};
+    init_chain<T>(c, cd, A, S, d_start, 1.0, m, st);
+    cd.x = d_x;
+    cd.ccoef = dc.as<double>();
+    cs.h.push_back(cd);
+    chainset_finalize(c, cs, sch);
+    r.hxv += run_steps(c, cs, m, 0, 0.0, st);
+    const double n2 = device_dot(c, d_x, d_x, S.dim, st);
+    launch_scale(d_x, S.dim, 1.0 / std::sqrt(n2), st);
+    EDL_HIP(hipStreamSynchronize(st));
+  }
+  return r;
+}
+
+struct GsStats {
+  int steps_lo = 0, steps_hi = 0;
+  double e0_lo = 0.0;
+};
+static std::map<const Ctx*, GsStats>& gs_stats() {
+  static std::map<const Ctx*, GsStats> m;
+  return m;
+}
+
+static void install_gs(Ctx& c, int isector, int64_t dim) {
+  if (c.d_gs && c.gs_dim != dim) {
+    (void)hipFree(c.d_gs);
+    c.d_gs = nullptr;
+  }
+  if (!c.d_gs) EDL_HIP(hipMalloc(&c.d_gs, sizeof(double) * dim));
+  c.gs_dim = dim;
+  c.gs_sector = isector;
+}
+
+}  // namespace edl
+
+using namespace edl;
+
+// ------------------------------------------------------------------ C-ABI
+#define API_BEGIN try {
+#define API_END                                          \
+  }                                                      \
+  catch (const std::exception& e) {                      \
+    if (ctx) ctx->c.last_error = e.what();               \
+    return 1;                                            \
+  }                                                      \
+  return 0;
+
+static thread_local std::string g_create_error;
+
+extern "C" int edl_ctx_create(edl_ctx** out, int device) {
+  try {
+    EDL_CHECK(out != nullptr, "edl_ctx_create: null output pointer");
+    int n = 0;
+    EDL_HIP(hipGetDeviceCount(&n));
+    EDL_CHECK(n > 0, "edl_ctx_create: no HIP device visible");
+    EDL_CHECK(device >= 0 && device < n, "edl_ctx_create: device index out of range");
+    EDL_HIP(hipSetDevice(device));
+    auto* ctx = new edl_ctx();
+    ctx->c.device = device;
+    EDL_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop{};
+    EDL_HIP(hipGetDeviceProperties(&prop, device));
+    ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    init_binomials();
+    *out = ctx;
+    return 0;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    if (out) *out = nullptr;
+    return 1;
+  }
+}
+
+extern "C" int edl_ctx_destroy(edl_ctx* ctx) {
+  if (!ctx) return 0;
+  API_BEGIN
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  release_sectors(ctx->c);
+  if (ctx->c.d_gs) (void)hipFree(ctx->c.d_gs);
+  if (ctx->c.d_scratch) (void)hipFree(ctx->c.d_scratch);
+  for (void* p : ctx->c.user_bufs) (void)hipFree(p);
+  (void)hipStreamDestroy(ctx->c.stream);
+  gs_stats().erase(&ctx->c);
+  delete ctx;
+  return 0;
+  API_END
+}
+
+extern "C" const char* edl_last_error(const edl_ctx* ctx) {
+  return ctx ? ctx->c.last_error.c_str() : g_create_error.c_str();
+}
+
+extern "C" int edl_set_hamiltonian(edl_ctx* ctx, int Ns, int Nspin, int Norb, const int* Nsites, const double* Hre,
+                                   const double* Him, const double* Uloc, double Ust, double Jh, double Jx, double Jp,
+                                   double xmu, int hfmode, int ed_filling) {
+  API_BEGIN
+  EDL_CHECK(ctx && Nsites && Hre && Uloc, "edl_set_hamiltonian: null argument");
+  EDL_CHECK(Ns >= 1 && Ns <= kMaxNs, "edl_set_hamiltonian: Ns must be in [1, 31]");
+  EDL_CHECK(Nspin == 1 || Nspin == 2, "edl_set_hamiltonian: Nspin must be 1 or 2");
+  EDL_CHECK(Norb >= 1 && Norb <= kMaxOrb, "edl_set_hamiltonian: Norb out of range");
+  int sum = 0;
+  for (int k = 0; k < Norb; ++k) sum += Nsites[k];
+  EDL_CHECK(sum == Ns, "edl_set_hamiltonian: Ns != sum(Nsites)");
+  EDL_CHECK(!(Norb > 1 && (Jx != 0.0 || Jp != 0.0)),
+            "edl_set_hamiltonian: spin-exchange / pair-hopping (Jhflag) is not supported");
+  const size_t n = (size_t)Nspin * Ns * Ns;
+  for (size_t k = 0; k < n; ++k)
+    EDL_CHECK(!Him || Him[k] == 0.0, "edl_set_hamiltonian: complex Hij is not supported (real kernels)");
+  // Hermiticity (GRAPH_MATRIX.f90:324-326, tolerance 1e-6)
+  for (int s = 0; s < Nspin; ++s)
+    for (int i = 0; i < Ns; ++i)
+      for (int j = 0; j < Ns; ++j) {
+        const double a = Hre[s + (size_t)Nspin * (i + (size_t)Ns * j)];
+        const double b = Hre[s + (size_t)Nspin * (j + (size_t)Ns * i)];
+        EDL_CHECK(std::fabs(a - b) <= 1e-6, "Hij_build ERROR: Hij is not Hermitian");
+      }
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  release_sectors(ctx->c);
+  Model& M = ctx->c.model;
+  M.Ns = Ns;
+  M.Nspin = Nspin;
+  M.Norb = Norb;
+  M.Nsites.assign(Nsites, Nsites + Norb);
+  M.Hij.assign(Hre, Hre + n);
+  M.Uloc.assign(Uloc, Uloc + Norb);
+  M.Ust = Ust;
+  M.Jh = Jh;
+  M.Jx = Jx;
+  M.Jp = Jp;
+  M.xmu = xmu;
+  M.hfmode = hfmode ? 1 : 0;
+  M.ed_filling = ed_filling;
+  InterParams I{};
+  // U n_up n_dw per orbital (HxV_local.f90:23-30), only when any Uloc /= 0
+  bool anyU = false;
+  for (int k = 0; k < Norb; ++k) anyU = anyU || Uloc[k] != 0.0;
+  I.norb = anyU ? Norb : 1;
+  for (int k = 0; k < Norb && anyU; ++k) {
+    uint32_t mask = 0;
+    for (int isite = 1; isite <= Nsites[k]; ++isite) {
+      const int io = (k == 0) ? isite : isite + k * Nsites[k - 1];  // pack_indices, AUX_FUNX.f90:42-53
+      EDL_CHECK(io >= 1 && io <= Ns, "pack_indices out of range");
+      mask |= 1u << (io - 1);
+    }
+    I.orb_mask[k] = mask;
+    I.U[k] = Uloc[k];
+  }
+  // same-site inter-orbital pairs (HxV_local.f90:34-45)
+  int np = 0;
+  for (int a = 0; a < Norb; ++a)
+    for (int b = a + 1; b < Norb; ++b)
+      for (int isite = 1; isite <= std::min(Nsites[a], Nsites[b]); ++isite) {
+        const int io = (a == 0) ? isite : isite + a * Nsites[a - 1];
+        const int jo = (b == 0) ? isite : isite + b * Nsites[b - 1];
+        EDL_CHECK(np < kMaxPairs, "too many inter-orbital pairs");
+        EDL_CHECK(io >= 1 && io <= Ns && jo >= 1 && jo <= Ns, "pack_indices out of range");
+        I.pi[np] = (uint8_t)(io - 1);
+        I.pj[np] = (uint8_t)(jo - 1);
+        ++np;
+      }
+  I.npairs = np;
+  I.Ust = Ust;
+  M.inter = I;
+  M.set = true;
+  return 0;
+  API_END
+}
+
+extern "C" int edl_sector_index(int Ns, int nup, int ndw) {
+  if (nup < 0 || ndw < 0 || nup > Ns || ndw > Ns) return 0;
+  return 1 + nup * (Ns + 1) + ndw;  // get_Sector, SECTOR.f90:370-381
+}
+
+extern "C" int edl_sector_qn(int Ns, int isector, int* nup, int* ndw) {
+  if (isector < 1 || isector > (Ns + 1) * (Ns + 1) || !nup || !ndw) return 1;
+  *ndw = (isector - 1) % (Ns + 1);
+  *nup = ((isector - 1) / (Ns + 1)) % (Ns + 1);
+  return 0;
+}
+
+extern "C" int edl_sector_build(edl_ctx* ctx, int isector, int64_t* dim, int32_t* dimup, int32_t* dimdw) {
+  API_BEGIN
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  if (dim) *dim = s.dim;
+  if (dimup) *dimup = s.sp[0].dim;
+  if (dimdw) *dimdw = s.sp[1].dim;
+  API_END
+}
+
+extern "C" int edl_sector_maps(edl_ctx* ctx, int isector, int32_t* map_up, int32_t* map_dw) {
+  API_BEGIN
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  if (map_up)
+    EDL_HIP(hipMemcpyAsync(map_up, s.sp[0].map, sizeof(int32_t) * s.sp[0].dim, hipMemcpyDeviceToHost, ctx->c.stream));
+  if (map_dw)
+    EDL_HIP(hipMemcpyAsync(map_dw, s.sp[1].map, sizeof(int32_t) * s.sp[1].dim, hipMemcpyDeviceToHost, ctx->c.stream));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+extern "C" int edl_sector_hops(edl_ctx* ctx, int isector, int ispin, int32_t* kmax, int32_t* nbr, double* coef) {
+  API_BEGIN
+  EDL_CHECK(ispin == 1 || ispin == 2, "edl_sector_hops: ispin must be 1 or 2");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  const SpinTable& t = s.sp[ispin - 1];
+  if (kmax) *kmax = t.kmax;
+  const size_t n = (size_t)t.kmax * t.dim;
+  if (nbr && n) EDL_HIP(hipMemcpyAsync(nbr, t.nbr, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+  if (coef && n) EDL_HIP(hipMemcpyAsync(coef, t.coef64, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+extern "C" int edl_sector_rank(edl_ctx* ctx, int isector, int ispin, int64_t n, const int32_t* states, int32_t* pos) {
+  API_BEGIN
+  EDL_CHECK(ispin == 1 || ispin == 2, "edl_sector_rank: ispin must be 1 or 2");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  if (n <= 0) return 0;
+  DevBuf ds(sizeof(int32_t) * n), dp(sizeof(int32_t) * n);
+  EDL_HIP(hipMemcpyAsync(ds.p, states, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->c.stream));
+  rank_strings(ctx->c, s.sp[ispin - 1], ctx->c.model.Ns, n, ds.as<int32_t>(), dp.as<int32_t>());
+  EDL_HIP(hipMemcpyAsync(pos, dp.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+static Schedule operator_schedule(Ctx& c) {
+  Schedule w;
+  w.block_threads = 256;
+  w.rows_per_thread = 1;
+  w.grid = c.num_cu * 8;
+  return w;
+}
+
+extern "C" int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v, void* hv) {
+  API_BEGIN
+  EDL_CHECK(v && hv, "edl_hxv: null vector");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_hxv: prec must be EDL_FP64 or EDL_FP32");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  EventTimer tm(ctx->c.stream);
+  if (prec == EDL_FP64)
+    hxv_operator<double>(ctx->c, s, (const double*)v, (double*)hv, ctx->c.stream, operator_schedule(ctx->c));
+  else
+    hxv_operator<float>(ctx->c, s, (const float*)v, (float*)hv, ctx->c.stream, operator_schedule(ctx->c));
+  ctx->c.last_ms = tm.stop();
+  ctx->c.last_hxv = 1;
+  API_END
+}
+
+template <typename T>
+static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_t stride) {
+  const int64_t D = s.dim;
+  hipStream_t st = c.stream;
+  std::vector<double> tmp(D);
+  for (int64_t i = 0; i < D; ++i) tmp[i] = v[i * stride];
+  DevBuf d64(sizeof(double) * D), dv(sizeof(T) * D), dh(sizeof(T) * D);
+  EDL_HIP(hipMemcpyAsync(d64.p, tmp.data(), sizeof(double) * D, hipMemcpyHostToDevice, st));
+  launch_convert<T>(d64.as<double>(), dv.as<T>(), D, st);
+  hxv_operator<T>(c, s, dv.as<T>(), dh.as<T>(), st, operator_schedule(c));
+  launch_to_fp64<T>(dh.as<T>(), d64.as<double>(), D, st);
+  EDL_HIP(hipMemcpyAsync(tmp.data(), d64.p, sizeof(double) * D, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  for (int64_t i = 0; i < D; ++i) hv[i * stride] = tmp[i];
+}
+
+extern "C" int edl_hxv_host(edl_ctx* ctx, int isector, int prec, const double* v, double* hv) {
+  API_BEGIN
+  EDL_CHECK(v && hv, "edl_hxv_host: null vector");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_hxv_host: prec must be EDL_FP64 or EDL_FP32");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  if (prec == EDL_FP64) hxv_host_impl<double>(ctx->c, s, v, hv, 1);
+  else hxv_host_impl<float>(ctx->c, s, v, hv, 1);
+  API_END
+}
+
+extern "C" int edl_hxv_host_complex(edl_ctx* ctx, int isector, int prec, const double* v, double* hv) {
+  API_BEGIN
+  EDL_CHECK(v && hv, "edl_hxv_host_complex: null vector");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_hxv_host_complex: prec must be EDL_FP64 or EDL_FP32");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  Sector& s = get_sector(ctx->c, isector);
+  for (int part = 0; part < 2; ++part) {
+    if (prec == EDL_FP64) hxv_host_impl<double>(ctx->c, s, v + part, hv + part, 2);
+    else hxv_host_impl<float>(ctx->c, s, v + part, hv + part, 2);
+  }
+  API_END
+}
+
+extern "C" int edl_device_alloc(edl_ctx* ctx, int64_t bytes, void** dev_ptr) {
+  API_BEGIN
+  EDL_CHECK(dev_ptr && bytes > 0, "edl_device_alloc: bad arguments");
+  EDL_HIP(hipSetDevice(ctx->c.device));
+  void* p = nullptr;
+  EDL_HIP(hipMalloc(&p, (size_t)bytes));
+  ctx->c.user_bufs.push_back(p);
+  *dev_ptr = p;
+  API_END
+}
+
+extern "C" int edl_copy_h2d(edl_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  API_BEGIN
+  EDL_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->c.stream));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+extern "C" int edl_copy_d2h(edl_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  API_BEGIN
+  EDL_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+  EDL_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+extern "C" int edl_lanczos_gs(edl_ctx* ctx, int isector, int prec, int niter, double tol, uint64_t seed,
+                              double refine_tol, double* e0, int* nsteps, double* alpha, double* beta,
+                              double* gs_host) {
+  API_BEGIN
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32 || prec == EDL_MIXED, "edl_lanczos_gs: bad prec");
+  EDL_CHECK(niter >= 1, "edl_lanczos_gs: niter must be >= 1");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = get_sector(c, isector);
+  const int64_t D = S.dim;
+  hipStream_t st = c.stream;
+  DevBuf start(sizeof(double) * D);
+  launch_start_vector(start.as<double>(), D, seed, st);
+  install_gs(c, isector, D);
+  GsStats stats;
+  EventTimer tm(st);
+  GsRun r;
+  int64_t hxv = 0;
+  if (prec == EDL_FP64) {
+    r = gs_run<double>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    stats.steps_hi = r.out.nsteps;
+  } else if (prec == EDL_FP32) {
+    r = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    stats.steps_lo = r.out.nsteps;
+  } else {
+    GsRun lo = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    stats.steps_lo = lo.out.nsteps;
+    stats.e0_lo = lo.e0;
+    hxv += lo.hxv;
+    // FP64 restart from the FP32 Ritz vector (now in c.d_gs)
+    EDL_HIP(hipMemcpyAsync(start.p, c.d_gs, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
+    const double rt = refine_tol > 0.0 ? refine_tol : 1e-13;
+    r = gs_run<double>(c, S, start.as<double>(), niter, rt, 5, c.d_gs);
+    stats.steps_hi = r.out.nsteps;
+  }
+  hxv += r.hxv;
+  c.last_ms = tm.stop();
+  c.last_hxv = hxv;
+  c.gs_energy = r.e0;
+  gs_stats()[&c] = stats;
+  if (e0) *e0 = r.e0;
+  if (nsteps) *nsteps = r.out.nsteps;
+  const int n = std::min<int>(niter, (int)r.out.alpha.size());
+  if (alpha) {
+    for (int k = 0; k < niter; ++k) alpha[k] = k < n ? r.out.alpha[k] : 0.0;
+  }
+  if (beta) {
+    for (int k = 0; k < niter; ++k) beta[k] = k < n ? r.out.beta[k] : 0.0;
+  }
+  if (gs_host) {
+    EDL_HIP(hipMemcpyAsync(gs_host, c.d_gs, sizeof(double) * D, hipMemcpyDeviceToHost, st));
+    EDL_HIP(hipStreamSynchronize(st));
+  }
+  API_END
+}
+
+extern "C" int edl_lanczos_stats(edl_ctx* ctx, int* steps_lo, int* steps_hi, double* e0_lo) {
+  API_BEGIN
+  const GsStats s = gs_stats()[&ctx->c];
+  if (steps_lo) *steps_lo = s.steps_lo;
+  if (steps_hi) *steps_hi = s.steps_hi;
+  if (e0_lo) *e0_lo = s.e0_lo;
+  API_END
+}
+
+extern "C" int edl_set_state(edl_ctx* ctx, int isector, double energy, const double* vec) {
+  API_BEGIN
+  EDL_CHECK(vec, "edl_set_state: null vector");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = get_sector(c, isector);
+  install_gs(c, isector, S.dim);
+  c.gs_energy = energy;
+  EDL_HIP(hipMemcpyAsync(c.d_gs, vec, sizeof(double) * S.dim, hipMemcpyHostToDevice, c.stream));
+  EDL_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+extern "C" int edl_get_state(edl_ctx* ctx, double* vec) {
+  API_BEGIN
+  Ctx& c = ctx->c;
+  EDL_CHECK(c.d_gs, "edl_get_state: no state installed");
+  EDL_HIP(hipMemcpyAsync(vec, c.d_gs, sizeof(double) * c.gs_dim, hipMemcpyDeviceToHost, c.stream));
+  EDL_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+// target sector of a seed (getCsector / getCDGsector, SETUP.f90:265-310)
+static int target_sector(const Ctx& c, int isector, const SeedSpec& s) {
+  const int Ns = c.model.Ns;
+  const int ndw = (isector - 1) % (Ns + 1), nup = ((isector - 1) / (Ns + 1)) % (Ns + 1);
+  if (s.op == EDL_OP_SZ) return isector;
+  const int d = (s.op == EDL_OP_CDG) ? +1 : -1;
+  int u = nup, w = ndw;
+  if (s.ispin == 1) u += d; else w += d;
+  if (u < 0 || w < 0 || u > Ns || w > Ns) return 0;
+  return 1 + u * (Ns + 1) + w;
+}
+
+template <typename T>
+static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const std::vector<int>& idx,
+                             const std::vector<int>& targets, int nlanc, double* alpha, double* beta, double* norm2,
+                             int* nsteps, const double* host_seed) {
+  hipStream_t st = c.stream;
+  std::vector<ChainAlloc<T>> allocs(idx.size());
+  std::vector<ChainDev> chains;
+  std::vector<int> nmaxs;
+  std::vector<int> which;
+  int64_t maxdim = 0;
+  for (size_t k = 0; k < idx.size(); ++k) maxdim = std::max<int64_t>(maxdim, get_sector(c, targets[idx[k]]).dim);
+  DevBuf tmp(sizeof(double) * maxdim);
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const int s = idx[k];
+    Sector& J = get_sector(c, targets[s]);
+    if (host_seed) {
+      EDL_HIP(hipMemcpyAsync(tmp.p, host_seed, sizeof(double) * J.dim, hipMemcpyHostToDevice, st));
+    } else {
+      const Sector& I = get_sector(c, c.gs_sector);
+      launch_seed(seeds[s], c.d_gs, I.sp[0].dim, J, tmp.as<double>(), st);
+    }
+    const double n2 = device_dot(c, tmp.as<double>(), tmp.as<double>(), J.dim, st);
+    const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
+    ChainDev cd{};
+    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), n2, nmax, st);
+    EDL_HIP(hipStreamSynchronize(st));  // tmp is reused by the next seed
+    norm2[s] = n2;
+    chains.push_back(cd);
+    nmaxs.push_back(nmax);
+    which.push_back(s);
+  }
+  if (chains.empty()) return 0;
+  Schedule sch = autotune<T>(c, chains, st);
+  ChainSet<T> cs;
+  cs.h = chains;
+  chainset_finalize(c, cs, sch);
+  const int steps = *std::max_element(nmaxs.begin(), nmaxs.end());
+  run_steps(c, cs, steps, 0, 0.0, st);
+  int64_t hxv = 0;
+  for (size_t k = 0; k < chains.size(); ++k) {
+    ChainOut o = read_chain<T>(cs.h[k], nmaxs[k], st);
+    const int s = which[k];
+    for (int j = 0; j < nlanc; ++j) {
+      alpha[(size_t)s * nlanc + j] = j < nmaxs[k] ? o.alpha[j] : 0.0;
+      beta[(size_t)s * nlanc + j] = j < nmaxs[k] ? o.beta[j] : 0.0;
+    }
+    if (nsteps) nsteps[s] = o.nsteps;
+    hxv += o.nsteps;
+  }
+  return hxv;
+}
+
+extern "C" int edl_tridiag(edl_ctx* ctx, int isector, int prec, const double* seed, int nlanc, double* alpha,
+                           double* beta, double* norm2, int* nsteps) {
+  API_BEGIN
+  EDL_CHECK(seed && alpha && beta && norm2, "edl_tridiag: null argument");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_tridiag: prec must be EDL_FP64 or EDL_FP32");
+  EDL_CHECK(nlanc >= 1, "edl_tridiag: nlanc must be >= 1");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  get_sector(c, isector);
+  std::vector<SeedSpec> seeds(1, SeedSpec{0, 0, 0, 0});
+  std::vector<int> idx(1, 0), tg(1, isector);
+  int ns = 0;
+  EventTimer tm(c.stream);
+  int64_t h;
+  if (prec == EDL_FP64) h = tridiag_group<double>(c, seeds, idx, tg, nlanc, alpha, beta, norm2, &ns, seed);
+  else h = tridiag_group<float>(c, seeds, idx, tg, nlanc, alpha, beta, norm2, &ns, seed);
+  c.last_ms = tm.stop();
+  c.last_hxv = h;
+  if (nsteps) *nsteps = ns;
+  API_END
+}
+
+extern "C" int edl_tridiag_batched(edl_ctx* ctx, int nseeds, const edl_seed_desc* seeds, int nlanc, int prec,
+                                   int group, double* alpha, double* beta, double* norm2, int* nsteps,
+                                   int* isector_out) {
+  API_BEGIN
+  EDL_CHECK(nseeds >= 0 && (nseeds == 0 || seeds), "edl_tridiag_batched: bad seed list");
+  EDL_CHECK(alpha && beta && norm2, "edl_tridiag_batched: null output");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_tridiag_batched: prec must be EDL_FP64 or EDL_FP32");
+  EDL_CHECK(nlanc >= 1, "edl_tridiag_batched: nlanc must be >= 1");
+  Ctx& c = ctx->c;
+  EDL_CHECK(c.d_gs, "edl_tridiag_batched: no ground state installed");
+  EDL_HIP(hipSetDevice(c.device));
+  const int Ns = c.model.Ns;
+  std::vector<SeedSpec> sp(nseeds);
+  std::vector<int> targets(nseeds, 0), active;
+  for (int s = 0; s < nseeds; ++s) {
+    sp[s] = SeedSpec{seeds[s].op, seeds[s].ispin, seeds[s].io, seeds[s].jo};
+    EDL_CHECK(sp[s].op >= EDL_OP_C && sp[s].op <= EDL_OP_SZ, "edl_tridiag_batched: bad op");
+    EDL_CHECK(sp[s].op == EDL_OP_SZ || sp[s].ispin == 1 || sp[s].ispin == 2, "edl_tridiag_batched: bad ispin");
+    EDL_CHECK(sp[s].io >= 1 && sp[s].io <= Ns && sp[s].jo >= 0 && sp[s].jo <= Ns,
+              "edl_tridiag_batched: site index out of range");
+    targets[s] = target_sector(c, c.gs_sector, sp[s]);
+    if (isector_out) isector_out[s] = targets[s];
+    for (int j = 0; j < nlanc; ++j) alpha[(size_t)s * nlanc + j] = beta[(size_t)s * nlanc + j] = 0.0;
+    norm2[s] = 0.0;
+    if (nsteps) nsteps[s] = 0;
+    if (targets[s] != 0) active.push_back(s);
+  }
+  for (int s : active) get_sector(c, targets[s]);
+  const int g = (group <= 0) ? (int)active.size() : group;
+  EventTimer tm(c.stream);
+  int64_t h = 0;
+  for (size_t b = 0; b < active.size(); b += g) {
+    std::vector<int> idx(active.begin() + b, active.begin() + std::min(active.size(), b + g));
+    if (prec == EDL_FP64)
+      h += tridiag_group<double>(c, sp, idx, targets, nlanc, alpha, beta, norm2, nsteps, nullptr);
+    else
+      h += tridiag_group<float>(c, sp, idx, targets, nlanc, alpha, beta, norm2, nsteps, nullptr);
+  }
+  c.last_ms = tm.stop();
+  c.last_hxv = h;
+  API_END
+}
+
+extern "C" int edl_lanczos_poles(int nlanc, const double* alpha, const double* beta, double norm2, double Ei,
+                                 int isign, double* poles, double* weights) {
+  try {
+    EDL_CHECK(nlanc >= 1 && alpha && beta && poles && weights, "edl_lanczos_poles: bad arguments");
+    std::vector<double> e(nlanc, 0.0), w(nlanc), z1(nlanc);
+    for (int i = 0; i + 1 < nlanc; ++i) e[i] = beta[i + 1];  // subdiag(2:Nlanc) = blanc(2:Nlanc)
+    tridiag_eig(nlanc, alpha, e.data(), w.data(), nullptr, z1.data());
+    for (int j = 0; j < nlanc; ++j) {
+      weights[j] = z1[j] * z1[j] * norm2;
+      poles[j] = isign * (w[j] - Ei);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return 1;
+  }
+}
+
+extern "C" int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, double* docc) {
+  API_BEGIN
+  Ctx& c = ctx->c;
+  EDL_CHECK(c.d_gs, "edl_observables: no ground state installed");
+  EDL_CHECK(dens_up && dens_dw && docc, "edl_observables: null output");
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = get_sector(c, c.gs_sector);
+  device_observables(c, S, c.d_gs, dens_up, dens_dw, docc);
+  API_END
+}
+
+extern "C" int edl_last_timing(edl_ctx* ctx, double* ms, int64_t* hxv_count) {
+  API_BEGIN
+  if (ms) *ms = ctx->c.last_ms;
+  if (hxv_count) *hxv_count = ctx->c.last_hxv;
+  API_END
+}
+
+template <typename T>
+static double bench_impl(Ctx& c, Sector& s, int reps) {
+  hipStream_t st = c.stream;
+  const int64_t D = s.dim;
+  DevBuf d64(sizeof(double) * D), v(sizeof(T) * D), h(sizeof(T) * D);
+  launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
+  launch_convert<T>(d64.as<double>(), v.as<T>(), D, st);
+  const Schedule w = operator_schedule(c);
+  hxv_operator<T>(c, s, v.as<T>(), h.as<T>(), st, w);
+  EventTimer tm(st);
+  for (int r = 0; r < reps; ++r) hxv_operator<T>(c, s, v.as<T>(), h.as<T>(), st, w);
+  return tm.stop() / reps;
+}
+
+extern "C" int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_hxv) {
+  API_BEGIN
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_bench_hxv: prec must be EDL_FP64 or EDL_FP32");
+  EDL_CHECK(reps >= 1 && ms_per_hxv, "edl_bench_hxv: bad arguments");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& s = get_sector(c, isector);
+  *ms_per_hxv = (prec == EDL_FP64) ? bench_impl<double>(c, s, reps) : bench_impl<float>(c, s, reps);
+  c.last_ms = *ms_per_hxv * reps;
+  c.last_hxv = reps;
+  API_END
+}

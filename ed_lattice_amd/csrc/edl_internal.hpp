@@ -1,0 +1,126 @@
+// Internal declarations shared by the HIP translation units of libedlat_hip.
+// The public C-ABI is include/edlat_gpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace edl {
+
+constexpr int kMaxNs = 31;      // spin strings are held in 32-bit words
+constexpr int kMaxOrb = 8;
+constexpr int kMaxPairs = 64;   // same-site inter-orbital pairs (Ust/Jh terms)
+constexpr int kBlock = 256;     // threads per block of the streaming kernels
+constexpr int kRedBlocks = 2048;  // persistent grid of the reduction kernels
+
+struct Error : std::exception {
+  std::string msg;
+  explicit Error(std::string m) : msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+
+#define EDL_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::edl::Error(std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define EDL_CHECK(cond, text)                 \
+  do {                                        \
+    if (!(cond)) throw ::edl::Error(text);    \
+  } while (0)
+
+// Parameters of the diagonal that are evaluated per element: the density-
+// density interaction between the up and the down string of one basis state.
+struct InterParams {
+  int norb;                  // number of orbital masks
+  uint32_t orb_mask[kMaxOrb];  // sites of orbital k
+  double U[kMaxOrb];         // Uloc(k)
+  int npairs;                // same-site, different-orbital pairs (io, jo)
+  uint8_t pi[kMaxPairs], pj[kMaxPairs];  // 0-based positions
+  double Ust;
+};
+
+// One spin species of one sector: sorted strings, hopping table, diagonal.
+struct SpinTable {
+  int N = 0;           // electrons of this spin
+  int dim = 0;         // C(Ns, N)
+  int kmax = 0;        // ELL width of the hopping table
+  int32_t* map = nullptr;      // [dim]   ascending strings
+  int32_t* nbr = nullptr;      // [kmax][dim] neighbour index, 0-based
+  double* coef64 = nullptr;    // [kmax][dim] Hij(io,jo) * fermionic sign
+  float* coef32 = nullptr;     // same, rounded to fp32
+  double* diag = nullptr;      // [dim] one-spin part of the diagonal
+};
+
+struct Sector {
+  int isector = 0, Nup = 0, Ndw = 0;
+  int64_t dim = 0;
+  SpinTable sp[2];
+};
+
+// Kernel-side view of a sector, passed by value.
+struct SectorView {
+  int dimup, dimdw;
+  int kup, kdw;
+  const int32_t* map_up;
+  const int32_t* map_dw;
+  const int32_t* nbr_up;
+  const int32_t* nbr_dw;
+  const void* coef_up;   // double* or float* as the kernel's scalar type
+  const void* coef_dw;
+  const double* dg_up;
+  const double* dg_dw;
+};
+
+struct Model {
+  bool set = false;
+  int Ns = 0, Nspin = 1, Norb = 1;
+  std::vector<int> Nsites;
+  std::vector<double> Hij;   // real Hij(Nspin,Ns,Ns), column-major
+  std::vector<double> Uloc;
+  double Ust = 0, Jh = 0, Jx = 0, Jp = 0, xmu = 0;
+  int hfmode = 1, ed_filling = 0;
+  InterParams inter{};
+};
+
+struct Ctx;
+
+// sector.hip
+void build_sector_tables(Ctx& c, Sector& s);
+void free_sector(Sector& s);
+void rank_strings(Ctx& c, const SpinTable& t, int Ns, int64_t n, const int32_t* d_states, int32_t* d_out);
+void init_binomials();
+
+// hxv.hip / lanczos.hip launchers are declared in their own headers.
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  Model model;
+  std::map<int, std::unique_ptr<Sector>> sectors;
+  int num_cu = 256;
+  // ground state kept on the device after edl_lanczos_gs / edl_set_state
+  int gs_sector = 0;
+  double gs_energy = 0.0;
+  double* d_gs = nullptr;   // fp64 [dim]
+  int64_t gs_dim = 0;
+  // scratch owned by the context, grown on demand
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // timing of the last operator / Lanczos call (HIP events on `stream`)
+  double last_ms = 0.0;
+  int64_t last_hxv = 0;
+  // buffers handed out by edl_device_alloc
+  std::vector<void*> user_bufs;
+};
+
+Sector& get_sector(Ctx& c, int isector);
+
+}  // namespace edl

@@ -1,0 +1,169 @@
+// Seed vectors of the GF and chi continued-fraction runs, built on the
+// device from the ground state, and ground-state observables.
+//
+// Reference seed builders, all in scatter form on the host:
+//   c^dg_io |gs>, c_io |gs>          GF_ELECTRONS.f90:241-245, 267-271
+//   (c^dg_io + c^dg_jo)|gs>, (c_io + c_jo)|gs>   GF_ELECTRONS.f90:343-353, 376-386
+//   S^z_io |gs>, (S^z_io + S^z_jo)|gs>           CHI_SPIN_ELECTRONS.f90:216-220, 275-280
+// with apply_op_C / apply_op_CDG / apply_op_Sz of SECTOR.f90:217-318.
+// Here each element j of the target sector gathers its unique source
+// element of the ground-state sector: for c^dg_p the source string is the
+// target string with bit p cleared, for c_p the target with bit p set; the
+// sign (-1)^popcount(bits below p) is the c/cdg sign of AUX_FUNX.f90:76-102
+// (bits below p agree between source and target).  Up and down operators
+// commute: no sign from the other spin (SECTOR.f90:233-240).
+#include "chains.hpp"
+#include "seeds.hpp"
+
+namespace edl {
+
+struct SeedTerm {
+  int op;      // 1 = C, 2 = CDG, 3 = SZ
+  int ispin;   // 1 up, 2 down (C, CDG)
+  int pos;     // 0-based site
+};
+
+// One term of a seed at target element (jup, jdw); returns coefficient * gs.
+static __device__ __forceinline__ double seed_term(const SeedTerm& T, const double* __restrict__ gs, int gs_dimup,
+                                                   uint32_t tu, uint32_t td, int jup, int jdw) {
+  if (T.op == 3) {
+    const double sz = 0.5 * ((double)((tu >> T.pos) & 1u) - (double)((td >> T.pos) & 1u));
+    return sz * gs[(size_t)jdw * gs_dimup + jup];
+  }
+  const uint32_t t = (T.ispin == 1) ? tu : td;
+  const uint32_t bit = 1u << T.pos;
+  uint32_t src;
+  if (T.op == 2) {  // c^dg_p: target must hold the electron
+    if (!(t & bit)) return 0.0;
+    src = t & ~bit;
+  } else {          // c_p: target must miss it
+    if (t & bit) return 0.0;
+    src = t | bit;
+  }
+  const double sg = (__popc(t & (bit - 1u)) & 1) ? -1.0 : 1.0;
+  const int32_t r = rank_of(src);
+  const size_t o = (T.ispin == 1) ? (size_t)jdw * gs_dimup + (size_t)r : (size_t)r * gs_dimup + (size_t)jup;
+  return sg * gs[o];
+}
+
+__global__ void k_seed(const double* __restrict__ gs, int gs_dimup, const int32_t* __restrict__ map_up,
+                       const int32_t* __restrict__ map_dw, int dimup, int64_t dim, SeedTerm a, SeedTerm b, int nterms,
+                       double* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += (int64_t)gridDim.x * blockDim.x) {
+    const int jup = (int)(j % dimup), jdw = (int)(j / dimup);
+    const uint32_t tu = (uint32_t)map_up[jup], td = (uint32_t)map_dw[jdw];
+    double v = seed_term(a, gs, gs_dimup, tu, td, jup, jdw);
+    if (nterms > 1) v += seed_term(b, gs, gs_dimup, tu, td, jup, jdw);
+    out[j] = v;
+  }
+}
+
+// Per-site double occupancy sum_i |gs(i)|^2 n_up(p) n_dw(p), and the row /
+// column sums of |gs|^2 from which the densities follow.
+template <int NS>
+__global__ void k_docc(const double* __restrict__ gs, const int32_t* __restrict__ map_up,
+                       const int32_t* __restrict__ map_dw, int dimup, int64_t dim, double* __restrict__ part) {
+  __shared__ double sh[NS][16];
+  double acc[NS];
+#pragma unroll
+  for (int p = 0; p < NS; ++p) acc[p] = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += (int64_t)gridDim.x * blockDim.x) {
+    const int jup = (int)(j % dimup), jdw = (int)(j / dimup);
+    const uint32_t both = (uint32_t)map_up[jup] & (uint32_t)map_dw[jdw];
+    const double w = gs[j] * gs[j];
+#pragma unroll
+    for (int p = 0; p < NS; ++p) acc[p] += (double)((both >> p) & 1u) * w;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int p = 0; p < NS; ++p) {
+    double v = acc[p];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[p][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += sh[threadIdx.x][w];
+    part[(size_t)blockIdx.x * NS + threadIdx.x] = s;
+  }
+}
+
+// sum over columns (rowsum[iup]) and over rows (colsum[idw]) of |gs|^2
+__global__ void k_colsum(const double* __restrict__ gs, int dimup, int dimdw, double* __restrict__ colsum) {
+  __shared__ double sh[16];
+  const int idw = blockIdx.x;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < dimup; i += blockDim.x) {
+    const double g = gs[(size_t)idw * dimup + i];
+    v += g * g;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    colsum[idw] = s;
+  }
+}
+__global__ void k_rowsum(const double* __restrict__ gs, int dimup, int dimdw, double* __restrict__ rowsum) {
+  const int iup = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iup >= dimup) return;
+  double v = 0.0;
+  for (int idw = 0; idw < dimdw; ++idw) {
+    const double g = gs[(size_t)idw * dimup + iup];
+    v += g * g;
+  }
+  rowsum[iup] = v;
+}
+
+void launch_seed(const SeedSpec& s, const double* gs, int gs_dimup, const Sector& target, double* out,
+                 hipStream_t st) {
+  SeedTerm a{s.op, s.ispin, s.io - 1}, b{s.op, s.ispin, (s.jo > 0 ? s.jo : s.io) - 1};
+  const int nterms = s.jo > 0 ? 2 : 1;
+  int64_t g = (target.dim + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  hipLaunchKernelGGL(k_seed, dim3((int)g), dim3(kBlock), 0, st, gs, gs_dimup, target.sp[0].map, target.sp[1].map,
+                     target.sp[0].dim, target.dim, a, b, nterms, out);
+  EDL_HIP(hipGetLastError());
+}
+
+void device_observables(Ctx& c, const Sector& s, const double* gs, double* dens_up, double* dens_dw, double* docc) {
+  const int Ns = c.model.Ns;
+  hipStream_t st = c.stream;
+  const int dimup = s.sp[0].dim, dimdw = s.sp[1].dim;
+  const int nb = 1024;
+  double* part = nullptr;
+  double* rs = nullptr;
+  double* cs = nullptr;
+  EDL_HIP(hipMalloc(&part, sizeof(double) * nb * 32));
+  EDL_HIP(hipMalloc(&rs, sizeof(double) * dimup));
+  EDL_HIP(hipMalloc(&cs, sizeof(double) * dimdw));
+  hipLaunchKernelGGL(k_docc<32>, dim3(nb), dim3(kBlock), 0, st, gs, s.sp[0].map, s.sp[1].map, dimup, s.dim, part);
+  hipLaunchKernelGGL(k_rowsum, dim3((dimup + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gs, dimup, dimdw, rs);
+  hipLaunchKernelGGL(k_colsum, dim3(dimdw), dim3(kBlock), 0, st, gs, dimup, dimdw, cs);
+  EDL_HIP(hipGetLastError());
+  std::vector<double> hp((size_t)nb * 32), hr(dimup), hc(dimdw);
+  std::vector<int32_t> mu(dimup), md(dimdw);
+  EDL_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(hr.data(), rs, sizeof(double) * dimup, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(hc.data(), cs, sizeof(double) * dimdw, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(mu.data(), s.sp[0].map, sizeof(int32_t) * dimup, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(md.data(), s.sp[1].map, sizeof(int32_t) * dimdw, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  EDL_HIP(hipFree(part));
+  EDL_HIP(hipFree(rs));
+  EDL_HIP(hipFree(cs));
+  for (int p = 0; p < Ns; ++p) {
+    double d = 0.0, u = 0.0, w = 0.0;
+    for (int b = 0; b < nb; ++b) d += hp[(size_t)b * 32 + p];
+    for (int i = 0; i < dimup; ++i) u += ((mu[i] >> p) & 1) ? hr[i] : 0.0;
+    for (int i = 0; i < dimdw; ++i) w += ((md[i] >> p) & 1) ? hc[i] : 0.0;
+    docc[p] = d;
+    dens_up[p] = u;
+    dens_dw[p] = w;
+  }
+}
+
+}  // namespace edl

@@ -1,0 +1,288 @@
+"""Parity of the HIP engine against the oracle (GPU).
+
+Integer work (maps, ranks, hop tables) must be bit-exact; FP64 H x v within
+1e-13 of the oracle relative to max|Hv|; FP32 H x v within 2e-6; FP64 E0
+within 1e-10 relative; GF/chi continued fractions within 1e-8 on a
+frequency grid."""
+import numpy as np
+import pytest
+
+from ed_lattice_amd import EDLattice, eval_gf, lattice
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+
+
+@pytest.fixture(scope="module")
+def ed():
+    e = EDLattice(0)
+    yield e
+    e.close()
+
+
+def _use(ed, orc, H, U, **kw):
+    ed.set_hamiltonian(H, U, **kw)
+    return orc.Model(H, U, **kw)
+
+
+def _relerr(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("Ns,nup,ndw", [(8, 4, 4), (12, 6, 6), (12, 7, 6), (12, 5, 6), (16, 8, 8), (18, 9, 9),
+                                        (6, 0, 6), (5, 5, 0)])
+def test_maps_bit_exact(ed, orc, Ns, nup, ndw):
+    print(f"maps Ns={Ns} ({nup},{ndw})", flush=True)
+    ed.set_hamiltonian(lattice.ring(Ns), [1.0])
+    mu, md = ed.sector_maps(ed.sector_index(nup, ndw))
+    assert np.array_equal(mu, orc.build_map(Ns, nup))
+    assert np.array_equal(md, orc.build_map(Ns, ndw))
+
+
+def test_rank_equals_binary_search(ed, orc):
+    Ns = 12
+    ed.set_hamiltonian(lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(7, 5)
+    states = RNG.integers(0, 1 << Ns, 5000).astype(np.int32)
+    mu = orc.build_map(Ns, 7)
+    states[:100] = mu[RNG.integers(0, mu.size, 100)]
+    pos = ed.sector_rank(isec, 1, states)
+    ref = np.array([orc.binary_search(mu, int(s)) for s in states])
+    assert np.array_equal(pos, ref)
+
+
+@pytest.mark.parametrize("H,Ns,N", [(lattice.ring(8), 8, 4), (lattice.square(3, 4), 12, 7),
+                                    (lattice.disordered_rect(2, 4, seed=11), 8, 3)])
+def test_hop_tables_reproduce_c_cdg(ed, orc, H, Ns, N):
+    """Every table entry of output string s is a source string s' with
+    c^dg_io c_jo s' = sg1*sg2 s (c/cdg of AUX_FUNX.f90) and coefficient
+    Hij(io,jo)*sg1*sg2; and every such source is listed."""
+    ed.set_hamiltonian(H, [2.0])
+    isec = ed.sector_index(N, N)
+    nbr, coef = ed.sector_hops(isec, 1)
+    m = orc.build_map(Ns, N)
+    for a, s in enumerate(m):
+        got = {}
+        for k in range(nbr.shape[0]):
+            if coef[k, a] != 0.0:
+                got[int(nbr[k, a])] = coef[k, a]
+        want = {}
+        for io in range(1, Ns + 1):
+            for jo in range(1, Ns + 1):
+                if io == jo or H[0, io - 1, jo - 1] == 0.0:
+                    continue
+                src = s ^ (1 << (io - 1)) ^ (1 << (jo - 1))
+                if not ((s >> (io - 1)) & 1) or ((s >> (jo - 1)) & 1):
+                    continue
+                k1, g1 = orc.c_op(jo, int(src))
+                k2, g2 = orc.cdg_op(io, k1)
+                assert k2 == s
+                want[orc.binary_search(m, int(src)) - 1] = H[0, io - 1, jo - 1] * g1 * g2
+        assert got.keys() == want.keys()
+        for j in want:
+            assert got[j] == want[j]
+
+
+CASES = [
+    ("ring8_U4", lambda: (lattice.ring(8), [4.0], {}), (4, 4)),
+    ("rect3x4_U6_76", lambda: (lattice.square(3, 4), [6.0], {}), (7, 6)),
+    ("rect2x4_dis_mu", lambda: (lattice.disordered_rect(2, 4, seed=3), [3.5], dict(xmu=0.4)), (3, 5)),
+    ("ring7_nohf_fill", lambda: (lattice.ring(7), [2.0], dict(hfmode=False, xmu=0.3, ed_filling=7)), (4, 3)),
+    ("nspin2", lambda: (np.stack([lattice.disordered_rect(2, 3, seed=5)[0], lattice.disordered_rect(2, 3, seed=6)[0]]),
+                        [2.5], {}), (2, 4)),
+    ("two_orbital", lambda: (_two_orb(), [3.0, 2.0], dict(Nsites=[2, 2], Ust=1.5, Jh=0.4, xmu=0.1)), (2, 2)),
+]
+
+
+def _two_orb():
+    H = np.zeros((1, 4, 4))
+    for a, b, t in [(0, 1, -1.0), (2, 3, -0.6), (0, 2, -0.3), (1, 3, -0.3)]:
+        H[0, a, b] = H[0, b, a] = t
+    H[0, 0, 0], H[0, 3, 3] = 0.2, -0.1
+    return H
+
+
+@pytest.mark.parametrize("name,mk,sec", CASES, ids=[c[0] for c in CASES])
+def test_hxv_fp64_and_fp32_vs_oracle(ed, orc, name, mk, sec):
+    print(f"hxv case {name} sector {sec}", flush=True)
+    H, U, kw = mk()
+    m = _use(ed, orc, H, U, **kw)
+    isec = ed.sector_index(*sec)
+    dim, _, _ = ed.build_sector(isec)
+    for _ in range(3):
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+    vc = RNG.standard_normal(dim) + 1j * RNG.standard_normal(dim)
+    refc = orc.matvec(m, *sec, vc, nthreads=8)
+    assert _relerr(ed.direct_matvec(isec, vc, "fp64"), refc) < 1e-13
+
+
+def test_hxv_rect3x4_gs_sector(ed, orc):
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    dim, _, _ = ed.build_sector(isec)
+    v = RNG.standard_normal(dim)
+    ref = orc.matvec(m, 6, 6, v, nthreads=8).real
+    assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+
+
+def test_hxv_sq4x4_full_size_rows_and_hermiticity(ed, orc):
+    """4x4 PBC (8,8), dim 165 636 900: sampled rows against the oracle's row
+    evaluator, and <u,Hv> = <Hu,v>."""
+    m = _use(ed, orc, lattice.square(4, 4), [8.0])
+    isec = ed.sector_index(8, 8)
+    dim, _, _ = ed.build_sector(isec)
+    assert dim == 165636900
+    v = RNG.standard_normal(dim)
+    hv = ed.direct_matvec(isec, v, "fp64")
+    rows = np.concatenate([[0, dim - 1], RNG.integers(0, dim, 300)])
+    ref = orc.hxv_rows(m, 8, 8, rows, v).real
+    assert np.max(np.abs(hv[rows] - ref)) < 1e-13 * np.max(np.abs(hv))
+    u = RNG.standard_normal(dim)
+    hu = ed.direct_matvec(isec, u, "fp64")
+    a, b = u @ hv, hu @ v
+    assert abs(a - b) < 1e-12 * np.sqrt((u @ u) * (v @ v)) * 40
+
+
+def test_lanczos_gs_ring8_vs_dense(ed, orc):
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    Hd = orc.hamiltonian_csr(m, 4, 4).toarray().real
+    w, V = np.linalg.eigh(Hd)
+    r = ed.lanczos_eigh(isec, "fp64", niter=200, want_vector=True)
+    assert abs(r["e0"] - w[0]) < 1e-10 * abs(w[0])
+    assert 1 - abs(r["gs"] @ V[:, 0]) < 1e-9
+    # alpha/beta of the first steps equal the oracle's recurrence
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    a, b, _ = orc.lanczos_tridiag(Hc, lattice.splitmix_vector(Hd.shape[0], 0x5EED), 20)
+    np.testing.assert_allclose(r["alpha"][:20], a, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(r["beta"][1:20], b[1:], rtol=1e-10, atol=1e-10)
+    obs = ed.observables()
+    ref = orc.observables(8, 4, 4, r["gs"])
+    for k in ("dens_up", "dens_dw", "docc"):
+        np.testing.assert_allclose(obs[k], ref[k], atol=1e-12)
+
+
+@pytest.mark.parametrize("cfg", ["ring8", "rect3x4"])
+def test_mixed_precision_e0(ed, orc, cfg):
+    c = lattice.CONFIGS[cfg]
+    H = c["H"]()
+    m = _use(ed, orc, H, [c["U"]])
+    isec = ed.sector_index(c["nup"], c["ndw"])
+    r64 = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    d64 = ed.observables()["docc"].mean()
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=300, want_vector=True, refine_tol=1e-13)
+    dmx = ed.observables()["docc"].mean()
+    print(f"{cfg}: E0 fp64 {r64['e0']:.14f} mixed {rmx['e0']:.14f} fp32-only {rmx['e0_fp32']:.14f} "
+          f"steps fp32 {rmx['steps_fp32']} fp64 {rmx['steps_fp64']} docc {d64:.12f} {dmx:.12f}", flush=True)
+    assert abs(rmx["e0"] - r64["e0"]) < 1e-10 * abs(r64["e0"])
+    assert abs(dmx - d64) < 1e-7
+
+
+def test_rect3x4_e0_vs_oracle(ed, orc):
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    r = ed.lanczos_eigh(isec, "fp64", niter=200)
+    Hc = orc.hamiltonian_csr(m, 6, 6)
+    e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
+    assert abs(r["e0"] - e0) < 1e-10 * abs(e0)
+
+
+def _gz(poles, weights, z):
+    return (weights[None, :] / (z[:, None] - poles[None, :])).sum(1)
+
+
+def test_tridiag_seed_vs_oracle(ed, orc):
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(5, 4)
+    dim, _, _ = ed.build_sector(isec)
+    seed = RNG.standard_normal(dim)
+    a, b, n2 = ed.tridiag_Hv_sector(isec, seed, 100, "fp64")
+    assert abs(n2 - seed @ seed) < 1e-10 * n2
+    Hc = orc.hamiltonian_csr(m, 5, 4)
+    ao, bo, _ = orc.lanczos_tridiag(Hc, seed, 100)
+    np.testing.assert_allclose(a[:15], ao[:15], rtol=1e-10, atol=1e-10)
+    z = np.linspace(-8, 8, 81) + 0.05j
+    p, w = ed.lanczos_poles(a, b, n2, -3.0, 1)
+    po, wo = orc.gf_poles(ao, bo, n2, -3.0, 1)
+    np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8 * n2)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-8), ("fp32", 2e-4)])
+def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol):
+    """Config ring8: all diagonal + off-diagonal GF seeds of spin up in one
+    batched launch, continued fractions vs the oracle's seeds + Lanczos."""
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    gs, e0 = r["gs"], r["e0"]
+    sites = [1, 2, 5]
+    gf, raw = ed.build_gf_electrons(e0, sites, offdiag=True, spins=(1, 2), nlanc=200, prec=prec)
+    z = np.linspace(-6, 6, 121) + 0.1j
+    for (ispin, io, jo), chans in gf.items():
+        for ch, (op, isign) in enumerate([(2, 1), (1, -1)]):
+            d = 1 if op == 2 else -1
+            sec_j = (4 + d, 4) if ispin == 1 else (4, 4 + d)
+            s = orc.seed_c_cdg(8, op, io, ispin, (4, 4), sec_j, gs).real
+            if jo != io:
+                s = s + orc.seed_c_cdg(8, op, jo, ispin, (4, 4), sec_j, gs).real
+            Hj = orc.hamiltonian_csr(m, *sec_j)
+            ao, bo, nst = orc.lanczos_tridiag(Hj, s, 200)
+            po, wo = orc.gf_poles(ao, bo, s @ s, e0, isign)
+            p, w = chans[ch]
+            np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=tol)
+            assert abs(w.sum() - s @ s) < 1e-10
+
+
+def test_gf_batched_equals_sequential_rect3x4(ed, orc):
+    """Config rect3x4: 2 x 78 seeds (i <= j pairs, particle + hole) of spin up;
+    one batched launch vs one seed at a time, and a few seeds vs the oracle."""
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    seeds = []
+    for i in range(1, 13):
+        for j in range(i, 13):
+            jo = 0 if i == j else j
+            seeds += [("cdg", 1, i, jo), ("c", 1, i, jo)]
+    assert len(seeds) == 156
+    rb = ed.tridiag_batched(seeds, 150, "fp64")
+    rs = ed.tridiag_batched(seeds[:12], 150, "fp64", group=1)
+    np.testing.assert_allclose(rb["alpha"][:12], rs["alpha"], rtol=1e-9, atol=1e-9)
+    print(f"rect3x4 156 seeds: batched {rb['ms']:.1f} ms, 12 sequential {rs['ms']:.1f} ms", flush=True)
+    z = np.linspace(-10, 10, 101) + 0.1j
+    csr = {}
+    for k in (0, 1, 5):
+        op, ispin, io, jo = seeds[k]
+        opc = 2 if op == "cdg" else 1
+        d = 1 if opc == 2 else -1
+        s = orc.seed_c_cdg(12, opc, io, 1, (6, 6), (6 + d, 6), r["gs"]).real
+        if jo:
+            s = s + orc.seed_c_cdg(12, opc, jo, 1, (6, 6), (6 + d, 6), r["gs"]).real
+        assert abs(rb["norm2"][k] - s @ s) < 1e-10
+        if d not in csr:
+            csr[d] = orc.hamiltonian_csr(m, 6 + d, 6)
+        ao, bo, _ = orc.lanczos_tridiag(csr[d], s, 150)
+        isign = 1 if opc == 2 else -1
+        po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], isign)
+        p, w = ed.lanczos_poles(rb["alpha"][k], rb["beta"][k], rb["norm2"][k], r["e0"], isign)
+        np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)
+
+
+def test_chi_batched_vs_oracle(ed, orc):
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    chi, raw = ed.build_chi_spin(r["e0"], [1, 3], offdiag=True, nlanc=150)
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    z = np.linspace(0, 8, 81) + 0.1j
+    for (io, jo), (p, w) in chi.items():
+        s = orc.seed_sz(8, io, (4, 4), r["gs"]).real
+        if jo != io:
+            s = s + orc.seed_sz(8, jo, (4, 4), r["gs"]).real
+        ao, bo, _ = orc.lanczos_tridiag(Hc, s, 150)
+        po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], 1)
+        np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)

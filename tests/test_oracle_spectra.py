@@ -1,0 +1,171 @@
+"""Pins of the oracle's Lanczos, energies, GF and chi poles (CPU only).
+
+Closed forms: free fermions (U = 0), the two-site Hubbard model, the atomic
+limit; exact Lehmann sums from dense diagonalisation of tiny sectors; sum
+rules; twin symmetry E0(Nup,Ndw) = E0(Ndw,Nup)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from ed_lattice_amd import lattice
+
+
+def _e0_dense(orc, m, nup, ndw):
+    return np.linalg.eigvalsh(orc.hamiltonian_csr(m, nup, ndw).toarray().real)[0]
+
+
+def test_free_fermion_ground_state_energy(orc):
+    """U = 0: E0 = sum of the lowest Nup + lowest Ndw levels of Hij.  The
+    periodic wrap bond carries the fermionic sign, so a dropped Jordan-Wigner
+    sign (hard-core bosons) gives a different number."""
+    for H, Ns, nup, ndw in [(lattice.ring(8), 8, 3, 4), (lattice.square(2, 3), 6, 2, 3),
+                            (lattice.ring(6), 6, 2, 2)]:
+        m = orc.Model(H, [0.0])
+        eps = np.linalg.eigvalsh(H[0])
+        expect = eps[:nup].sum() + eps[:ndw].sum()
+        assert abs(_e0_dense(orc, m, nup, ndw) - expect) < 1e-11
+
+
+def test_two_site_hubbard(orc):
+    """E0 = U/2 - sqrt(U^2/4 + 4t^2) at half filling, minus U*N/2 from the
+    Hartree shift of hfmode (HxV_local.f90:67-75)."""
+    H = lattice.hij_from_bonds(2, [(0, 1)], t=1.3)
+    for U in (0.5, 4.0, 10.0):
+        e = U / 2 - np.sqrt(U * U / 4 + 4 * 1.3 ** 2)
+        assert abs(_e0_dense(orc, orc.Model(H, [U], hfmode=False), 1, 1) - e) < 1e-12
+        assert abs(_e0_dense(orc, orc.Model(H, [U]), 1, 1) - (e - U)) < 1e-12
+
+
+def test_atomic_limit(orc):
+    """t = 0 with onsite energies: E0 = min over occupation patterns."""
+    Ns = 5
+    eps = np.array([0.3, -0.7, 0.1, -0.2, 0.5])
+    H = np.diag(eps)[None]
+    U, nup, ndw = 2.0, 2, 3
+    m = orc.Model(H, [U], xmu=0.15)
+    best = np.inf
+    for up in itertools.combinations(range(Ns), nup):
+        for dw in itertools.combinations(range(Ns), ndw):
+            nu = np.zeros(Ns)
+            nd = np.zeros(Ns)
+            nu[list(up)] = 1
+            nd[list(dw)] = 1
+            e = (eps * (nu + nd)).sum() + U * (nu * nd).sum() - 0.15 * (nu + nd).sum() - 0.5 * U * (nu + nd).sum()
+            best = min(best, e)
+    assert abs(_e0_dense(orc, m, nup, ndw) - best) < 1e-12
+
+
+def test_twin_symmetry(orc):
+    m = orc.Model(lattice.square(2, 3), [3.0])
+    assert abs(_e0_dense(orc, m, 2, 4) - _e0_dense(orc, m, 4, 2)) < 1e-11
+
+
+def test_lanczos_matches_dense(orc):
+    """ring8 (4,4), U = 4 (dim 4900): 200-step Lanczos E0 and vector vs eigh."""
+    m = orc.Model(lattice.ring(8), [4.0])
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    w, V = np.linalg.eigh(Hc.toarray().real)
+    v0 = lattice.splitmix_vector(Hc.shape[0], 0x5EED)
+    e0, x, _ = orc.lanczos_ground_state(Hc, v0, 200)
+    assert abs(e0 - w[0]) < 1e-10 * abs(w[0])
+    assert 1 - abs(x @ V[:, 0]) < 1e-9
+
+
+def _lehmann(orc, m, Ns, sec_i, op, ipos, ispin):
+    """Exact poles/weights of <gs| A^dg (z - H)^-1 A |gs> by dense diagonalisation."""
+    Hi = orc.hamiltonian_csr(m, *sec_i).toarray().real
+    wi, Vi = np.linalg.eigh(Hi)
+    gs = Vi[:, 0]
+    if op == "sz":
+        seed = orc.seed_sz(Ns, ipos, sec_i, gs).real
+        wj, Vj = wi, Vi
+    else:
+        d = 1 if op == "cdg" else -1
+        sec_j = (sec_i[0] + d, sec_i[1]) if ispin == 1 else (sec_i[0], sec_i[1] + d)
+        seed = orc.seed_c_cdg(Ns, 2 if op == "cdg" else 1, ipos, ispin, sec_i, sec_j, gs).real
+        wj, Vj = np.linalg.eigh(orc.hamiltonian_csr(m, *sec_j).toarray().real)
+    amp = Vj.T @ seed
+    return wi[0], wj, amp ** 2, seed
+
+
+@pytest.mark.parametrize("op,ispin", [("cdg", 1), ("c", 1), ("cdg", 2), ("sz", 1)])
+def test_lanczos_poles_reproduce_lehmann_sum(orc, op, ispin):
+    """Continued fraction from the oracle Lanczos equals the exact Lehmann sum
+    on a frequency grid (ring6, U = 4, (3,3), target Krylov space exhausted)."""
+    Ns = 6
+    m = orc.Model(lattice.ring(6), [4.0])
+    e0, ej, wex, seed = _lehmann(orc, m, Ns, (3, 3), op, 2, ispin)
+    d = 1 if op == "cdg" else (-1 if op == "c" else 0)
+    sec_j = (3, 3) if op == "sz" else ((3 + d, 3) if ispin == 1 else (3, 3 + d))
+    Hj = orc.hamiltonian_csr(m, *sec_j)
+    norm2 = seed @ seed
+    nl = Hj.shape[0]
+    a, b, nst = orc.lanczos_tridiag(Hj, seed, nl)
+    isign = -1 if op == "c" else 1
+    p, w = orc.gf_poles(a[:nst], b[:nst], norm2, e0, isign)
+    z = np.linspace(-6, 6, 61) + 0.3j
+    g_l = (w[None, :] / (z[:, None] - p[None, :])).sum(1)
+    g_x = (wex[None, :] / (z[:, None] - isign * (ej[None, :] - e0))).sum(1)
+    np.testing.assert_allclose(g_l, g_x, atol=1e-10)
+    assert abs(w.sum() - norm2) < 1e-12
+
+
+def test_free_fermion_gf_closed_form(orc):
+    """U = 0, ring8 (3,3): G_11 poles at the single-particle levels with
+    weight multiplicity/L, and the mixed seed gives G_12 of the plane waves."""
+    Ns = 8
+    m = orc.Model(lattice.ring(8), [0.0])
+    Hi = orc.hamiltonian_csr(m, 3, 3)
+    w, V = np.linalg.eigh(Hi.toarray().real)
+    gs = V[:, 0]
+    e0 = w[0]
+    ks = 2 * np.pi * np.arange(Ns) / Ns
+    eps = -2 * np.cos(ks)
+    occ = np.argsort(eps)[:3]
+    filled = np.zeros(Ns, bool)
+    filled[occ] = True
+    z = np.linspace(-3, 3, 41) + 0.2j
+
+    def g_closed(i, j, particle):
+        sel = ~filled if particle else filled
+        return (np.cos(ks[sel] * (i - j))[None, :] / (z[:, None] - eps[sel][None, :])).sum(1) / Ns
+
+    def g_lanczos(io, jo, op):
+        d = 1 if op == 2 else -1
+        sec_j = (3 + d, 3)
+        s = orc.seed_c_cdg(Ns, op, io, 1, (3, 3), sec_j, gs).real
+        if jo:
+            s = s + orc.seed_c_cdg(Ns, op, jo, 1, (3, 3), sec_j, gs).real
+        Hj = orc.hamiltonian_csr(m, *sec_j)
+        a, b, nst = orc.lanczos_tridiag(Hj, s, 150)
+        p, wt = orc.gf_poles(a[:nst], b[:nst], s @ s, e0, 1 if op == 2 else -1)
+        return (wt[None, :] / (z[:, None] - p[None, :])).sum(1)
+
+    # diagonal: particle (cdg) + hole (c)
+    np.testing.assert_allclose(g_lanczos(1, 0, 2), g_closed(0, 0, True), atol=1e-9)
+    np.testing.assert_allclose(g_lanczos(1, 0, 1), g_closed(0, 0, False), atol=1e-9)
+    # off-diagonal from the mixed seed: G_12 = (G_mix - G_11 - G_22) / 2
+    for op, part in [(2, True), (1, False)]:
+        gmix = g_lanczos(1, 2, op)
+        g12 = 0.5 * (gmix - g_lanczos(1, 0, op) - g_lanczos(2, 0, op))
+        np.testing.assert_allclose(g12, g_closed(0, 1, part), atol=1e-9)
+
+
+def test_gf_sum_rules_and_chi(orc):
+    """Sum of particle weights = 1 - n, hole weights = n; chi weights sum to
+    <(S^z_i)^2>."""
+    Ns = 8
+    m = orc.Model(lattice.ring(8), [4.0])
+    Hi = orc.hamiltonian_csr(m, 4, 4)
+    e0, gs, _ = orc.lanczos_ground_state(Hi, lattice.splitmix_vector(Hi.shape[0], 0x5EED), 300)
+    obs = orc.observables(Ns, 4, 4, gs)
+    sp = orc.seed_c_cdg(Ns, 2, 3, 1, (4, 4), (5, 4), gs).real
+    sh = orc.seed_c_cdg(Ns, 1, 3, 1, (4, 4), (3, 4), gs).real
+    assert abs(sp @ sp - (1 - obs["dens_up"][2])) < 1e-10
+    assert abs(sh @ sh - obs["dens_up"][2]) < 1e-10
+    sz = orc.seed_sz(Ns, 3, (4, 4), gs).real
+    expect = 0.25 * (obs["dens_up"][2] + obs["dens_dw"][2] - 2 * obs["docc"][2])
+    assert abs(sz @ sz - expect) < 1e-10
+    # half filling + particle-hole symmetry of the ring: n = 1/2 per spin
+    np.testing.assert_allclose(obs["dens_up"], 0.5, atol=1e-8)
