@@ -4,9 +4,11 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 
 #include "../../include/edlat_gpu.h"
 #include "chains.hpp"
+#include "hxv2.hpp"
 #include "seeds.hpp"
 #include "tridiag.hpp"
 
@@ -86,6 +88,116 @@ struct EventTimer {
 
 template <typename T>
 double device_sumsq(const T* x, int64_t n, hipStream_t st);
+
+// Context-owned scratch (the intermediate of the two-pass operator).  It is
+// only grown between launches, never while a graph that uses it is alive.
+static void* scratch(Ctx& c, size_t bytes) {
+  if (c.scratch_bytes < bytes) {
+    EDL_HIP(hipStreamSynchronize(c.stream));
+    if (c.d_scratch) EDL_HIP(hipFree(c.d_scratch));
+    c.d_scratch = nullptr;
+    c.scratch_bytes = 0;
+    EDL_HIP(hipMalloc(&c.d_scratch, bytes));
+    c.scratch_bytes = bytes;
+  }
+  return c.d_scratch;
+}
+
+template <typename T>
+static const T* lut_dev(const SpinTable& t);
+template <>
+const double* lut_dev<double>(const SpinTable& t) { return t.d_lut64; }
+template <>
+const float* lut_dev<float>(const SpinTable& t) { return t.d_lut32; }
+
+// Tuned two-pass schedule per (sector shape, precision).
+static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg>& tp_cache() {
+  static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg> m;
+  return m;
+}
+
+// Pick panel widths, block sizes and grids of the two passes by timing the
+// operator on the sector itself (3 launches per candidate after a warm-up).
+template <typename T>
+static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
+  const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T));
+  auto it = tp_cache().find(key);
+  if (it != tp_cache().end()) return it->second;
+  hipStream_t st = c.stream;
+  const TwoPassView V = two_pass_view(s);
+  const int64_t D = s.dim;
+  DevBuf d64(sizeof(double) * D), q(sizeof(T) * D), t(sizeof(T) * D), o(sizeof(T) * D);
+  launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
+  launch_convert<T>(d64.as<double>(), q.as<T>(), D, st);
+  TwoPassCfg best;
+  best.grid_up = c.num_cu;
+  best.grid_dw = c.num_cu * 8;
+  double best_up = 1e300, best_dw = 1e300;
+  const int lds = 160 * 1024;
+  for (int C : {1, 2, 4})
+    for (int th : {512, 1024})
+      for (int gm : {1, 2}) {
+        const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
+        if (need > (size_t)lds) continue;
+        if (gm == 2 && 2 * need > (size_t)lds) continue;
+        TwoPassCfg w = best;
+        w.C = C;
+        w.threads_up = th;
+        w.grid_up = c.num_cu * gm;
+        launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
+        EventTimer tm(st);
+        for (int r = 0; r < 3; ++r)
+          launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
+        const double ms = tm.stop();
+        if (ms < best_up) {
+          best_up = ms;
+          best.C = C;
+          best.threads_up = th;
+          best.grid_up = w.grid_up;
+        }
+      }
+  for (int R : {32, 64})
+    for (int th : {256, 512, 1024})
+      for (int gm : {2, 4, 8, 16}) {
+        TwoPassCfg w = best;
+        w.R = R;
+        w.threads_dw = th;
+        w.grid_dw = c.num_cu * gm;
+        w.grid_dw -= w.grid_dw % 8;
+        launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+        EventTimer tm(st);
+        for (int r = 0; r < 3; ++r)
+          launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+        const double ms = tm.stop();
+        if (ms < best_dw) {
+          best_dw = ms;
+          best.R = R;
+          best.threads_dw = th;
+          best.grid_dw = w.grid_dw;
+        }
+      }
+  tp_cache()[key] = best;
+  return best;
+}
+
+// H v on one sector: two-pass kernels where the columns fit in LDS,
+// otherwise the single-pass gather kernel.
+template <typename T>
+static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, hipStream_t st) {
+  if (two_pass_ok(s, sizeof(T))) {
+    const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
+    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * s.dim));
+    const TwoPassView V = two_pass_view(s);
+    launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v, v, tmp, nullptr, cfg, st);
+    launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), const_cast<T*>(v), const_cast<T*>(v), tmp, out, nullptr, cfg, st);
+  } else {
+    Schedule w;
+    w.block_threads = 256;
+    w.rows_per_thread = 1;
+    w.grid = c.num_cu * 8;
+    hxv_operator<T>(c, s, v, out, st, w);
+  }
+}
 
 // ----------------------------------------------------------- chain runs
 struct ChainOut {
@@ -197,12 +309,31 @@ static Schedule autotune(Ctx& c, const std::vector<ChainDev>& chains, hipStream_
 // two phases of a step captured once in a hipGraph and replayed.  With
 // check > 0 and a single chain, the lowest Ritz value is checked every
 // `check` steps and the run ends when it changes by <= tol*|E|.
+// Phase A of a single large chain done by the two-pass operator.
 template <typename T>
-static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, hipStream_t st) {
+struct TwoPassStep {
+  TwoPassView V;
+  TwoPassCfg cfg;
+  const T* lut_up;
+  const T* lut_dw;
+  T* tmp;
+  LzA A;
+};
+
+template <typename T>
+static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, hipStream_t st,
+                     const TwoPassStep<T>* tp = nullptr) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  lz_phase_a(c, cs, st);
+  if (tp) {
+    T* q0 = static_cast<T*>(cs.h[0].q0);
+    T* q1 = static_cast<T*>(cs.h[0].q1);
+    launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, cs.h[0].sc, tp->cfg, st);
+    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->A, tp->cfg, st);
+  } else {
+    lz_phase_a(c, cs, st);
+  }
   lz_phase_b(c, cs, st);
   EDL_HIP(hipStreamEndCapture(st, &g));
   EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -276,15 +407,37 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   const int nmax = (int)std::min<int64_t>(niter, S.dim);
   GsRun r;
   Schedule sch;
+  // Large sectors: phase A by the two-pass operator (its partial sums and
+  // counter are separate from phase B's).
+  const bool two = two_pass_ok(S, sizeof(T));
+  TwoPassStep<T> tps{};
+  DevBuf tp_part, tp_counter;
+  if (two) {
+    tps.cfg = two_pass_cfg<T>(c, S);
+    tps.V = two_pass_view(S);
+    tps.lut_up = lut_dev<T>(S.sp[0]);
+    tps.lut_dw = lut_dev<T>(S.sp[1]);
+    tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * S.dim));
+    tp_part = DevBuf(sizeof(double) * tps.cfg.grid_dw);
+    tp_counter = DevBuf(sizeof(unsigned));
+    launch_zero_bytes(tp_counter.p, 4, st);
+    tps.A.part = tp_part.as<double>();
+    tps.A.counter = tp_counter.as<unsigned>();
+    sch.block_threads = 256;
+    sch.rows_per_thread = 4;
+    sch.grid = c.num_cu * 8;
+  }
   {
     ChainSet<T> cs;
     ChainAlloc<T> A;
     ChainDev cd{};
     init_chain<T>(c, cd, A, S, d_start, 1.0, nmax, st);
-    sch = autotune<T>(c, {cd}, st);
+    if (!two) sch = autotune<T>(c, {cd}, st);
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
-    r.hxv += run_steps(c, cs, nmax, check, tol, st);
+    tps.A.sc = cd.sc;
+    tps.A.alpha = cd.alpha;
+    r.hxv += run_steps(c, cs, nmax, check, tol, st, two ? &tps : nullptr);
     r.out = read_chain<T>(cs.h[0], nmax, st);
   }
   std::vector<double> cvec;
@@ -302,7 +455,9 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     cd.ccoef = dc.as<double>();
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
-    r.hxv += run_steps(c, cs, m, 0, 0.0, st);
+    tps.A.sc = cd.sc;
+    tps.A.alpha = cd.alpha;
+    r.hxv += run_steps(c, cs, m, 0, 0.0, st, two ? &tps : nullptr);
     const double n2 = device_dot(c, d_x, d_x, S.dim, st);
     launch_scale(d_x, S.dim, 1.0 / std::sqrt(n2), st);
     EDL_HIP(hipStreamSynchronize(st));
@@ -544,9 +699,9 @@ extern "C" int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v, void*
   Sector& s = get_sector(ctx->c, isector);
   EventTimer tm(ctx->c.stream);
   if (prec == EDL_FP64)
-    hxv_operator<double>(ctx->c, s, (const double*)v, (double*)hv, ctx->c.stream, operator_schedule(ctx->c));
+    apply_h<double>(ctx->c, s, (const double*)v, (double*)hv, ctx->c.stream);
   else
-    hxv_operator<float>(ctx->c, s, (const float*)v, (float*)hv, ctx->c.stream, operator_schedule(ctx->c));
+    apply_h<float>(ctx->c, s, (const float*)v, (float*)hv, ctx->c.stream);
   ctx->c.last_ms = tm.stop();
   ctx->c.last_hxv = 1;
   API_END
@@ -561,7 +716,7 @@ static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_
   DevBuf d64(sizeof(double) * D), dv(sizeof(T) * D), dh(sizeof(T) * D);
   EDL_HIP(hipMemcpyAsync(d64.p, tmp.data(), sizeof(double) * D, hipMemcpyHostToDevice, st));
   launch_convert<T>(d64.as<double>(), dv.as<T>(), D, st);
-  hxv_operator<T>(c, s, dv.as<T>(), dh.as<T>(), st, operator_schedule(c));
+  apply_h<T>(c, s, dv.as<T>(), dh.as<T>(), st);
   launch_to_fp64<T>(dh.as<T>(), d64.as<double>(), D, st);
   EDL_HIP(hipMemcpyAsync(tmp.data(), d64.p, sizeof(double) * D, hipMemcpyDeviceToHost, st));
   EDL_HIP(hipStreamSynchronize(st));
@@ -876,10 +1031,9 @@ static double bench_impl(Ctx& c, Sector& s, int reps) {
   DevBuf d64(sizeof(double) * D), v(sizeof(T) * D), h(sizeof(T) * D);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
   launch_convert<T>(d64.as<double>(), v.as<T>(), D, st);
-  const Schedule w = operator_schedule(c);
-  hxv_operator<T>(c, s, v.as<T>(), h.as<T>(), st, w);
+  apply_h<T>(c, s, v.as<T>(), h.as<T>(), st);
   EventTimer tm(st);
-  for (int r = 0; r < reps; ++r) hxv_operator<T>(c, s, v.as<T>(), h.as<T>(), st, w);
+  for (int r = 0; r < reps; ++r) apply_h<T>(c, s, v.as<T>(), h.as<T>(), st);
   return tm.stop() / reps;
 }
 

@@ -56,6 +56,15 @@ struct SpinTable {
   double* coef64 = nullptr;    // [kmax][dim] Hij(io,jo) * fermionic sign
   float* coef32 = nullptr;     // same, rounded to fp32
   double* diag = nullptr;      // [dim] one-spin part of the diagonal
+  // Packed table: [kmax][dim] words nbr | cid << 24, with the coefficient
+  // lut[cid] = Hij(pair) * sign (cid = 2*pair + (sign < 0), 255 = padding).
+  // Available when dim < 2^24 and there are at most 127 hopping pairs.
+  uint32_t* packed = nullptr;
+  bool packed_ok = false;
+  double lut64[256] = {};
+  float lut32[256] = {};
+  double* d_lut64 = nullptr;   // device copies of the coefficient lut
+  float* d_lut32 = nullptr;
 };
 
 struct Sector {

@@ -3,3 +3,4 @@
 #include "sector.hip"
 #include "hxv.hip"
 #include "seeds.hip"
+#include "hxv2.hip"

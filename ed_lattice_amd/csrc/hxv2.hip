@@ -1,0 +1,299 @@
+// Two-pass on-the-fly H x v for sectors whose columns fit in LDS.
+//
+//   pass U (column panels):  T(:, c) = [D + H_up] v(:, c)
+//     a block stages C whole columns of v in LDS, interleaved by row, and
+//     every row gathers its up-hop neighbours from LDS (HxV_up.f90,
+//     HxV_local.f90); the packed hop word of a row is read once for all C
+//     columns.
+//   pass D (row panels):     out(r, :) = T(r, :) + [H_dw v](r, :)
+//     XCD x walks the row panels p = x, x+8, ...; all blocks of that XCD
+//     sweep the columns of the same panel at the same time, so the panel
+//     (R rows x DimDw columns) is read from HBM once and its down-hop
+//     neighbour segments (R contiguous rows of another column,
+//     HxV_dw.f90) are served from that XCD's L2.
+//
+// Hop tables are packed words nbr | cid << 24 with the signed coefficient
+// lut[cid] kept in LDS.  Pass D carries the Lanczos phase-A fusion
+// (- beta v_{k-1}, alpha partial sums) when given a chain.
+#include "chains.hpp"
+#include "hxv2.hpp"
+
+namespace edl {
+
+template <typename T, int C>
+struct Vec;
+template <typename T>
+struct Vec<T, 1> {
+  typedef T type;
+  static __device__ __forceinline__ T get(const type& v, int) { return v; }
+  static __device__ __forceinline__ void set(type& v, int, T x) { v = x; }
+};
+template <typename T>
+struct Vec<T, 2> {
+  typedef T type __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ T get(const type& v, int j) { return j == 0 ? v.x : v.y; }
+  static __device__ __forceinline__ void set(type& v, int j, T x) {
+    if (j == 0) v.x = x; else v.y = x;
+  }
+};
+template <typename T>
+struct Vec<T, 4> {
+  typedef T type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ T get(const type& v, int j) { return v[j]; }
+  static __device__ __forceinline__ void set(type& v, int j, T x) { v[j] = x; }
+};
+
+static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t su, uint32_t sd) {
+  const uint32_t both = su & sd;
+  double e = 0.0;
+  for (int k = 0; k < I.norb; ++k) e += I.U[k] * (double)__popc(both & I.orb_mask[k]);
+  for (int p = 0; p < I.npairs; ++p) {
+    const uint32_t a = I.pi[p], b = I.pj[p];
+    e += I.Ust * (double)((((su >> a) & (sd >> b)) & 1u) + (((su >> b) & (sd >> a)) & 1u));
+  }
+  return e;
+}
+
+// ------------------------------------------------------------------ pass U
+// With a chain (sc != null) the input is q_k = (k odd ? qb : qa), scaled by 1/s_k.
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up,
+                                                  const T* qa, const T* qb, T* __restrict__ tout,
+                                                  const LzScalars* __restrict__ sc) {
+  typedef typename Vec<T, C>::type VT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut = reinterpret_cast<T*>(smem);
+  VT* P = reinterpret_cast<VT*>(smem + 256 * sizeof(T) + (16 - (256 * sizeof(T)) % 16) % 16);
+  double scale = 1.0;
+  const T* __restrict__ q = qa;
+  if (sc) {
+    const LzScalars s = *sc;
+    if (s.stopped) return;
+    scale = 1.0 / s.s_cur;
+    q = (s.k & 1) ? qb : qa;
+  }
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const int npan = (dimdw + C - 1) / C;
+  const T ts = (T)scale;
+  for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
+    const int c0 = pan * C;
+    __syncthreads();
+    for (int r = threadIdx.x; r < dimup; r += blockDim.x) {
+      VT x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, (c0 + j < dimdw) ? q[(size_t)(c0 + j) * dimup + r] : (T)0);
+      P[r] = x;
+    }
+    __syncthreads();
+    uint32_t sd[C];
+    double dgd[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = (c0 + j < dimdw) ? c0 + j : c0;
+      sd[j] = (uint32_t)V.map_dw[c];
+      dgd[j] = V.dg_dw[c];
+    }
+    for (int iup = threadIdx.x; iup < dimup; iup += blockDim.x) {
+      const uint32_t su = (uint32_t)V.map_up[iup];
+      const double dgu = V.dg_up[iup];
+      const VT self = P[iup];
+      T acc[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+      const uint32_t* __restrict__ pk = V.pk_up + iup;
+      for (int k = 0; k < V.kup; ++k) {
+        const uint32_t w = pk[(size_t)k * dimup];
+        const T cf = lut[w >> 24];
+        const VT x = P[w & 0xFFFFFFu];
+#pragma unroll
+        for (int j = 0; j < C; ++j) acc[j] += cf * Vec<T, C>::get(x, j);
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        if (c0 + j < dimdw) tout[(size_t)(c0 + j) * dimup + iup] = acc[j] * ts;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ pass D
+// MODE 0: out = tin + H_dw q.   MODE 1 (Lanczos phase A): out (the buffer of
+// q_{k-1}) = tin + H_dw q / s_k - beta_k q_{k-1} / s_{k-1}, alpha partials.
+template <typename T, int R, int MODE>
+__global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
+                                                  const T* __restrict__ tin, T* outp, LzA A) {
+  __shared__ T lut[256];
+  __shared__ double sh[16];
+  __shared__ int flag;
+  LzScalars s{};
+  const T* __restrict__ q = qa;
+  T* __restrict__ out = outp;
+  if (MODE == 1) {
+    s = *A.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+    out = (s.k & 1) ? qa : qb;
+  }
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_dw[k];
+  __syncthreads();
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int xcd = (G % 8 == 0) ? b % 8 : 0;
+  const int nbx = (G % 8 == 0) ? G / 8 : G;
+  const int jb = (G % 8 == 0) ? b / 8 : b;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int SLOTS = 64 / R;
+  const int r = lane % R, slot = lane / R;
+  const int nwaves = blockDim.x >> 6;
+  const int npan = (dimup + R - 1) / R;
+  const int cb = (int)((int64_t)jb * dimdw / nbx), ce = (int)((int64_t)(jb + 1) * dimdw / nbx);
+  const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
+  const T ts = (T)inv_s;
+  const T bo = (MODE == 1) ? (T)(s.beta_k / s.s_old) : (T)0;
+  double part = 0.0;
+  for (int p = xcd; p < npan; p += ((G % 8 == 0) ? 8 : 1)) {
+    const int iup = p * R + r;
+    const bool ok = iup < dimup;
+    for (int idw = cb + wave * SLOTS + slot; idw < ce; idw += nwaves * SLOTS) {
+      const uint32_t* __restrict__ pk = V.pk_dw + idw;
+      T acc = (T)0;
+      for (int k = 0; k < V.kdw; ++k) {
+        const uint32_t w = pk[(size_t)k * dimdw];
+        const T cf = lut[w >> 24];
+        if (ok) acc += cf * q[(size_t)(w & 0xFFFFFFu) * dimup + iup];
+      }
+      if (ok) {
+        const size_t i = (size_t)idw * dimup + iup;
+        if (MODE == 0) {
+          out[i] = tin[i] + acc;
+        } else {
+          const T tv = tin[i] + acc * ts - bo * out[i];
+          out[i] = tv;
+          part += (double)tv * (double)q[i];
+        }
+      }
+    }
+  }
+  if (MODE == 1) {
+    // deterministic partial per block, last block reduces (alpha_k)
+    double v = part * inv_s;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < nwaves; ++w) t += sh[w];
+      A.part[b] = t;
+      __threadfence();
+      const unsigned old = atomicAdd(A.counter, 1u);
+      flag = (old + 1u == (unsigned)G) ? 1 : 0;
+    }
+    __syncthreads();
+    if (flag) {
+      __threadfence();
+      double a = 0.0;
+      for (int k = threadIdx.x; k < G; k += blockDim.x)
+        a += __hip_atomic_load(&A.part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      __syncthreads();
+      if (lane == 0) sh[wave] = a;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < nwaves; ++w) t += sh[w];
+        A.alpha[s.k] = t;
+        A.sc->alpha_k = t;
+        *A.counter = 0u;
+        __threadfence();
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static int lds_limit_bytes() { return 160 * 1024; }
+
+static size_t up_lds_bytes(int dimup, int C, size_t sz) {
+  const size_t head = 256 * sz;
+  return head + (16 - head % 16) % 16 + (size_t)dimup * C * sz;
+}
+
+bool two_pass_ok(const Sector& s, size_t elem) {
+  return s.sp[0].packed_ok && s.sp[1].packed_ok && up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes();
+}
+
+TwoPassView two_pass_view(const Sector& s) {
+  TwoPassView v{};
+  v.dimup = s.sp[0].dim;
+  v.dimdw = s.sp[1].dim;
+  v.kup = s.sp[0].kmax;
+  v.kdw = s.sp[1].kmax;
+  v.map_up = s.sp[0].map;
+  v.map_dw = s.sp[1].map;
+  v.pk_up = s.sp[0].packed;
+  v.pk_dw = s.sp[1].packed;
+  v.dg_up = s.sp[0].diag;
+  v.dg_dw = s.sp[1].diag;
+  return v;
+}
+
+template <typename T, int C>
+static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, const T* qa, const T* qb, T* tout,
+                        const LzScalars* sc, int grid, int threads, hipStream_t st) {
+  const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
+  static bool attr = false;
+  if (!attr) {
+    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                lds_limit_bytes()));
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pass_up<T, C>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout, sc);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, const T* qa, const T* qb, T* tout,
+                    const LzScalars* sc, const TwoPassCfg& cfg, hipStream_t st) {
+  const int npan = (V.dimdw + cfg.C - 1) / cfg.C;
+  const int grid = cfg.grid_up < npan ? cfg.grid_up : npan;
+  switch (cfg.C) {
+    case 1: launch_up_c<T, 1>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
+    case 2: launch_up_c<T, 2>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
+    case 4: launch_up_c<T, 4>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
+    default: throw Error("launch_pass_up: unsupported panel width");
+  }
+}
+
+template <typename T, int MODE>
+static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA& A,
+                        const TwoPassCfg& cfg, hipStream_t st) {
+  if (cfg.R == 32)
+    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), 0, st, V, lut, qa, qb,
+                       tin, out, A);
+  else if (cfg.R == 64)
+    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), 0, st, V, lut, qa, qb,
+                       tin, out, A);
+  else
+    throw Error("launch_pass_dw: unsupported panel height");
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA* A,
+                    const TwoPassCfg& cfg, hipStream_t st) {
+  if (A) launch_dw_r<T, 1>(V, lut, qa, qb, tin, out, *A, cfg, st);
+  else launch_dw_r<T, 0>(V, lut, qa, qb, tin, out, LzA{}, cfg, st);
+}
+
+template void launch_pass_up<double>(const TwoPassView&, const InterParams&, const double*, const double*,
+                                     const double*, double*, const LzScalars*, const TwoPassCfg&, hipStream_t);
+template void launch_pass_up<float>(const TwoPassView&, const InterParams&, const float*, const float*, const float*,
+                                    float*, const LzScalars*, const TwoPassCfg&, hipStream_t);
+template void launch_pass_dw<double>(const TwoPassView&, const double*, double*, double*, const double*, double*,
+                                     const LzA*, const TwoPassCfg&, hipStream_t);
+template void launch_pass_dw<float>(const TwoPassView&, const float*, float*, float*, const float*, float*,
+                                    const LzA*, const TwoPassCfg&, hipStream_t);
+
+}  // namespace edl

@@ -80,7 +80,7 @@ __global__ void k_hop_count(const int32_t* map, int dim, const HopPair* pairs, i
 }
 
 __global__ void k_hop_fill(const int32_t* map, int dim, const HopPair* pairs, int npairs, int kmax,
-                           int32_t* nbr, double* coef64, float* coef32) {
+                           int32_t* nbr, double* coef64, float* coef32, uint32_t* packed) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= dim) return;
   uint32_t s = (uint32_t)map[a];
@@ -94,9 +94,11 @@ __global__ void k_hop_fill(const int32_t* map, int dim, const HopPair* pairs, in
       double sg = (__popc(s & between) & 1) ? -1.0 : 1.0;
       double v = pairs[k].h * sg;
       size_t o = (size_t)c * dim + a;
-      nbr[o] = rank_of(src);
+      const int32_t r = rank_of(src);
+      nbr[o] = r;
       coef64[o] = v;
       coef32[o] = (float)v;
+      if (packed) packed[o] = ((uint32_t)r & 0xFFFFFFu) | ((uint32_t)(2 * k + (sg < 0.0 ? 1 : 0)) << 24);
       ++c;
     }
   }
@@ -105,6 +107,7 @@ __global__ void k_hop_fill(const int32_t* map, int dim, const HopPair* pairs, in
     nbr[o] = a;
     coef64[o] = 0.0;
     coef32[o] = 0.0f;
+    if (packed) packed[o] = ((uint32_t)a & 0xFFFFFFu) | (255u << 24);
   }
 }
 
@@ -179,8 +182,22 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
   EDL_HIP(hipMalloc(&t.nbr, sizeof(int32_t) * n));
   EDL_HIP(hipMalloc(&t.coef64, sizeof(double) * n));
   EDL_HIP(hipMalloc(&t.coef32, sizeof(float) * n));
+  t.packed_ok = (np <= 127) && (t.dim < (1 << 24));
+  if (t.packed_ok) {
+    EDL_HIP(hipMalloc(&t.packed, sizeof(uint32_t) * n));
+    for (int k = 0; k < 256; ++k) t.lut64[k] = 0.0;
+    for (int k = 0; k < np; ++k) {
+      t.lut64[2 * k] = pairs[k].h;
+      t.lut64[2 * k + 1] = -pairs[k].h;
+    }
+    for (int k = 0; k < 256; ++k) t.lut32[k] = (float)t.lut64[k];
+    EDL_HIP(hipMalloc(&t.d_lut64, sizeof(t.lut64)));
+    EDL_HIP(hipMalloc(&t.d_lut32, sizeof(t.lut32)));
+    EDL_HIP(hipMemcpyAsync(t.d_lut64, t.lut64, sizeof(t.lut64), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(t.d_lut32, t.lut32, sizeof(t.lut32), hipMemcpyHostToDevice, st));
+  }
   hipLaunchKernelGGL(k_hop_fill, dim3(blocks_for(t.dim)), dim3(kBlock), 0, st, t.map, t.dim, d_pairs, np, kmax,
-                     t.nbr, t.coef64, t.coef32);
+                     t.nbr, t.coef64, t.coef32, t.packed);
   EDL_HIP(hipGetLastError());
 
   // one-spin diagonal (HxV_local.f90:16-18, 47-60, 63-91)
@@ -241,6 +258,9 @@ void free_sector(Sector& s) {
     if (t.coef64) (void)hipFree(t.coef64);
     if (t.coef32) (void)hipFree(t.coef32);
     if (t.diag) (void)hipFree(t.diag);
+    if (t.packed) (void)hipFree(t.packed);
+    if (t.d_lut64) (void)hipFree(t.d_lut64);
+    if (t.d_lut32) (void)hipFree(t.d_lut32);
     t = SpinTable{};
   }
 }

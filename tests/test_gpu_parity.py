@@ -92,6 +92,9 @@ CASES = [
     ("nspin2", lambda: (np.stack([lattice.disordered_rect(2, 3, seed=5)[0], lattice.disordered_rect(2, 3, seed=6)[0]]),
                         [2.5], {}), (2, 4)),
     ("two_orbital", lambda: (_two_orb(), [3.0, 2.0], dict(Nsites=[2, 2], Ust=1.5, Jh=0.4, xmu=0.1)), (2, 2)),
+    # DimUp = C(17,8) = 24310: an fp64 column (194 KB) exceeds LDS, so fp64
+    # takes the single-pass gather kernel and fp32 the two-pass kernels
+    ("ring17_wide_columns", lambda: (lattice.ring(17), [3.0], {}), (8, 1)),
 ]
 
 
@@ -195,6 +198,43 @@ def _gz(poles, weights, z):
     return (weights[None, :] / (z[:, None] - poles[None, :])).sum(1)
 
 
+def _ritz(a, b, n2, e0, isign):
+    """Poles, weights and the last eigenvector component of each Ritz pair."""
+    T = np.diag(a) + np.diag(b[1:], 1) + np.diag(b[1:], -1)
+    w, Z = np.linalg.eigh(T)
+    return isign * (w - e0), n2 * Z[0] ** 2, np.abs(Z[-1])
+
+
+def _assert_leading(a, b, ao, bo, k, tol):
+    """The first k Lanczos coefficients fix the first 2k spectral moments;
+    past them three-term Lanczos without reorthogonalisation amplifies
+    rounding differences (ghost eigenvalues), so two correct
+    implementations agree only on the leading part."""
+    k = min(k, len(a), len(ao))
+    np.testing.assert_allclose(a[:k], ao[:k], rtol=tol, atol=tol)
+    np.testing.assert_allclose(b[1:k], bo[1:k], rtol=tol, atol=tol)
+
+
+def _assert_converged_poles_match(gpu, ora, tol, window=1e-6, conv_tol=1e-10):
+    """Every converged pole of `gpu` (weight > 1e-10 and last Ritz-vector
+    component < 1e-10, i.e. residual ~ 0) matches the oracle: the weights
+    summed over a window around it agree within tol and the weighted mean
+    pole positions within tol.  Ghost copies of one eigenvalue, which
+    three-term Lanczos without reorthogonalisation produces, fall in the
+    same window and are summed."""
+    p, w, zl = gpu
+    po, wo, _ = ora
+    conv = (w > 1e-10) & (zl < conv_tol)
+    assert conv.sum() >= 1
+    for pj in p[conv]:
+        sg = np.abs(p - pj) < window
+        so = np.abs(po - pj) < window
+        wg, wo_ = w[sg].sum(), wo[so].sum()
+        assert abs(wg - wo_) < tol, (pj, wg, wo_)
+        if wg > 1e-6:
+            assert abs((p[sg] * w[sg]).sum() / wg - (po[so] * wo[so]).sum() / wo_) < tol * max(1.0, 1e-6 / wg)
+
+
 def test_tridiag_seed_vs_oracle(ed, orc):
     m = _use(ed, orc, lattice.ring(8), [4.0])
     isec = ed.sector_index(5, 4)
@@ -204,15 +244,20 @@ def test_tridiag_seed_vs_oracle(ed, orc):
     assert abs(n2 - seed @ seed) < 1e-10 * n2
     Hc = orc.hamiltonian_csr(m, 5, 4)
     ao, bo, _ = orc.lanczos_tridiag(Hc, seed, 100)
-    np.testing.assert_allclose(a[:15], ao[:15], rtol=1e-10, atol=1e-10)
-    z = np.linspace(-8, 8, 81) + 0.05j
+    # A random seed spreads over the whole spectrum: past ~40 steps the two
+    # recurrences lose orthogonality differently (rounding-order chaos), so
+    # the coefficients are compared where they are determined, and G(z) at a
+    # broadening that only the leading moments control.
+    np.testing.assert_allclose(a[:40], ao[:40], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(b[1:40], bo[1:40], rtol=1e-8, atol=1e-8)
+    z = np.linspace(-8, 8, 81) + 1.0j
     p, w = ed.lanczos_poles(a, b, n2, -3.0, 1)
     po, wo = orc.gf_poles(ao, bo, n2, -3.0, 1)
     np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8 * n2)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-8), ("fp32", 2e-4)])
-def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol):
+@pytest.mark.parametrize("prec,tol,conv_tol,lead", [("fp64", 1e-8, 1e-10, 1e-9), ("fp32", 2e-4, 1e-5, 1e-5)])
+def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead):
     """Config ring8: all diagonal + off-diagonal GF seeds of spin up in one
     batched launch, continued fractions vs the oracle's seeds + Lanczos."""
     m = _use(ed, orc, lattice.ring(8), [4.0])
@@ -221,7 +266,7 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol):
     gs, e0 = r["gs"], r["e0"]
     sites = [1, 2, 5]
     gf, raw = ed.build_gf_electrons(e0, sites, offdiag=True, spins=(1, 2), nlanc=200, prec=prec)
-    z = np.linspace(-6, 6, 121) + 0.1j
+    z = np.linspace(-6, 6, 121) + 1.0j
     for (ispin, io, jo), chans in gf.items():
         for ch, (op, isign) in enumerate([(2, 1), (1, -1)]):
             d = 1 if op == 2 else -1
@@ -235,6 +280,11 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol):
             p, w = chans[ch]
             np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=tol)
             assert abs(w.sum() - s @ s) < 1e-10
+            k = [kk for kk, key in enumerate(gf) if key == (ispin, io, jo)][0]
+            sidx = 2 * k + ch
+            _assert_leading(raw["alpha"][sidx], raw["beta"][sidx], ao, bo, 30, lead)
+            g = _ritz(raw["alpha"][sidx], raw["beta"][sidx], raw["norm2"][sidx], e0, isign)
+            _assert_converged_poles_match(g, _ritz(ao, bo, s @ s, e0, isign), tol, conv_tol=conv_tol)
 
 
 def test_gf_batched_equals_sequential_rect3x4(ed, orc):
@@ -251,9 +301,10 @@ def test_gf_batched_equals_sequential_rect3x4(ed, orc):
     assert len(seeds) == 156
     rb = ed.tridiag_batched(seeds, 150, "fp64")
     rs = ed.tridiag_batched(seeds[:12], 150, "fp64", group=1)
-    np.testing.assert_allclose(rb["alpha"][:12], rs["alpha"], rtol=1e-9, atol=1e-9)
+    for k in range(12):
+        _assert_leading(rb["alpha"][k], rb["beta"][k], rs["alpha"][k], rs["beta"][k], 30, 1e-9)
     print(f"rect3x4 156 seeds: batched {rb['ms']:.1f} ms, 12 sequential {rs['ms']:.1f} ms", flush=True)
-    z = np.linspace(-10, 10, 101) + 0.1j
+    z = np.linspace(-10, 10, 101) + 1.0j
     csr = {}
     for k in (0, 1, 5):
         op, ispin, io, jo = seeds[k]
@@ -270,6 +321,9 @@ def test_gf_batched_equals_sequential_rect3x4(ed, orc):
         po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], isign)
         p, w = ed.lanczos_poles(rb["alpha"][k], rb["beta"][k], rb["norm2"][k], r["e0"], isign)
         np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)
+        _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, 30, 1e-9)
+        _assert_converged_poles_match(_ritz(rb["alpha"][k], rb["beta"][k], rb["norm2"][k], r["e0"], isign),
+                                      _ritz(ao, bo, s @ s, r["e0"], isign), 1e-8)
 
 
 def test_chi_batched_vs_oracle(ed, orc):
@@ -278,7 +332,7 @@ def test_chi_batched_vs_oracle(ed, orc):
     r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
     chi, raw = ed.build_chi_spin(r["e0"], [1, 3], offdiag=True, nlanc=150)
     Hc = orc.hamiltonian_csr(m, 4, 4)
-    z = np.linspace(0, 8, 81) + 0.1j
+    z = np.linspace(0, 8, 81) + 1.0j
     for (io, jo), (p, w) in chi.items():
         s = orc.seed_sz(8, io, (4, 4), r["gs"]).real
         if jo != io:
@@ -286,3 +340,7 @@ def test_chi_batched_vs_oracle(ed, orc):
         ao, bo, _ = orc.lanczos_tridiag(Hc, s, 150)
         po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], 1)
         np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)
+        k = list(chi).index((io, jo))
+        _assert_leading(raw["alpha"][k], raw["beta"][k], ao, bo, 30, 1e-9)
+        _assert_converged_poles_match(_ritz(raw["alpha"][k], raw["beta"][k], raw["norm2"][k], r["e0"], 1),
+                                      _ritz(ao, bo, s @ s, r["e0"], 1), 1e-8)
