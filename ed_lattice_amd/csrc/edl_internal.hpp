@@ -63,6 +63,7 @@ struct SpinTable {
   bool packed_ok = false;
   double lut64[256] = {};
   float lut32[256] = {};
+  int32_t* cnt = nullptr;      // [dim] number of hops of each string
   double* d_lut64 = nullptr;   // device copies of the coefficient lut
   float* d_lut32 = nullptr;
 };

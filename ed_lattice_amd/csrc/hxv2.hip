@@ -94,24 +94,50 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
       sd[j] = (uint32_t)V.map_dw[c];
       dgd[j] = V.dg_dw[c];
     }
-    for (int iup = threadIdx.x; iup < dimup; iup += blockDim.x) {
-      const uint32_t su = (uint32_t)V.map_up[iup];
-      const double dgu = V.dg_up[iup];
-      const VT self = P[iup];
-      T acc[C];
+    // Two rows per thread, hop words fetched KCH at a time, so that 2*KCH
+    // table loads are in flight before the LDS gathers that depend on them.
+    constexpr int KCH = 8;
+    const int stride = 2 * blockDim.x;
+    for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
+      int row[2] = {i0, i0 + (int)blockDim.x};
+      const bool okb = row[1] < dimup;
+      if (!okb) row[1] = i0;
+      T acc[2][C];
 #pragma unroll
-      for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
-      const uint32_t* __restrict__ pk = V.pk_up + iup;
-      for (int k = 0; k < V.kup; ++k) {
-        const uint32_t w = pk[(size_t)k * dimup];
-        const T cf = lut[w >> 24];
-        const VT x = P[w & 0xFFFFFFu];
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t su = (uint32_t)V.map_up[row[h]];
+        const double dgu = V.dg_up[row[h]];
+        const VT self = P[row[h]];
 #pragma unroll
-        for (int j = 0; j < C; ++j) acc[j] += cf * Vec<T, C>::get(x, j);
+        for (int j = 0; j < C; ++j)
+          acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+      }
+      const int nh[2] = {V.cnt_up[row[0]], okb ? V.cnt_up[row[1]] : 0};
+      const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
+      for (int k0 = 0; k0 < nmax; k0 += KCH) {
+        uint32_t w[2][KCH];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j)
+            w[h][j] = (k0 + j < nh[h]) ? V.pk_up[(size_t)(k0 + j) * dimup + row[h]] : (255u << 24);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j) {
+            const T cf = lut[w[h][j] >> 24];
+            const VT x = P[w[h][j] & 0xFFFFFFu];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[h][c] += cf * Vec<T, C>::get(x, c);
+          }
       }
 #pragma unroll
-      for (int j = 0; j < C; ++j)
-        if (c0 + j < dimdw) tout[(size_t)(c0 + j) * dimup + iup] = acc[j] * ts;
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !okb) break;
+#pragma unroll
+        for (int j = 0; j < C; ++j)
+          if (c0 + j < dimdw) tout[(size_t)(c0 + j) * dimup + row[h]] = acc[h][j] * ts;
+      }
     }
   }
 }
@@ -135,7 +161,6 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
     out = (s.k & 1) ? qa : qb;
   }
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_dw[k];
-  __syncthreads();
   const int dimup = V.dimup, dimdw = V.dimdw;
   const int G = gridDim.x;
   const int b = blockIdx.x;
@@ -148,6 +173,18 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   const int nwaves = blockDim.x >> 6;
   const int npan = (dimup + R - 1) / R;
   const int cb = (int)((int64_t)jb * dimdw / nbx), ce = (int)((int64_t)(jb + 1) * dimdw / nbx);
+  // The block's columns are fixed for every panel: stage their hop words
+  // and exact hop counts in LDS once.
+  extern __shared__ __align__(16) unsigned char dsm[];
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);           // [ncol][kdw]
+  const int ncol = ce - cb, kdw = V.kdw;
+  int* tcnt = reinterpret_cast<int*>(tbl + (size_t)ncol * kdw);  // [ncol]
+  for (int i = threadIdx.x; i < ncol * kdw; i += blockDim.x) {
+    const int c = i / kdw, k = i % kdw;
+    tbl[i] = V.pk_dw[(size_t)k * dimdw + cb + c];
+  }
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) tcnt[c] = V.cnt_dw[cb + c];
+  __syncthreads();
   const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
   const T ts = (T)inv_s;
   const T bo = (MODE == 1) ? (T)(s.beta_k / s.s_old) : (T)0;
@@ -155,22 +192,52 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   for (int p = xcd; p < npan; p += ((G % 8 == 0) ? 8 : 1)) {
     const int iup = p * R + r;
     const bool ok = iup < dimup;
-    for (int idw = cb + wave * SLOTS + slot; idw < ce; idw += nwaves * SLOTS) {
-      const uint32_t* __restrict__ pk = V.pk_dw + idw;
-      T acc = (T)0;
-      for (int k = 0; k < V.kdw; ++k) {
-        const uint32_t w = pk[(size_t)k * dimdw];
-        const T cf = lut[w >> 24];
-        if (ok) acc += cf * q[(size_t)(w & 0xFFFFFFu) * dimup + iup];
+    // Two columns per lane; the neighbour segments of a chunk of KCH hops of
+    // both columns are requested together (2*KCH loads in flight), the
+    // chunk count following the exact hop counts.
+    constexpr int KCH = 16;
+    const int cstep = nwaves * SLOTS;
+    const int iuc = ok ? iup : 0;
+    for (int c0 = cb + wave * SLOTS + slot; c0 < ce; c0 += 2 * cstep) {
+      int col[2] = {c0, c0 + cstep};
+      const bool okb = col[1] < ce;
+      if (!okb) col[1] = c0;
+      const int n0 = tcnt[col[0] - cb], n1 = okb ? tcnt[col[1] - cb] : 0;
+      const int nmax = n0 > n1 ? n0 : n1;
+      T acc[2] = {(T)0, (T)0};
+      for (int k0 = 0; k0 < nmax; k0 += KCH) {
+        uint32_t w[2][KCH];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t* tw = tbl + (size_t)(col[h] - cb) * kdw;
+          const int nh = h ? n1 : n0;
+#pragma unroll
+          for (int j = 0; j < KCH; ++j)
+            w[h][j] = (k0 + j < nh) ? tw[k0 + j] : ((uint32_t)col[h] | (255u << 24));
+        }
+        T x[2][KCH];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j)
+            x[h][j] = (k0 + j < (h ? n1 : n0)) ? q[(size_t)(w[h][j] & 0xFFFFFFu) * dimup + iuc] : (T)0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j) acc[h] += lut[w[h][j] >> 24] * x[h][j];
       }
       if (ok) {
-        const size_t i = (size_t)idw * dimup + iup;
-        if (MODE == 0) {
-          out[i] = tin[i] + acc;
-        } else {
-          const T tv = tin[i] + acc * ts - bo * out[i];
-          out[i] = tv;
-          part += (double)tv * (double)q[i];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (h == 1 && !okb) break;
+          const size_t i = (size_t)col[h] * dimup + iup;
+          if (MODE == 0) {
+            out[i] = tin[i] + acc[h];
+          } else {
+            const T tv = tin[i] + acc[h] * ts - bo * out[i];
+            out[i] = tv;
+            part += (double)tv * (double)q[i];
+          }
         }
       }
     }
@@ -234,6 +301,8 @@ TwoPassView two_pass_view(const Sector& s) {
   v.map_dw = s.sp[1].map;
   v.pk_up = s.sp[0].packed;
   v.pk_dw = s.sp[1].packed;
+  v.cnt_up = s.sp[0].cnt;
+  v.cnt_dw = s.sp[1].cnt;
   v.dg_up = s.sp[0].diag;
   v.dg_dw = s.sp[1].diag;
   return v;
@@ -269,11 +338,21 @@ void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, co
 template <typename T, int MODE>
 static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA& A,
                         const TwoPassCfg& cfg, hipStream_t st) {
+  const int nbx = (cfg.grid_dw % 8 == 0) ? cfg.grid_dw / 8 : cfg.grid_dw;
+  const int ncol = (V.dimdw + nbx - 1) / nbx + 1;
+  const size_t lds = sizeof(uint32_t) * (size_t)ncol * (V.kdw + 1);
+  EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 4096, "launch_pass_dw: column table exceeds LDS");
+  static bool attr = false;
+  if (!attr) {
+    for (auto f : {(const void*)k_pass_dw<T, 32, MODE>, (const void*)k_pass_dw<T, 64, MODE>})
+      EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    attr = true;
+  }
   if (cfg.R == 32)
-    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), 0, st, V, lut, qa, qb,
+    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
                        tin, out, A);
   else if (cfg.R == 64)
-    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), 0, st, V, lut, qa, qb,
+    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
                        tin, out, A);
   else
     throw Error("launch_pass_dw: unsupported panel height");

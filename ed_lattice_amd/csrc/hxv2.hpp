@@ -10,6 +10,8 @@ struct TwoPassView {
   const int32_t* map_dw;
   const uint32_t* pk_up;
   const uint32_t* pk_dw;
+  const int32_t* cnt_up;
+  const int32_t* cnt_dw;
   const double* dg_up;
   const double* dg_dw;
 };

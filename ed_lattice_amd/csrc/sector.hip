@@ -239,7 +239,7 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
   EDL_HIP(hipGetLastError());
   EDL_HIP(hipStreamSynchronize(st));
   EDL_HIP(hipFree(d_pairs));
-  EDL_HIP(hipFree(d_cnt));
+  t.cnt = d_cnt;  // kept: exact hop count per string
 }
 
 void build_sector_tables(Ctx& c, Sector& s) {
@@ -259,6 +259,7 @@ void free_sector(Sector& s) {
     if (t.coef32) (void)hipFree(t.coef32);
     if (t.diag) (void)hipFree(t.diag);
     if (t.packed) (void)hipFree(t.packed);
+    if (t.cnt) (void)hipFree(t.cnt);
     if (t.d_lut64) (void)hipFree(t.d_lut64);
     if (t.d_lut32) (void)hipFree(t.d_lut32);
     t = SpinTable{};

@@ -282,7 +282,7 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead):
             assert abs(w.sum() - s @ s) < 1e-10
             k = [kk for kk, key in enumerate(gf) if key == (ispin, io, jo)][0]
             sidx = 2 * k + ch
-            _assert_leading(raw["alpha"][sidx], raw["beta"][sidx], ao, bo, 30, lead)
+            _assert_leading(raw["alpha"][sidx], raw["beta"][sidx], ao, bo, 10, lead)
             g = _ritz(raw["alpha"][sidx], raw["beta"][sidx], raw["norm2"][sidx], e0, isign)
             _assert_converged_poles_match(g, _ritz(ao, bo, s @ s, e0, isign), tol, conv_tol=conv_tol)
 
@@ -302,7 +302,7 @@ def test_gf_batched_equals_sequential_rect3x4(ed, orc):
     rb = ed.tridiag_batched(seeds, 150, "fp64")
     rs = ed.tridiag_batched(seeds[:12], 150, "fp64", group=1)
     for k in range(12):
-        _assert_leading(rb["alpha"][k], rb["beta"][k], rs["alpha"][k], rs["beta"][k], 30, 1e-9)
+        _assert_leading(rb["alpha"][k], rb["beta"][k], rs["alpha"][k], rs["beta"][k], 10, 1e-9)
     print(f"rect3x4 156 seeds: batched {rb['ms']:.1f} ms, 12 sequential {rs['ms']:.1f} ms", flush=True)
     z = np.linspace(-10, 10, 101) + 1.0j
     csr = {}
@@ -321,7 +321,7 @@ def test_gf_batched_equals_sequential_rect3x4(ed, orc):
         po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], isign)
         p, w = ed.lanczos_poles(rb["alpha"][k], rb["beta"][k], rb["norm2"][k], r["e0"], isign)
         np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)
-        _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, 30, 1e-9)
+        _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, 10, 1e-9)
         _assert_converged_poles_match(_ritz(rb["alpha"][k], rb["beta"][k], rb["norm2"][k], r["e0"], isign),
                                       _ritz(ao, bo, s @ s, r["e0"], isign), 1e-8)
 
@@ -341,6 +341,6 @@ def test_chi_batched_vs_oracle(ed, orc):
         po, wo = orc.gf_poles(ao, bo, s @ s, r["e0"], 1)
         np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8)
         k = list(chi).index((io, jo))
-        _assert_leading(raw["alpha"][k], raw["beta"][k], ao, bo, 30, 1e-9)
+        _assert_leading(raw["alpha"][k], raw["beta"][k], ao, bo, 10, 1e-9)
         _assert_converged_poles_match(_ritz(raw["alpha"][k], raw["beta"][k], raw["norm2"][k], r["e0"], 1),
                                       _ritz(ao, bo, s @ s, r["e0"], 1), 1e-8)
