@@ -1,0 +1,101 @@
+"""The Fortran host driver (ed_lattice_amd/fortran/edlat_gpu.f90).
+
+CPU: it is built, resolves libedlat_hip.so, parses the reference input and
+Hij files, and stops with the library's message when no GPU is visible.
+GPU: on the ring8 workload (8-site ring, U = 4, (4,4), GF_FLAG) its E0,
+densities and G(iw) equal the oracle's."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ed_lattice_amd import lattice
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ed_lattice_amd", "fortran", "_build", "edlat_gpu")
+
+INPUT = """NORB=1
+NSITES=8          ! sites
+NSPIN=1
+ULOC=4.0
+XMU=0.0
+HFMODE=T
+ED_FILLING=0
+ED_SPARSE_H=F
+ED_METHOD=lanczos
+LANC_METHOD=lanczos
+LANC_NSTATES_SECTOR=1
+LANC_NITER=200
+LANC_NGFITER=200
+GF_FLAG=T
+OFFDIAG_GF_FLAG={offdiag}
+CHISPIN_FLAG=T
+LMATS=400
+LREAL=400
+HIJ_FILE=hij.in
+SECTOR_NUP=4
+SECTOR_NDW=4
+ED_PRECISION={prec}
+"""
+
+
+def _prepare(tmp_path, offdiag="F", prec="fp64"):
+    (tmp_path / "inputED.conf").write_text(INPUT.format(offdiag=offdiag, prec=prec))
+    lattice.write_hij(str(tmp_path / "hij.in"), lattice.ring(8))
+
+
+def test_driver_built_and_linked():
+    assert os.path.exists(EXE), "build() compiles the Fortran host"
+    out = subprocess.run(["ldd", EXE], capture_output=True, text=True).stdout
+    assert "libedlat_hip.so" in out and "not found" not in out.split("libedlat_hip.so")[1].split("\n")[0]
+
+
+def test_driver_stops_cleanly_without_gpu(tmp_path):
+    import ctypes
+
+    from ed_lattice_amd import _lib
+
+    h = ctypes.c_void_p()
+    lib = _lib.load()
+    if lib.edl_ctx_create(ctypes.byref(h), 0) == 0:
+        lib.edl_ctx_destroy(h)
+        pytest.skip("a GPU is visible")
+    _prepare(tmp_path)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "edl_ctx_create ERROR" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_driver_ring8_vs_oracle(tmp_path, orc, prec):
+    _prepare(tmp_path, offdiag="T", prec=prec)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    m = orc.Model(lattice.ring(8), [4.0])
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    w, V = np.linalg.eigh(Hc.toarray().real)
+    egs = float((tmp_path / "egs.ed").read_text().split()[0])
+    assert abs(egs - w[0]) < 1e-10 * abs(w[0])
+    gs = V[:, 0]
+    obs = orc.observables(8, 4, 4, gs)
+    dens = np.array((tmp_path / "dens.ed").read_text().split(), float)
+    docc = np.array((tmp_path / "docc.ed").read_text().split(), float)
+    np.testing.assert_allclose(dens, obs["dens_up"] + obs["dens_dw"], atol=1e-9)
+    np.testing.assert_allclose(docc, obs["docc"], atol=1e-9)
+    # G_11(iw): particle + hole continued fractions of the oracle
+    g = np.loadtxt(tmp_path / "G_i0001j0001_l1m1_s1_iw.ed")
+    wm = g[:, 0]
+    gf = g[:, 2] + 1j * g[:, 1]
+    z = 1j * wm
+    ref = np.zeros_like(z)
+    for op, isign, sec in [(2, 1, (5, 4)), (1, -1, (3, 4))]:
+        s = orc.seed_c_cdg(8, op, 1, 1, (4, 4), sec, gs).real
+        a, b, n = orc.lanczos_tridiag(orc.hamiltonian_csr(m, *sec), s, 200)
+        p, wt = orc.gf_poles(a[:n], b[:n], s @ s, w[0], isign)
+        ref += (wt[None, :] / (z[:, None] - p[None, :])).sum(1)
+    np.testing.assert_allclose(gf, ref, atol=1e-6)
+    assert (tmp_path / "gfmatrix.restart").exists()
+    assert (tmp_path / "spinChi_i0001_l1_tau.ed").exists()
