@@ -53,6 +53,35 @@ def cpu_baseline(H, U, nup, ndw, dim, target_s=15.0):
                        f"{full_s:.1f} s per full H x v")
 
 
+def seeds_benchmark(ed, nseq=24):
+    """3x4 PBC, U = 6, ground state (6,6): the 2 x 78 spin-up GF seeds (i <= j
+    pairs, particle in (7,6) and hole in (5,6), dims 731 808), 150 Lanczos
+    steps each, in one batched launch per step vs one seed at a time (a
+    sample of `nseq` seeds, scaled to all 156)."""
+    from ed_lattice_amd import lattice
+
+    ed.set_hamiltonian(lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=False)
+    seeds = []
+    for i in range(1, 13):
+        for j in range(i, 13):
+            jo = 0 if i == j else j
+            seeds += [("cdg", 1, i, jo), ("c", 1, i, jo)]
+    out = {"workload": "3x4 PBC U=6, GS (6,6); 156 spin-up GF seeds (78 i<=j pairs x particle/hole) in "
+                       "(7,6)/(5,6), dim 731808, 150 Lanczos steps each, fp64"}
+    ed.tridiag_batched(seeds[:2], 150, "fp64")  # tune + warm-up
+    rb = ed.tridiag_batched(seeds, 150, "fp64")
+    rs = ed.tridiag_batched(seeds[:nseq], 150, "fp64", group=1)
+    seq_ms = rs["ms"] * len(seeds) / nseq
+    out.update(batched_ms=rb["ms"], sequential_ms_est=seq_ms, sequential_sample=nseq,
+               speedup=seq_ms / rb["ms"], hxv_batched=rb["hxv"],
+               hxv_per_s_batched=rb["hxv"] / (rb["ms"] * 1e-3))
+    rb32 = ed.tridiag_batched(seeds, 150, "fp32")
+    out.update(batched_ms_fp32=rb32["ms"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-seeds", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -119,6 +149,8 @@ def main():
             "rel_dE0_mixed_vs_fp64": abs(rmx["e0"] - r64["e0"]) / abs(r64["e0"]),
             "rel_dE0_fp32_only_vs_fp64": abs(rmx["e0_fp32"] - r64["e0"]) / abs(r64["e0"]),
         }
+    if not args.no_seeds:
+        out["gf_seeds"] = seeds_benchmark(ed)
     ed.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(H, U, nup, ndw, dim)

@@ -973,7 +973,19 @@ extern "C" int edl_tridiag_batched(edl_ctx* ctx, int nseeds, const edl_seed_desc
     if (targets[s] != 0) active.push_back(s);
   }
   for (int s : active) get_sector(c, targets[s]);
-  const int g = (group <= 0) ? (int)active.size() : group;
+  // group = 0: as many chains per launch as fit in 70% of the free HBM
+  // (each chain holds two stored vectors; one fp64 seed buffer is shared).
+  int g = (group <= 0) ? (int)active.size() : group;
+  if (group <= 0 && !active.empty()) {
+    size_t free_b = 0, total_b = 0;
+    EDL_HIP(hipMemGetInfo(&free_b, &total_b));
+    int64_t maxdim = 0;
+    for (int s : active) maxdim = std::max<int64_t>(maxdim, get_sector(c, targets[s]).dim);
+    const double per_chain = 2.0 * (double)maxdim * (prec == EDL_FP64 ? 8.0 : 4.0);
+    const double budget = 0.7 * (double)free_b - 8.0 * (double)maxdim;
+    const int fit = (int)std::max(1.0, budget / per_chain);
+    g = std::min<int>(g, fit);
+  }
   EventTimer tm(c.stream);
   int64_t h = 0;
   for (size_t b = 0; b < active.size(); b += g) {

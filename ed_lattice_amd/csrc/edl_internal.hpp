@@ -15,7 +15,6 @@ constexpr int kMaxNs = 31;      // spin strings are held in 32-bit words
 constexpr int kMaxOrb = 8;
 constexpr int kMaxPairs = 64;   // same-site inter-orbital pairs (Ust/Jh terms)
 constexpr int kBlock = 256;     // threads per block of the streaming kernels
-constexpr int kRedBlocks = 2048;  // persistent grid of the reduction kernels
 
 struct Error : std::exception {
   std::string msg;
@@ -74,20 +73,6 @@ struct Sector {
   SpinTable sp[2];
 };
 
-// Kernel-side view of a sector, passed by value.
-struct SectorView {
-  int dimup, dimdw;
-  int kup, kdw;
-  const int32_t* map_up;
-  const int32_t* map_dw;
-  const int32_t* nbr_up;
-  const int32_t* nbr_dw;
-  const void* coef_up;   // double* or float* as the kernel's scalar type
-  const void* coef_dw;
-  const double* dg_up;
-  const double* dg_dw;
-};
-
 struct Model {
   bool set = false;
   int Ns = 0, Nspin = 1, Norb = 1;
@@ -107,7 +92,8 @@ void free_sector(Sector& s);
 void rank_strings(Ctx& c, const SpinTable& t, int Ns, int64_t n, const int32_t* d_states, int32_t* d_out);
 void init_binomials();
 
-// hxv.hip / lanczos.hip launchers are declared in their own headers.
+// The launchers of hxv.hip / hxv2.hip / seeds.hip are declared in chains.hpp,
+// hxv2.hpp and seeds.hpp.
 
 struct Ctx {
   int device = 0;

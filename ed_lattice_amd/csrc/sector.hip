@@ -65,7 +65,6 @@ struct HopPair {
   uint8_t p, q;
   double h;
 };
-constexpr int kMaxHopPairs = kMaxNs * kMaxNs;
 
 __global__ void k_hop_count(const int32_t* map, int dim, const HopPair* pairs, int npairs, int32_t* cnt) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;

@@ -118,8 +118,9 @@ int edl_tridiag(edl_ctx* ctx, int isector, int prec, const double* seed, int nla
  * sequential per-seed loops of GF_ELECTRONS.f90:202-405 and
  * CHI_SPIN_ELECTRONS.f90:189-293.  Outputs: alpha, beta [nseeds*nlanc]
  * (row s), norm2[nseeds], nsteps[nseeds], isector_out[nseeds] (0 where the
- * target sector does not exist).  group = seeds advanced per launch (0 = all
- * in one launch; 1 = one seed at a time, for comparison). */
+ * target sector does not exist).  group = seeds advanced per launch (0 = as
+ * many as fit in 70% of the free HBM, i.e. all of them unless the vectors are
+ * very large; 1 = one seed at a time, for comparison). */
 typedef struct {
   int op, ispin, io, jo;
 } edl_seed_desc;
