@@ -193,7 +193,7 @@ static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, hipStream_t st)
   } else {
     Schedule w;
     w.block_threads = 256;
-    w.rows_per_thread = 1;
+
     w.grid = c.num_cu * 8;
     hxv_operator<T>(c, s, v, out, st, w);
   }
@@ -273,20 +273,16 @@ static Schedule autotune(Ctx& c, const std::vector<ChainDev>& chains, hipStream_
   for (auto& cd : chains) key.dims += (int64_t)cd.dimup * cd.dimdw;
   auto it = tune_cache().find(key);
   if (it != tune_cache().end()) return it->second;
-  const int threads[] = {256, 512, 1024};
-  const int rpt[] = {1, 2};
-  const int gmul[] = {4, 8, 16};
+  const int threads[] = {64, 128, 256, 512, 1024};
+  const int gmul[] = {2, 4, 8, 16};
   Schedule best;
   best.block_threads = 256;
-  best.rows_per_thread = 1;
   best.grid = c.num_cu * 8;
   double best_ms = 1e300;
   for (int th : threads)
-    for (int r : rpt)
-      for (int gm : gmul) {
+    for (int gm : gmul) {
         Schedule w;
         w.block_threads = th;
-        w.rows_per_thread = r;
         w.grid = c.num_cu * gm;
         ChainSet<T> cs;
         cs.h = chains;
@@ -424,7 +420,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     tps.A.part = tp_part.as<double>();
     tps.A.counter = tp_counter.as<unsigned>();
     sch.block_threads = 256;
-    sch.rows_per_thread = 4;
+
     sch.grid = c.num_cu * 8;
   }
   {
@@ -686,7 +682,7 @@ extern "C" int edl_sector_rank(edl_ctx* ctx, int isector, int ispin, int64_t n, 
 static Schedule operator_schedule(Ctx& c) {
   Schedule w;
   w.block_threads = 256;
-  w.rows_per_thread = 1;
+
   w.grid = c.num_cu * 8;
   return w;
 }

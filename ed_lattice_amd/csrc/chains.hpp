@@ -21,42 +21,43 @@ struct LzScalars {
   double norm2;  // squared norm of the seed before normalisation
 };
 
+// One chain as the kernels see it.  Its work is cut into tiles of one
+// column (idw) each; tile partial sums are reduced in column order by the
+// block that completes the last tile, so results do not depend on which
+// block ran which tile.
 struct ChainDev {
   // sector view
   int dimup, dimdw, kup, kdw;
   const int32_t* map_up;
   const int32_t* map_dw;
-  const int32_t* nbr_up;
-  const int32_t* nbr_dw;
-  const void* cu;   // hop coefficients, scalar type of the chain
+  const int32_t* nbr_up;   // [kup][dimup]
+  const int32_t* nbr_dw;   // [kdw][dimdw]
+  const int32_t* cnt_up;   // exact hop counts
+  const int32_t* cnt_dw;
+  const void* cu;          // hop coefficients, scalar type of the chain
   const void* cd;
   const double* dg_up;
   const double* dg_dw;
   // schedule
-  int ntr;          // row tiles per column
-  int rows_per_tile;
-  int64_t tile0;    // first global tile of the chain
-  int64_t ntiles;   // ntr * dimdw
-  int lb_first;     // first logical block touching the chain
-  int span;         // number of logical blocks touching the chain
+  int64_t tile0;           // first global tile of the chain
+  int ntiles;              // = dimdw
   // state
-  void* q0;         // stored vectors; q_k lives in q[k & 1]
+  void* q0;                // stored vectors; q_k lives in q[k & 1]
   void* q1;
-  double* x;        // fp64 eigenvector accumulator (replay), or null
-  const double* ccoef;  // replay coefficients c_k, or null
-  double* part;     // [span] block partial sums
-  unsigned* counter;
+  double* x;               // fp64 eigenvector accumulator (replay), or null
+  const double* ccoef;     // replay coefficients c_k, or null
+  double* part;            // [ntiles] tile partial sums
+  unsigned* counter;       // tiles completed in the current launch
   LzScalars* sc;
-  double* alpha;    // [nmax]
-  double* beta;     // [nmax]; beta[0] = 0
+  double* alpha;           // [nmax]
+  double* beta;            // [nmax]; beta[0] = 0
   int nmax;
-  double brk_tol;   // relative breakdown threshold on beta
+  double brk_tol;          // relative breakdown threshold on beta
 };
 
 struct Schedule {
   int block_threads = kBlock;
-  int rows_per_thread = 1;
-  int grid = 0;        // logical blocks (multiple of 8 when >= 8)
+  int grid = 0;            // blocks; a multiple of 8 uses the XCD-synchronous order
   int64_t total_tiles = 0;
 };
 
@@ -82,7 +83,7 @@ void lz_phase_a(Ctx& c, ChainSet<T>& cs, hipStream_t st);
 template <typename T>
 void lz_phase_b(Ctx& c, ChainSet<T>& cs, hipStream_t st);
 
-// Plain operator out = H v on one sector (directMatVec).
+// Plain operator out = H v on one sector (directMatVec), single pass.
 template <typename T>
 void hxv_operator(Ctx& c, const Sector& s, const T* v, T* out, hipStream_t st, const Schedule& want);
 

@@ -1,5 +1,6 @@
-// On-the-fly H x v on the up (x) down factorised basis, and the two fused
-// phases of a Lanczos step.
+// Single-pass on-the-fly H x v over a list of Lanczos chains (the batched
+// GF/chi seeds, and ground states whose columns do not fit in LDS), and the
+// two phases of a Lanczos step.
 //
 // A vector of sector (Nup, Ndw) is the column-major matrix V(iup, idw),
 // element i = iup + idw*DimUp (SECTOR.f90:443-480, 0-based here).  In
@@ -9,16 +10,15 @@
 //                  + sum_k cu_k(iup) v(nu_k(iup), idw)      (HxV_up.f90)
 //                  + sum_k cd_k(idw) v(iup, nd_k(idw))      (HxV_dw.f90)
 //
-// with one writer per element, so no atomics and no write races.  The up
-// hops gather inside one column; the down hops read whole neighbour
-// columns, coalesced across iup.  The neighbour lists of the column are
-// uniform over a tile and are read once per tile.
+// with one writer per element, so no atomics on data and no write races.
 //
-// Work is cut into tiles (one column, a range of rows) and handed to a
-// persistent grid of logical blocks, each taking a contiguous tile range.
-// Physical block b runs logical block (b % 8) * (G / 8) + b / 8 so that the
-// contiguous tile ranges of one XCD cover neighbouring columns and their
-// up-hop gathers stay in that XCD's L2.
+// Work unit: a tile = one column of one chain.  The tiles of all chains
+// form one list.  With a grid of G = 8*nb blocks, XCD x (the hardware sends
+// block b to XCD b % 8) owns the list slice [x*T/8, (x+1)*T/8) and its nb
+// blocks take tiles x0 + j, x0 + j + nb, ...: the blocks of one XCD sweep
+// the same chain together, so the chain's vector (a few MB for the seed
+// sectors) stays in that XCD's L2 and the down-hop neighbour columns are
+// L2 hits.
 #include "chains.hpp"
 
 namespace edl {
@@ -28,7 +28,7 @@ static __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Deterministic block sum (fixed shuffle tree, fixed wave order).
+// Deterministic block sum (fixed shuffle tree, fixed wave order), thread 0.
 static __device__ double block_sum(double v, double* sh) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   v = wave_sum(v);
@@ -38,22 +38,13 @@ static __device__ double block_sum(double v, double* sh) {
   double r = 0.0;
   if (threadIdx.x == 0)
     for (int w = 0; w < nw; ++w) r += sh[w];
-  return r;  // valid in thread 0
-}
-
-static __device__ __forceinline__ int logical_block(int G) {
-  const int b = blockIdx.x;
-  return (G >= 8 && (G % 8) == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  return r;
 }
 
 static __device__ __forceinline__ double inter_term(const InterParams& I, uint32_t su, uint32_t sd) {
   const uint32_t both = su & sd;
   double e = 0.0;
-  if (I.norb == 1) {
-    e = I.U[0] * (double)__popc(both & I.orb_mask[0]);
-  } else {
-    for (int k = 0; k < I.norb; ++k) e += I.U[k] * (double)__popc(both & I.orb_mask[k]);
-  }
+  for (int k = 0; k < I.norb; ++k) e += I.U[k] * (double)__popc(both & I.orb_mask[k]);
   for (int p = 0; p < I.npairs; ++p) {
     const uint32_t a = I.pi[p], b = I.pj[p];
     e += I.Ust * (double)((((su >> a) & (sd >> b)) & 1u) + (((su >> b) & (sd >> a)) & 1u));
@@ -61,25 +52,24 @@ static __device__ __forceinline__ double inter_term(const InterParams& I, uint32
   return e;
 }
 
-// (H q)(iup, idw) in the scalar type T.
-template <typename T>
-static __device__ __forceinline__ T hq_elem(const ChainDev& C, const InterParams& I, const T* __restrict__ q,
-                                            int iup, int idw, size_t col, uint32_t sd, double dgd) {
-  const T* __restrict__ cu = reinterpret_cast<const T*>(C.cu);
-  const T* __restrict__ cd = reinterpret_cast<const T*>(C.cd);
-  const uint32_t su = (uint32_t)C.map_up[iup];
-  const double dg = C.dg_up[iup] + dgd + inter_term(I, su, sd);
-  T acc = (T)dg * q[col + iup];
-  for (int k = 0; k < C.kup; ++k) {
-    const size_t o = (size_t)k * C.dimup + iup;
-    acc += cu[o] * q[col + (size_t)C.nbr_up[o]];
+// Tile walk of one block: XCD-synchronous strided order (see header).
+struct TileWalk {
+  int64_t t, end, step;
+  __device__ TileWalk(int64_t T) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G >= 8 && G % 8 == 0) {
+      const int x = b % 8, j = b / 8;
+      const int64_t x0 = (int64_t)x * T / 8, x1 = (int64_t)(x + 1) * T / 8;
+      t = x0 + j;
+      end = x1;
+      step = G / 8;
+    } else {
+      t = b;
+      end = T;
+      step = G;
+    }
   }
-  for (int k = 0; k < C.kdw; ++k) {
-    const size_t o = (size_t)k * C.dimdw + idw;
-    acc += cd[o] * q[(size_t)C.nbr_dw[o] * C.dimup + iup];
-  }
-  return acc;
-}
+};
 
 static __device__ __forceinline__ int chain_of(const int64_t* __restrict__ tile0, int nchains, int64_t t) {
   int lo = 0, hi = nchains - 1;
@@ -90,15 +80,60 @@ static __device__ __forceinline__ int chain_of(const int64_t* __restrict__ tile0
   return lo;
 }
 
-// Flush the block's partial sum of chain C; returns true in every thread of
-// the block that completed the chain (the last of its `span` blocks).
-static __device__ bool flush_partial(const ChainDev& C, int Lb, double acc, double* sh, int* flag) {
+// (H q)(:, idw) / s for the rows this thread owns in the column, written
+// through `emit(i, h)`.  Up hops gather inside the column (L1/L2 resident
+// while the block works on it), down hops read whole neighbour columns.
+template <typename T, typename Emit>
+static __device__ __forceinline__ void column_hq(const ChainDev& C, const InterParams& I, const T* __restrict__ q,
+                                                 int idw, Emit emit) {
+  const T* __restrict__ cu = reinterpret_cast<const T*>(C.cu);
+  const T* __restrict__ cd = reinterpret_cast<const T*>(C.cd);
+  const size_t col = (size_t)idw * C.dimup;
+  const uint32_t sd = (uint32_t)C.map_dw[idw];
+  const double dgd = C.dg_dw[idw];
+  const int ndw = C.cnt_dw[idw];
+  constexpr int KCH = 8;
+  for (int iup = threadIdx.x; iup < C.dimup; iup += blockDim.x) {
+    const uint32_t su = (uint32_t)C.map_up[iup];
+    T acc = (T)(C.dg_up[iup] + dgd + inter_term(I, su, sd)) * q[col + iup];
+    const int nup = C.cnt_up[iup];
+    for (int k0 = 0; k0 < nup; k0 += KCH) {
+      T x[KCH], c[KCH];
+#pragma unroll
+      for (int j = 0; j < KCH; ++j) {
+        const bool on = k0 + j < nup;
+        const size_t o = (size_t)(k0 + j) * C.dimup + iup;
+        c[j] = on ? cu[o] : (T)0;
+        x[j] = on ? q[col + (size_t)C.nbr_up[o]] : (T)0;
+      }
+#pragma unroll
+      for (int j = 0; j < KCH; ++j) acc += c[j] * x[j];
+    }
+    for (int k0 = 0; k0 < ndw; k0 += KCH) {
+      T x[KCH], c[KCH];
+#pragma unroll
+      for (int j = 0; j < KCH; ++j) {
+        const bool on = k0 + j < ndw;
+        const size_t o = (size_t)(k0 + j) * C.dimdw + idw;
+        c[j] = on ? cd[o] : (T)0;
+        x[j] = on ? q[(size_t)C.nbr_dw[o] * C.dimup + iup] : (T)0;
+      }
+#pragma unroll
+      for (int j = 0; j < KCH; ++j) acc += c[j] * x[j];
+    }
+    emit(col + iup, acc);
+  }
+}
+
+// Record a tile's partial sum; returns true in every thread of the block
+// that completed the chain's last tile of this launch.
+static __device__ bool tile_done(const ChainDev& C, int idw, double acc, double* sh, int* flag) {
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) {
-    C.part[Lb - C.lb_first] = s;
+    C.part[idw] = s;
     __threadfence();
     const unsigned old = atomicAdd(C.counter, 1u);
-    *flag = (old + 1u == (unsigned)C.span) ? 1 : 0;
+    *flag = (old + 1u == (unsigned)C.ntiles) ? 1 : 0;
   }
   __syncthreads();
   const bool last = *flag != 0;
@@ -106,12 +141,13 @@ static __device__ bool flush_partial(const ChainDev& C, int Lb, double acc, doub
   return last;
 }
 
-// Sum of the chain's partials in slot order (deterministic), thread 0.
-static __device__ double gather_partials(const ChainDev& C, double* sh) {
+// Sum of the chain's tile partials in column order (deterministic), thread 0.
+static __device__ double chain_total(const ChainDev& C, double* sh) {
   __threadfence();
   double v = 0.0;
-  for (int s = threadIdx.x; s < C.span; s += blockDim.x)
-    v += __hip_atomic_load(&C.part[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int per = (C.ntiles + blockDim.x - 1) / blockDim.x;  // contiguous slice per thread
+  const int a = threadIdx.x * per, b = min(a + per, C.ntiles);
+  for (int s = a; s < b; ++s) v += __hip_atomic_load(&C.part[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return block_sum(v, sh);
 }
 
@@ -120,40 +156,27 @@ static __device__ double gather_partials(const ChainDev& C, double* sh) {
 // alpha_k = <t, v_k>.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_lz_a(const ChainDev* __restrict__ chains, const int64_t* __restrict__ tile0,
-                                               int nchains, InterParams I, int G, int64_t Tall) {
+                                               int nchains, InterParams I, int64_t Tall) {
   __shared__ double sh[16];
   __shared__ int flag;
-  const int Lb = logical_block(G);
-  const int64_t tb = (int64_t)Lb * Tall / G, te = (int64_t)(Lb + 1) * Tall / G;
-  int c = chain_of(tile0, nchains, tb);
-  for (int64_t t = tb; t < te; ++c) {
+  for (TileWalk w(Tall); w.t < w.end; w.t += w.step) {
+    const int c = chain_of(tile0, nchains, w.t);
     const ChainDev& C = chains[c];
-    const int64_t tstop = te < tile0[c + 1] ? te : tile0[c + 1];
+    const int idw = (int)(w.t - C.tile0);
     const LzScalars sc = *C.sc;
-    if (sc.stopped) { t = tstop; continue; }
+    if (sc.stopped) continue;
     const T* __restrict__ qk = reinterpret_cast<const T*>((sc.k & 1) ? C.q1 : C.q0);
     T* __restrict__ qo = reinterpret_cast<T*>((sc.k & 1) ? C.q0 : C.q1);
     const double inv_s = 1.0 / sc.s_cur;
-    const T ts = (T)inv_s, tb_o = (T)(sc.beta_k / sc.s_old);
+    const T ts = (T)inv_s, tb = (T)(sc.beta_k / sc.s_old);
     double acc = 0.0;
-    for (; t < tstop; ++t) {
-      const int64_t lt = t - C.tile0;
-      const int idw = (int)(lt / C.ntr), rt = (int)(lt % C.ntr);
-      const int r0 = rt * C.rows_per_tile;
-      const int r1 = min(r0 + C.rows_per_tile, C.dimup);
-      const size_t col = (size_t)idw * C.dimup;
-      const uint32_t sd = (uint32_t)C.map_dw[idw];
-      const double dgd = C.dg_dw[idw];
-      for (int iup = r0 + threadIdx.x; iup < r1; iup += blockDim.x) {
-        const T h = hq_elem<T>(C, I, qk, iup, idw, col, sd, dgd);
-        const size_t i = col + iup;
-        const T tv = h * ts - tb_o * qo[i];
-        qo[i] = tv;
-        acc += (double)tv * (double)qk[i];
-      }
-    }
-    if (flush_partial(C, Lb, acc * inv_s, sh, &flag)) {
-      const double a = gather_partials(C, sh);
+    column_hq<T>(C, I, qk, idw, [&](size_t i, T h) {
+      const T tv = h * ts - tb * qo[i];
+      qo[i] = tv;
+      acc += (double)tv * (double)qk[i];
+    });
+    if (tile_done(C, idw, acc * inv_s, sh, &flag)) {
+      const double a = chain_total(C, sh);
       if (threadIdx.x == 0) {
         C.alpha[sc.k] = a;
         C.sc->alpha_k = a;
@@ -169,41 +192,33 @@ __global__ __launch_bounds__(1024) void k_lz_a(const ChainDev* __restrict__ chai
 // accumulation x += c_k v_k.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_lz_b(const ChainDev* __restrict__ chains, const int64_t* __restrict__ tile0,
-                                               int nchains, int G, int64_t Tall) {
+                                               int nchains, int64_t Tall) {
   __shared__ double sh[16];
   __shared__ int flag;
-  const int Lb = logical_block(G);
-  const int64_t tb = (int64_t)Lb * Tall / G, te = (int64_t)(Lb + 1) * Tall / G;
-  int c = chain_of(tile0, nchains, tb);
-  for (int64_t t = tb; t < te; ++c) {
+  for (TileWalk w(Tall); w.t < w.end; w.t += w.step) {
+    const int c = chain_of(tile0, nchains, w.t);
     const ChainDev& C = chains[c];
-    const int64_t tstop = te < tile0[c + 1] ? te : tile0[c + 1];
+    const int idw = (int)(w.t - C.tile0);
     const LzScalars sc = *C.sc;
-    if (sc.stopped) { t = tstop; continue; }
+    if (sc.stopped) continue;
     const T* __restrict__ qk = reinterpret_cast<const T*>((sc.k & 1) ? C.q1 : C.q0);
     T* __restrict__ qo = reinterpret_cast<T*>((sc.k & 1) ? C.q0 : C.q1);
     const double inv_s = 1.0 / sc.s_cur;
     const T am = (T)(sc.alpha_k * inv_s);
     double* __restrict__ x = C.x;
     const double cx = x ? C.ccoef[sc.k] * inv_s : 0.0;
+    const size_t col = (size_t)idw * C.dimup;
     double acc = 0.0;
-    for (; t < tstop; ++t) {
-      const int64_t lt = t - C.tile0;
-      const int idw = (int)(lt / C.ntr), rt = (int)(lt % C.ntr);
-      const int r0 = rt * C.rows_per_tile;
-      const int r1 = min(r0 + C.rows_per_tile, C.dimup);
-      const size_t col = (size_t)idw * C.dimup;
-      for (int iup = r0 + threadIdx.x; iup < r1; iup += blockDim.x) {
-        const size_t i = col + iup;
-        const T q = qk[i];
-        const T r = qo[i] - am * q;
-        qo[i] = r;
-        acc += (double)r * (double)r;
-        if (x) x[i] += cx * (double)q;
-      }
+    for (int iup = threadIdx.x; iup < C.dimup; iup += blockDim.x) {
+      const size_t i = col + iup;
+      const T qv = qk[i];
+      const T r = qo[i] - am * qv;
+      qo[i] = r;
+      acc += (double)r * (double)r;
+      if (x) x[i] += cx * (double)qv;
     }
-    if (flush_partial(C, Lb, acc, sh, &flag)) {
-      const double s2 = gather_partials(C, sh);
+    if (tile_done(C, idw, acc, sh, &flag)) {
+      const double s2 = chain_total(C, sh);
       if (threadIdx.x == 0) {
         const double b = sqrt(s2);
         LzScalars n = sc;
@@ -230,19 +245,9 @@ __global__ __launch_bounds__(1024) void k_lz_b(const ChainDev* __restrict__ chai
 // -------------------------------------------------------------- operator
 template <typename T>
 __global__ __launch_bounds__(1024) void k_hxv(ChainDev C, InterParams I, const T* __restrict__ v, T* __restrict__ out,
-                                              int G, int64_t Tall) {
-  const int Lb = logical_block(G);
-  const int64_t tb = (int64_t)Lb * Tall / G, te = (int64_t)(Lb + 1) * Tall / G;
-  for (int64_t t = tb; t < te; ++t) {
-    const int idw = (int)(t / C.ntr), rt = (int)(t % C.ntr);
-    const int r0 = rt * C.rows_per_tile;
-    const int r1 = min(r0 + C.rows_per_tile, C.dimup);
-    const size_t col = (size_t)idw * C.dimup;
-    const uint32_t sd = (uint32_t)C.map_dw[idw];
-    const double dgd = C.dg_dw[idw];
-    for (int iup = r0 + threadIdx.x; iup < r1; iup += blockDim.x)
-      out[col + iup] = hq_elem<T>(C, I, v, iup, idw, col, sd, dgd);
-  }
+                                              int64_t Tall) {
+  for (TileWalk w(Tall); w.t < w.end; w.t += w.step)
+    column_hq<T>(C, I, v, (int)w.t, [&](size_t i, T h) { out[i] = h; });
 }
 
 // --------------------------------------------------------------- helpers
@@ -283,7 +288,6 @@ __global__ void k_dot_partial(const double* __restrict__ a, const double* __rest
   const double s = block_sum(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-
 template <typename T>
 __global__ void k_sumsq_partial(const T* __restrict__ a, int64_t n, double* part) {
   __shared__ double sh[16];
@@ -324,18 +328,24 @@ void launch_zero_bytes(void* p, size_t bytes, hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3(grid_for((int64_t)nw)), dim3(kBlock), 0, st, (uint32_t*)p, nw);
   EDL_HIP(hipGetLastError());
 }
+
+static double host_sum_partials(const double* part, int g, hipStream_t st) {
+  std::vector<double> h(g);
+  EDL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
 double device_dot(Ctx& c, const double* a, const double* b, int64_t n, hipStream_t st) {
   const int g = grid_for(n);
   double* part = nullptr;
   EDL_HIP(hipMalloc(&part, sizeof(double) * g));
   hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, st, a, b, n, part);
   EDL_HIP(hipGetLastError());
-  std::vector<double> h(g);
-  EDL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
-  EDL_HIP(hipStreamSynchronize(st));
+  const double s = host_sum_partials(part, g, st);
   EDL_HIP(hipFree(part));
-  double s = 0.0;
-  for (double v : h) s += v;
   return s;
 }
 
@@ -346,12 +356,8 @@ double device_sumsq(const T* a, int64_t n, hipStream_t st) {
   EDL_HIP(hipMalloc(&part, sizeof(double) * g));
   hipLaunchKernelGGL(k_sumsq_partial<T>, dim3(g), dim3(kBlock), 0, st, a, n, part);
   EDL_HIP(hipGetLastError());
-  std::vector<double> h(g);
-  EDL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
-  EDL_HIP(hipStreamSynchronize(st));
+  const double s = host_sum_partials(part, g, st);
   EDL_HIP(hipFree(part));
-  double s = 0.0;
-  for (double v : h) s += v;
   return s;
 }
 template double device_sumsq<double>(const double*, int64_t, hipStream_t);
@@ -367,17 +373,13 @@ void fill_sector_view(const Sector& s, bool fp32, ChainDev& cd) {
   cd.map_dw = s.sp[1].map;
   cd.nbr_up = s.sp[0].nbr;
   cd.nbr_dw = s.sp[1].nbr;
+  cd.cnt_up = s.sp[0].cnt;
+  cd.cnt_dw = s.sp[1].cnt;
   cd.cu = fp32 ? (const void*)s.sp[0].coef32 : (const void*)s.sp[0].coef64;
   cd.cd = fp32 ? (const void*)s.sp[1].coef32 : (const void*)s.sp[1].coef64;
   cd.dg_up = s.sp[0].diag;
   cd.dg_dw = s.sp[1].diag;
-}
-
-static void tile_chain(ChainDev& cd, const Schedule& w) {
-  const int per = w.block_threads * w.rows_per_thread;
-  cd.ntr = (cd.dimup + per - 1) / per;
-  cd.rows_per_tile = (cd.dimup + cd.ntr - 1) / cd.ntr;
-  cd.ntiles = (int64_t)cd.ntr * cd.dimdw;
+  cd.ntiles = cd.dimdw;
 }
 
 static int pick_grid(Ctx& c, int64_t total_tiles, const Schedule& w) {
@@ -385,16 +387,6 @@ static int pick_grid(Ctx& c, int64_t total_tiles, const Schedule& w) {
   if (g > total_tiles) g = total_tiles;
   if (g >= 8) g -= g % 8;
   return (int)(g < 1 ? 1 : g);
-}
-
-// first logical block whose tile range contains tile t
-static int lb_of(int64_t t, int64_t Tall, int G) {
-  int lo = 0, hi = G - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if ((int64_t)mid * Tall / G <= t) lo = mid; else hi = mid - 1;
-  }
-  return lo;
 }
 
 template <typename T>
@@ -407,29 +399,24 @@ void chainset_finalize(Ctx& c, ChainSet<T>& cs, const Schedule& want) {
   cs.sch = want;
   int64_t tot = 0;
   std::vector<int64_t> t0(cs.h.size() + 1);
+  size_t part_total = 0;
   for (size_t k = 0; k < cs.h.size(); ++k) {
-    tile_chain(cs.h[k], want);
+    cs.h[k].ntiles = cs.h[k].dimdw;
     cs.h[k].tile0 = tot;
     t0[k] = tot;
     tot += cs.h[k].ntiles;
+    part_total += (size_t)cs.h[k].ntiles;
   }
   t0[cs.h.size()] = tot;
   cs.sch.total_tiles = tot;
   cs.sch.grid = pick_grid(c, tot, want);
-  const int G = cs.sch.grid;
-  size_t part_total = 0;
-  for (auto& cd : cs.h) {
-    cd.lb_first = lb_of(cd.tile0, tot, G);
-    cd.span = lb_of(cd.tile0 + cd.ntiles - 1, tot, G) - cd.lb_first + 1;
-    part_total += (size_t)cd.span;
-  }
   double* d_part = nullptr;
   EDL_HIP(hipMalloc(&d_part, sizeof(double) * (part_total ? part_total : 1)));
   cs.owned.push_back(d_part);
   size_t off = 0;
   for (auto& cd : cs.h) {
     cd.part = d_part + off;
-    off += (size_t)cd.span;
+    off += (size_t)cd.ntiles;
   }
   EDL_HIP(hipMalloc(&cs.d, sizeof(ChainDev) * cs.h.size()));
   cs.owned.push_back(cs.d);
@@ -442,14 +429,14 @@ void chainset_finalize(Ctx& c, ChainSet<T>& cs, const Schedule& want) {
 template <typename T>
 void lz_phase_a(Ctx& c, ChainSet<T>& cs, hipStream_t st) {
   hipLaunchKernelGGL(k_lz_a<T>, dim3(cs.sch.grid), dim3(cs.sch.block_threads), 0, st, cs.d, cs.d_tile0,
-                     (int)cs.h.size(), inter_of(c), cs.sch.grid, cs.sch.total_tiles);
+                     (int)cs.h.size(), inter_of(c), cs.sch.total_tiles);
   EDL_HIP(hipGetLastError());
 }
 
 template <typename T>
 void lz_phase_b(Ctx& c, ChainSet<T>& cs, hipStream_t st) {
   hipLaunchKernelGGL(k_lz_b<T>, dim3(cs.sch.grid), dim3(cs.sch.block_threads), 0, st, cs.d, cs.d_tile0,
-                     (int)cs.h.size(), cs.sch.grid, cs.sch.total_tiles);
+                     (int)cs.h.size(), cs.sch.total_tiles);
   EDL_HIP(hipGetLastError());
 }
 
@@ -457,9 +444,9 @@ template <typename T>
 void hxv_operator(Ctx& c, const Sector& s, const T* v, T* out, hipStream_t st, const Schedule& want) {
   ChainDev cd{};
   fill_sector_view(s, sizeof(T) == 4, cd);
-  tile_chain(cd, want);
   const int G = pick_grid(c, cd.ntiles, want);
-  hipLaunchKernelGGL(k_hxv<T>, dim3(G), dim3(want.block_threads), 0, st, cd, inter_of(c), v, out, G, cd.ntiles);
+  hipLaunchKernelGGL(k_hxv<T>, dim3(G), dim3(want.block_threads), 0, st, cd, inter_of(c), v, out,
+                     (int64_t)cd.ntiles);
   EDL_HIP(hipGetLastError());
 }
 
