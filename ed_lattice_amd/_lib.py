@@ -40,6 +40,8 @@ SIGNATURES = {
     "edl_hxv": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "edl_hxv_host": (c_int, [c_void_p, c_int, c_int, _dp, _dp]),
     "edl_hxv_host_complex": (c_int, [c_void_p, c_int, c_int, _dp, _dp]),
+    "edl_vec_random": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_uint64]),
+    "edl_vec_dot": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, _dp]),
     "edl_device_alloc": (c_int, [c_void_p, c_int64, POINTER(c_void_p)]),
     "edl_copy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
     "edl_copy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),

@@ -743,6 +743,39 @@ extern "C" int edl_hxv_host_complex(edl_ctx* ctx, int isector, int prec, const d
   API_END
 }
 
+extern "C" int edl_vec_random(edl_ctx* ctx, int prec, void* dev, int64_t n, uint64_t seed) {
+  API_BEGIN
+  EDL_CHECK(dev && n >= 0, "edl_vec_random: bad arguments");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_vec_random: prec must be EDL_FP64 or EDL_FP32");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  if (prec == EDL_FP64) {
+    launch_start_vector(static_cast<double*>(dev), n, seed, c.stream);
+  } else {
+    // in chunks through an fp64 staging buffer
+    const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 27);
+    DevBuf tmp(sizeof(double) * (chunk > 0 ? chunk : 1));
+    for (int64_t o = 0; o < n; o += chunk) {
+      const int64_t m = std::min(chunk, n - o);
+      launch_start_vector(tmp.as<double>(), m, seed + (uint64_t)o * 0x9E3779B97F4A7C15ull, c.stream);
+      launch_convert<float>(tmp.as<double>(), static_cast<float*>(dev) + o, m, c.stream);
+    }
+  }
+  EDL_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+extern "C" int edl_vec_dot(edl_ctx* ctx, int prec, const void* a, const void* b, int64_t n, double* out) {
+  API_BEGIN
+  EDL_CHECK(a && b && out && n >= 0, "edl_vec_dot: bad arguments");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_vec_dot: prec must be EDL_FP64 or EDL_FP32");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  *out = (prec == EDL_FP64) ? device_dot_t<double>(static_cast<const double*>(a), static_cast<const double*>(b), n, c.stream)
+                            : device_dot_t<float>(static_cast<const float*>(a), static_cast<const float*>(b), n, c.stream);
+  API_END
+}
+
 extern "C" int edl_device_alloc(edl_ctx* ctx, int64_t bytes, void** dev_ptr) {
   API_BEGIN
   EDL_CHECK(dev_ptr && bytes > 0, "edl_device_alloc: bad arguments");

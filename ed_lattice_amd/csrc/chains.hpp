@@ -95,6 +95,8 @@ void launch_convert(const double* src, T* dst, int64_t n, hipStream_t st);
 template <typename T>
 void launch_to_fp64(const T* src, double* dst, int64_t n, hipStream_t st);
 double device_dot(Ctx& c, const double* a, const double* b, int64_t n, hipStream_t st);
+template <typename T>
+double device_dot_t(const T* a, const T* b, int64_t n, hipStream_t st);
 void launch_scale(double* x, int64_t n, double s, hipStream_t st);
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 

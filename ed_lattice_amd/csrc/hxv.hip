@@ -289,6 +289,15 @@ __global__ void k_dot_partial(const double* __restrict__ a, const double* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 template <typename T>
+__global__ void k_dot_partial_t(const T* __restrict__ a, const T* __restrict__ b, int64_t n, double* part) {
+  __shared__ double sh[16];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v += (double)a[i] * (double)b[i];
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <typename T>
 __global__ void k_sumsq_partial(const T* __restrict__ a, int64_t n, double* part) {
   __shared__ double sh[16];
   double v = 0.0;
@@ -360,6 +369,19 @@ double device_sumsq(const T* a, int64_t n, hipStream_t st) {
   EDL_HIP(hipFree(part));
   return s;
 }
+template <typename T>
+double device_dot_t(const T* a, const T* b, int64_t n, hipStream_t st) {
+  const int g = grid_for(n);
+  double* part = nullptr;
+  EDL_HIP(hipMalloc(&part, sizeof(double) * g));
+  hipLaunchKernelGGL(k_dot_partial_t<T>, dim3(g), dim3(kBlock), 0, st, a, b, n, part);
+  EDL_HIP(hipGetLastError());
+  const double s = host_sum_partials(part, g, st);
+  EDL_HIP(hipFree(part));
+  return s;
+}
+template double device_dot_t<double>(const double*, const double*, int64_t, hipStream_t);
+template double device_dot_t<float>(const float*, const float*, int64_t, hipStream_t);
 template double device_sumsq<double>(const double*, int64_t, hipStream_t);
 template double device_sumsq<float>(const float*, int64_t, hipStream_t);
 

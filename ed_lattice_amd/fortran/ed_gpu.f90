@@ -18,7 +18,7 @@ module ED_GPU
   public :: edl_set_hamiltonian, edl_sector_index, edl_sector_qn
   public :: edl_sector_build, edl_sector_maps, edl_sector_hops, edl_sector_rank
   public :: edl_hxv, edl_hxv_host, edl_hxv_host_complex
-  public :: edl_device_alloc, edl_copy_h2d, edl_copy_d2h
+  public :: edl_device_alloc, edl_copy_h2d, edl_copy_d2h, edl_vec_random, edl_vec_dot
   public :: edl_lanczos_gs, edl_lanczos_stats, edl_set_state, edl_get_state
   public :: edl_tridiag, edl_tridiag_batched, edl_lanczos_poles, edl_observables
   public :: edl_last_timing, edl_bench_hxv
@@ -115,6 +115,21 @@ module ED_GPU
        integer(c_int), value :: isector, prec
        complex(c_double_complex) :: v(*), hv(*)
      end function edl_hxv_host_complex
+
+     integer(c_int) function edl_vec_random(ctx, prec, dev, n, seed) bind(C, name="edl_vec_random")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx, dev
+       integer(c_int), value :: prec
+       integer(c_int64_t), value :: n, seed
+     end function edl_vec_random
+
+     integer(c_int) function edl_vec_dot(ctx, prec, a, b, n, res) bind(C, name="edl_vec_dot")
+       import :: c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: ctx, a, b
+       integer(c_int), value :: prec
+       integer(c_int64_t), value :: n
+       real(c_double) :: res
+     end function edl_vec_dot
 
      integer(c_int) function edl_device_alloc(ctx, bytes, dev_ptr) bind(C, name="edl_device_alloc")
        import :: c_int, c_ptr, c_int64_t

@@ -165,6 +165,30 @@ class EDLattice:
         self._ck(self.lib.edl_hxv_host(self.h, isector, PREC[prec], _dp(vin), _dp(out)))
         return out
 
+    # ------------------------------------------------- device-side vectors
+    def alloc(self, nbytes):
+        """Device buffer owned by the context (freed when it is closed)."""
+        p = ctypes.c_void_p()
+        self._ck(self.lib.edl_device_alloc(self.h, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def vec_random(self, prec, ptr, n, seed):
+        self._ck(self.lib.edl_vec_random(self.h, PREC[prec], ptr, int(n), seed))
+
+    def vec_dot(self, prec, a, b, n):
+        out = ctypes.c_double()
+        self._ck(self.lib.edl_vec_dot(self.h, PREC[prec], a, b, int(n), ctypes.byref(out)))
+        return out.value
+
+    def hxv_device(self, isector, prec, v_ptr, hv_ptr):
+        """H v with both vectors already in HBM (edl_hxv)."""
+        self._ck(self.lib.edl_hxv(self.h, isector, PREC[prec], v_ptr, hv_ptr))
+        return self.last_timing()[0]
+
+    def copy_to_host(self, ptr, out):
+        self._ck(self.lib.edl_copy_d2h(self.h, out.ctypes.data, ptr, out.nbytes))
+        return out
+
     def bench_hxv(self, isector, prec="fp32", reps=10):
         ms = ctypes.c_double()
         self._ck(self.lib.edl_bench_hxv(self.h, isector, PREC[prec], reps, ctypes.byref(ms)))

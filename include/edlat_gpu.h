@@ -78,6 +78,11 @@ int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v_dev, void* hv_dev
 int edl_hxv_host(edl_ctx* ctx, int isector, int prec, const double* v, double* hv);
 /* Same with host complex(8) arrays, interleaved (re, im). */
 int edl_hxv_host_complex(edl_ctx* ctx, int isector, int prec, const double* v, double* hv);
+/* Device vector utilities for edl_hxv callers (no reference counterpart: the
+ * reference keeps its vectors on the host): fill with the SplitMix64 start
+ * vector U[-1,1) of `seed`, and the FP64-accumulated dot product. */
+int edl_vec_random(edl_ctx* ctx, int prec, void* dev, int64_t n, uint64_t seed);
+int edl_vec_dot(edl_ctx* ctx, int prec, const void* a_dev, const void* b_dev, int64_t n, double* out);
 /* Device buffer owned by the context (for edl_hxv callers); freed with it. */
 int edl_device_alloc(edl_ctx* ctx, int64_t bytes, void** dev_ptr);
 int edl_copy_h2d(edl_ctx* ctx, void* dst_dev, const void* src_host, int64_t bytes);

@@ -685,3 +685,96 @@ int or_observables(int Ns, int Nup, int Ndw, const double *state, double *dens_u
   sector_free(&S);
   return 0;
 }
+
+/* (H v)_i for listed rows of a sector larger than the reference's default
+ * integer can index (Dim > 2^31-1): the same restated per-element formulas
+ * (HxV_local.f90:1-92, HxV_up.f90:13-27, HxV_dw.f90:12-28), with 64-bit
+ * element indices, a real fp32 input vector and fp64 accumulation.  For
+ * each row the candidate sources are its hop partners; a source's scatter
+ * contributions are kept when their target (found by binary_search as in the
+ * reference) is the row.  Non-local (Jhflag) terms refused. */
+int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t *rows, const float *v,
+                       double *out) {
+  or_model m;
+  int rc = model_init(&m, OR_MODEL_PASS);
+  if (rc) return rc;
+  if (Jhflag_of(&m)) return -6;
+  long dimup = or_binomial(Ns, Nup), dimdw = or_binomial(Ns, Ndw);
+  int *mup = (int *)malloc(sizeof(int) * dimup), *mdw = (int *)malloc(sizeof(int) * dimdw);
+  or_build_map(Ns, Nup, mup);
+  or_build_map(Ns, Ndw, mdw);
+  double Hdiag[2 * OR_MAXNS];
+  make_Hdiag(&m, Hdiag);
+  /* a one-column sector view reused for the diagonal: local_htmp indexes
+   * element 1 = (iup, idw) of a 1x1 sector built on the fly */
+  for (int r = 0; r < nrows; r++) {
+    const int64_t i = rows[r];
+    const int iup = (int)(i % dimup) + 1, idw = (int)(i / dimup) + 1; /* 1-based */
+    double acc = 0.0;
+    {
+      or_sector S1;
+      int su = mup[iup - 1], sd = mdw[idw - 1];
+      S1.DimUp = 1;
+      S1.DimDw = 1;
+      S1.Dim = 1;
+      S1.mup = &su;
+      S1.mdw = &sd;
+      acc += local_htmp(&m, &S1, 1, Hdiag) * (double)v[i];
+    }
+    int bits[OR_MAXNS + 1];
+    /* up hops: sources jup in column idw */
+    bdecomp(mup[iup - 1], Ns, bits);
+    for (int a = 1; a <= Ns; a++)
+      for (int b = 1; b <= Ns; b++) {
+        if (!(bits[a] == 1 && bits[b] == 0)) continue;
+        int k1, k2;
+        double g1, g2;
+        or_c(a, mup[iup - 1], &k1, &g1);
+        or_cdg(b, k1, &k2, &g2);
+        int jup = or_binary_search(mup, (int)dimup, k2);
+        if (jup <= 0) continue;
+        int nj[OR_MAXNS + 1];
+        bdecomp(mup[jup - 1], Ns, nj);
+        for (int io = 1; io <= Ns; io++)
+          for (int jo = 1; jo <= Ns; jo++) {
+            double h = creal(Hij_at(&m, 1, io, jo));
+            if (!(h != 0.0 && nj[jo] == 1 && nj[io] == 0)) continue;
+            int t1, t2;
+            double s1, s2;
+            or_c(jo, mup[jup - 1], &t1, &s1);
+            or_cdg(io, t1, &t2, &s2);
+            if (or_binary_search(mup, (int)dimup, t2) == iup)
+              acc += h * s1 * s2 * (double)v[(int64_t)(jup - 1) + (int64_t)(idw - 1) * dimup];
+          }
+      }
+    /* down hops: sources jdw in row iup */
+    bdecomp(mdw[idw - 1], Ns, bits);
+    for (int a = 1; a <= Ns; a++)
+      for (int b = 1; b <= Ns; b++) {
+        if (!(bits[a] == 1 && bits[b] == 0)) continue;
+        int k1, k2;
+        double g1, g2;
+        or_c(a, mdw[idw - 1], &k1, &g1);
+        or_cdg(b, k1, &k2, &g2);
+        int jdw = or_binary_search(mdw, (int)dimdw, k2);
+        if (jdw <= 0) continue;
+        int nj[OR_MAXNS + 1];
+        bdecomp(mdw[jdw - 1], Ns, nj);
+        for (int io = 1; io <= Ns; io++)
+          for (int jo = 1; jo <= Ns; jo++) {
+            double h = creal(Hij_at(&m, m.Nspin, io, jo));
+            if (!(h != 0.0 && nj[jo] == 1 && nj[io] == 0)) continue;
+            int t1, t2;
+            double s1, s2;
+            or_c(jo, mdw[jdw - 1], &t1, &s1);
+            or_cdg(io, t1, &t2, &s2);
+            if (or_binary_search(mdw, (int)dimdw, t2) == idw)
+              acc += h * s1 * s2 * (double)v[(int64_t)(iup - 1) + (int64_t)(jdw - 1) * dimup];
+          }
+      }
+    out[r] = acc;
+  }
+  free(mup);
+  free(mdw);
+  return 0;
+}

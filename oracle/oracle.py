@@ -165,6 +165,24 @@ def hxv_rows(model, nup, ndw, rows, v):
     return out
 
 
+def hxv_rows_real64(model, nup, ndw, rows, v32):
+    """Rows of H v for a real fp32 vector with 64-bit element indices (for
+    sectors beyond the reference's 32-bit Dim), accumulated in fp64."""
+    L = lib()
+    L.or_hxv_rows_real64.argtypes = [c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), POINTER(c_double),
+                                     POINTER(c_double), c_double, c_double, c_double, c_double, c_double, c_int,
+                                     c_int, c_int, c_int, c_int, POINTER(c_int64), POINTER(ctypes.c_float),
+                                     POINTER(c_double)]
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    v = np.ascontiguousarray(v32, dtype=np.float32)
+    out = np.zeros(rows.size)
+    rc = L.or_hxv_rows_real64(*model.args(), nup, ndw, rows.size, rows.ctypes.data_as(POINTER(c_int64)),
+                              v.ctypes.data_as(POINTER(ctypes.c_float)), _dp(out))
+    if rc:
+        raise RuntimeError(f"or_hxv_rows_real64 failed ({rc})")
+    return out
+
+
 def seed_c_cdg(Ns, op, ipos, ispin, nI, nJ, state):
     """apply_op_C (op=1) / apply_op_CDG (op=2) loop of GF_ELECTRONS.f90:241-245."""
     st = np.ascontiguousarray(state, dtype=np.complex128)

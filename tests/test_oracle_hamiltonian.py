@@ -96,6 +96,17 @@ def test_hermitian_and_sampled_rows_large(orc):
     np.testing.assert_allclose(orc.hxv_rows(m, 6, 6, rows, v), Hv[rows], atol=1e-11)
 
 
+def test_rows_real64_equal_matvec(orc):
+    """The 64-bit row evaluator (used at the 3x6 size) = the faithful product."""
+    m = orc.Model(lattice.disordered_rect(2, 4, seed=8), [3.0], xmu=0.2)
+    nup, ndw = 4, 3
+    D = orc.binomial(8, 4) * orc.binomial(8, 3)
+    v = RNG.standard_normal(D).astype(np.float32)
+    ref = orc.matvec(m, nup, ndw, v.astype(np.float64)).real
+    rows = np.arange(D)
+    np.testing.assert_allclose(orc.hxv_rows_real64(m, nup, ndw, rows, v), ref, atol=1e-12)
+
+
 def test_dim_overflow_refused(orc):
     """3x6 (9,9) has Dim = 48620^2 > 2^31-1: the reference's default integer
     cannot hold it, and the restatement refuses it."""
