@@ -70,13 +70,16 @@ def seeds_benchmark(ed, nseq=24):
             seeds += [("cdg", 1, i, jo), ("c", 1, i, jo)]
     out = {"workload": "3x4 PBC U=6, GS (6,6); 156 spin-up GF seeds (78 i<=j pairs x particle/hole) in "
                        "(7,6)/(5,6), dim 731808, 150 Lanczos steps each, fp64"}
-    ed.tridiag_batched(seeds[:2], 150, "fp64")  # tune + warm-up
+    # first calls tune the tile/wave schedules (cached per chain-set shape)
+    ed.tridiag_batched(seeds, 150, "fp64")
+    ed.tridiag_batched(seeds[:1], 150, "fp64", group=1)
     rb = ed.tridiag_batched(seeds, 150, "fp64")
     rs = ed.tridiag_batched(seeds[:nseq], 150, "fp64", group=1)
     seq_ms = rs["ms"] * len(seeds) / nseq
     out.update(batched_ms=rb["ms"], sequential_ms_est=seq_ms, sequential_sample=nseq,
                speedup=seq_ms / rb["ms"], hxv_batched=rb["hxv"],
                hxv_per_s_batched=rb["hxv"] / (rb["ms"] * 1e-3))
+    ed.tridiag_batched(seeds, 150, "fp32")
     rb32 = ed.tridiag_batched(seeds, 150, "fp32")
     out.update(batched_ms_fp32=rb32["ms"])
     return out

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     for (int r = threadIdx.x; r < dimup; r += blockDim.x) {
       VT x;
 #pragma unroll
-      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, (c0 + j < dimdw) ? q[(size_t)(c0 + j) * dimup + r] : (T)0);
+      for (int j = 0; j < C; ++j)
+        Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * dimup + r]) : (T)0);
       P[r] = x;
     }
     __syncthreads();
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
         if (h == 1 && !okb) break;
 #pragma unroll
         for (int j = 0; j < C; ++j)
-          if (c0 + j < dimdw) tout[(size_t)(c0 + j) * dimup + row[h]] = acc[h][j] * ts;
+          if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * dimup + row[h]]);
       }
     }
   }
@@ -232,10 +233,10 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
           if (h == 1 && !okb) break;
           const size_t i = (size_t)col[h] * dimup + iup;
           if (MODE == 0) {
-            out[i] = tin[i] + acc[h];
+            __builtin_nontemporal_store(__builtin_nontemporal_load(&tin[i]) + acc[h], &out[i]);
           } else {
-            const T tv = tin[i] + acc[h] * ts - bo * out[i];
-            out[i] = tv;
+            const T tv = __builtin_nontemporal_load(&tin[i]) + acc[h] * ts - bo * __builtin_nontemporal_load(&out[i]);
+            __builtin_nontemporal_store(tv, &out[i]);
             part += (double)tv * (double)q[i];
           }
         }
