@@ -2,6 +2,8 @@
 // sector set-up, the operator, the Lanczos drivers and the batched seeds.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <tuple>
@@ -123,15 +125,37 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T));
   auto it = tp_cache().find(key);
   if (it != tp_cache().end()) return it->second;
+  // timed on the engine's own (padded) layout
+  const int ld = padded_ld(s.sp[0].dim);
+  const TwoPassView V = two_pass_view(s, ld);
+  // EDL_TWO_PASS_CFG_F32 / _F64 = "C,threads_up,grid_up,R,threads_dw,nchunk"
+  // pins the schedule of that precision (reproducible profiles of one
+  // configuration); no tuning then.
+  if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
+    TwoPassCfg p;
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R, &p.threads_dw,
+                              &p.nchunk);
+    EDL_CHECK(n == 6, "EDL_TWO_PASS_CFG_F32/_F64: expected C,threads_up,grid_up,R,threads_dw,nchunk");
+    EDL_CHECK((p.C == 1 || p.C == 2 || p.C == 4) && (p.R == 16 || p.R == 32 || p.R == 64),
+              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1, 2 or 4 and R 16, 32 or 64");
+    EDL_CHECK(p.threads_up % 64 == 0 && p.threads_up >= 64 && p.threads_up <= 1024 && p.threads_dw % 64 == 0 &&
+                  p.threads_dw >= 64 && p.threads_dw <= 1024 && p.grid_up >= 1 && p.nchunk >= 1 &&
+                  p.nchunk <= V.dimdw,
+              "EDL_TWO_PASS_CFG_F32/_F64: bad block size, grid or chunk count");
+    EDL_CHECK(256 * sizeof(T) + 16 + (size_t)V.dimup * p.C * sizeof(T) <= (size_t)(160 * 1024),
+              "EDL_TWO_PASS_CFG_F32/_F64: up-pass columns exceed LDS");
+    tp_cache()[key] = p;
+    return p;
+  }
   hipStream_t st = c.stream;
-  const TwoPassView V = two_pass_view(s);
-  const int64_t D = s.dim;
-  DevBuf d64(sizeof(double) * D), q(sizeof(T) * D), t(sizeof(T) * D), o(sizeof(T) * D);
+  const int64_t D = s.dim, Dp = (int64_t)ld * s.sp[1].dim;
+  DevBuf d64(sizeof(double) * D), q(sizeof(T) * Dp), t(sizeof(T) * Dp), o(sizeof(T) * Dp);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
-  launch_convert<T>(d64.as<double>(), q.as<T>(), D, st);
+  launch_zero_bytes(q.p, sizeof(T) * Dp, st);
+  launch_convert_ld<T>(d64.as<double>(), q.as<T>(), s.sp[0].dim, s.sp[1].dim, ld, st);
   TwoPassCfg best;
   best.grid_up = c.num_cu;
-  best.grid_dw = c.num_cu * 8;
+  best.nchunk = std::min(32, V.dimdw);
   double best_up = 1e300, best_dw = 1e300;
   const int lds = 160 * 1024;
   for (int C : {1, 2, 4})
@@ -156,14 +180,16 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best.grid_up = w.grid_up;
         }
       }
-  for (int R : {32, 64})
+  for (int R : {16, 32, 64})
     for (int th : {256, 512, 1024})
-      for (int gm : {2, 4, 8, 16}) {
+      for (int nch : {8, 16, 32, 64, 128, 256}) {
+        if (nch > V.dimdw) continue;
         TwoPassCfg w = best;
         w.R = R;
         w.threads_dw = th;
-        w.grid_dw = c.num_cu * gm;
-        w.grid_dw -= w.grid_dw % 8;
+        w.nchunk = nch;
+        const size_t tbl = sizeof(uint32_t) * (size_t)((V.dimdw + nch - 1) / nch + 1) * (V.kdw + 1);
+        if (tbl > (size_t)lds - 4096) continue;
         launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
         EventTimer tm(st);
         for (int r = 0; r < 3; ++r)
@@ -173,7 +199,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best_dw = ms;
           best.R = R;
           best.threads_dw = th;
-          best.grid_dw = w.grid_dw;
+          best.nchunk = nch;
         }
       }
   tp_cache()[key] = best;
@@ -182,15 +208,18 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
 
 // H v on one sector: two-pass kernels where the columns fit in LDS,
 // otherwise the single-pass gather kernel.
+// v and out have column stride ld (DimUp: natural layout; padded_ld: the
+// engine's layout, two-pass sectors only).
 template <typename T>
-static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, hipStream_t st) {
+static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, int ld, hipStream_t st) {
   if (two_pass_ok(s, sizeof(T))) {
     const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
-    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * s.dim));
-    const TwoPassView V = two_pass_view(s);
+    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
+    const TwoPassView V = two_pass_view(s, ld);
     launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v, v, tmp, nullptr, cfg, st);
     launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), const_cast<T*>(v), const_cast<T*>(v), tmp, out, nullptr, cfg, st);
   } else {
+    EDL_CHECK(ld == s.sp[0].dim, "apply_h: the single-pass operator takes the natural layout");
     Schedule w;
     w.block_threads = 256;
 
@@ -212,18 +241,23 @@ struct ChainAlloc {
   DevBuf q0, q1, sc, alpha, beta, counter;
 };
 
+// d_start is in the natural layout; the stored vectors get column stride ld
+// (the padding rows are zero and no kernel writes them).
 template <typename T>
 static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, const double* d_start, double norm2,
-                       int nmax, hipStream_t st) {
-  const int64_t D = S.dim;
+                       int nmax, hipStream_t st, int ld = 0) {
+  if (ld == 0) ld = S.sp[0].dim;
+  const int64_t D = (int64_t)ld * S.sp[1].dim;
   fill_sector_view(S, sizeof(T) == 4, cd);
+  cd.ld = ld;
   A.q0 = DevBuf(sizeof(T) * D);
   A.q1 = DevBuf(sizeof(T) * D);
   A.sc = DevBuf(sizeof(LzScalars));
   A.alpha = DevBuf(sizeof(double) * nmax);
   A.beta = DevBuf(sizeof(double) * nmax);
   A.counter = DevBuf(sizeof(unsigned));
-  launch_convert<T>(d_start, A.q0.template as<T>(), D, st);
+  if (ld != S.sp[0].dim) launch_zero_bytes(A.q0.p, sizeof(T) * D, st);
+  launch_convert_ld<T>(d_start, A.q0.template as<T>(), S.sp[0].dim, S.sp[1].dim, ld, st);
   launch_zero_bytes(A.q1.p, sizeof(T) * D, st);
   launch_zero_bytes(A.alpha.p, sizeof(double) * nmax, st);
   launch_zero_bytes(A.beta.p, sizeof(double) * nmax, st);
@@ -408,13 +442,15 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   const bool two = two_pass_ok(S, sizeof(T));
   TwoPassStep<T> tps{};
   DevBuf tp_part, tp_counter;
+  // Two-pass chains keep their vectors with the padded column stride.
+  const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
     tps.cfg = two_pass_cfg<T>(c, S);
-    tps.V = two_pass_view(S);
+    tps.V = two_pass_view(S, ld);
     tps.lut_up = lut_dev<T>(S.sp[0]);
     tps.lut_dw = lut_dev<T>(S.sp[1]);
-    tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * S.dim));
-    tp_part = DevBuf(sizeof(double) * tps.cfg.grid_dw);
+    tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * S.sp[1].dim));
+    tp_part = DevBuf(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
     tp_counter = DevBuf(sizeof(unsigned));
     launch_zero_bytes(tp_counter.p, 4, st);
     tps.A.part = tp_part.as<double>();
@@ -427,7 +463,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     ChainSet<T> cs;
     ChainAlloc<T> A;
     ChainDev cd{};
-    init_chain<T>(c, cd, A, S, d_start, 1.0, nmax, st);
+    init_chain<T>(c, cd, A, S, d_start, 1.0, nmax, st, ld);
     if (!two) sch = autotune<T>(c, {cd}, st);
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
@@ -446,7 +482,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     ChainSet<T> cs;
     ChainAlloc<T> A;
     ChainDev cd{};
-    init_chain<T>(c, cd, A, S, d_start, 1.0, m, st);
+    init_chain<T>(c, cd, A, S, d_start, 1.0, m, st, ld);
     cd.x = d_x;
     cd.ccoef = dc.as<double>();
     cs.h.push_back(cd);
@@ -695,9 +731,9 @@ extern "C" int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v, void*
   Sector& s = get_sector(ctx->c, isector);
   EventTimer tm(ctx->c.stream);
   if (prec == EDL_FP64)
-    apply_h<double>(ctx->c, s, (const double*)v, (double*)hv, ctx->c.stream);
+    apply_h<double>(ctx->c, s, (const double*)v, (double*)hv, s.sp[0].dim, ctx->c.stream);
   else
-    apply_h<float>(ctx->c, s, (const float*)v, (float*)hv, ctx->c.stream);
+    apply_h<float>(ctx->c, s, (const float*)v, (float*)hv, s.sp[0].dim, ctx->c.stream);
   ctx->c.last_ms = tm.stop();
   ctx->c.last_hxv = 1;
   API_END
@@ -709,11 +745,15 @@ static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_
   hipStream_t st = c.stream;
   std::vector<double> tmp(D);
   for (int64_t i = 0; i < D; ++i) tmp[i] = v[i * stride];
-  DevBuf d64(sizeof(double) * D), dv(sizeof(T) * D), dh(sizeof(T) * D);
+  // through the engine's own layout (padded columns on two-pass sectors)
+  const int ld = two_pass_ok(s, sizeof(T)) ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const int64_t Dp = (int64_t)ld * s.sp[1].dim;
+  DevBuf d64(sizeof(double) * D), dv(sizeof(T) * Dp), dh(sizeof(T) * Dp);
   EDL_HIP(hipMemcpyAsync(d64.p, tmp.data(), sizeof(double) * D, hipMemcpyHostToDevice, st));
-  launch_convert<T>(d64.as<double>(), dv.as<T>(), D, st);
-  apply_h<T>(c, s, dv.as<T>(), dh.as<T>(), st);
-  launch_to_fp64<T>(dh.as<T>(), d64.as<double>(), D, st);
+  if (ld != s.sp[0].dim) launch_zero_bytes(dv.p, sizeof(T) * Dp, st);
+  launch_convert_ld<T>(d64.as<double>(), dv.as<T>(), s.sp[0].dim, s.sp[1].dim, ld, st);
+  apply_h<T>(c, s, dv.as<T>(), dh.as<T>(), ld, st);
+  launch_to_fp64_ld<T>(dh.as<T>(), d64.as<double>(), s.sp[0].dim, s.sp[1].dim, ld, st);
   EDL_HIP(hipMemcpyAsync(tmp.data(), d64.p, sizeof(double) * D, hipMemcpyDeviceToHost, st));
   EDL_HIP(hipStreamSynchronize(st));
   for (int64_t i = 0; i < D; ++i) hv[i * stride] = tmp[i];
@@ -1069,12 +1109,16 @@ template <typename T>
 static double bench_impl(Ctx& c, Sector& s, int reps) {
   hipStream_t st = c.stream;
   const int64_t D = s.dim;
-  DevBuf d64(sizeof(double) * D), v(sizeof(T) * D), h(sizeof(T) * D);
+  // the engine's own layout (padded columns on two-pass sectors)
+  const int ld = two_pass_ok(s, sizeof(T)) ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const int64_t Dp = (int64_t)ld * s.sp[1].dim;
+  DevBuf d64(sizeof(double) * D), v(sizeof(T) * Dp), h(sizeof(T) * Dp);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
-  launch_convert<T>(d64.as<double>(), v.as<T>(), D, st);
-  apply_h<T>(c, s, v.as<T>(), h.as<T>(), st);
+  if (ld != s.sp[0].dim) launch_zero_bytes(v.p, sizeof(T) * Dp, st);
+  launch_convert_ld<T>(d64.as<double>(), v.as<T>(), s.sp[0].dim, s.sp[1].dim, ld, st);
+  apply_h<T>(c, s, v.as<T>(), h.as<T>(), ld, st);
   EventTimer tm(st);
-  for (int r = 0; r < reps; ++r) apply_h<T>(c, s, v.as<T>(), h.as<T>(), st);
+  for (int r = 0; r < reps; ++r) apply_h<T>(c, s, v.as<T>(), h.as<T>(), ld, st);
   return tm.stop() / reps;
 }
 
@@ -1088,5 +1132,69 @@ extern "C" int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, doub
   *ms_per_hxv = (prec == EDL_FP64) ? bench_impl<double>(c, s, reps) : bench_impl<float>(c, s, reps);
   c.last_ms = *ms_per_hxv * reps;
   c.last_hxv = reps;
+  API_END
+}
+
+// Mean duration of each kernel launch of the operator, bracketed by events.
+template <typename T>
+static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int32_t* cfg) {
+  hipStream_t st = c.stream;
+  const bool two = two_pass_ok(s, sizeof(T));
+  const int ld = two ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const int64_t D = s.dim, Dp = (int64_t)ld * s.sp[1].dim;
+  DevBuf d64(sizeof(double) * D), v(sizeof(T) * Dp), h(sizeof(T) * Dp);
+  launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
+  if (ld != s.sp[0].dim) launch_zero_bytes(v.p, sizeof(T) * Dp, st);
+  launch_convert_ld<T>(d64.as<double>(), v.as<T>(), s.sp[0].dim, s.sp[1].dim, ld, st);
+  apply_h<T>(c, s, v.as<T>(), h.as<T>(), ld, st);
+  std::vector<hipEvent_t> ev(3 * (size_t)reps);
+  for (auto& e : ev) EDL_HIP(hipEventCreate(&e));
+  TwoPassCfg cf{};
+  if (two) {
+    cf = two_pass_cfg<T>(c, s);
+    const TwoPassView V = two_pass_view(s, ld);
+    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)Dp));
+    for (int r = 0; r < reps; ++r) {
+      EDL_HIP(hipEventRecord(ev[3 * r], st));
+      launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, cf, st);
+      EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
+      launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
+      EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
+    }
+  } else {
+    for (int r = 0; r < reps; ++r) {
+      EDL_HIP(hipEventRecord(ev[3 * r], st));
+      apply_h<T>(c, s, v.as<T>(), h.as<T>(), ld, st);
+      EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
+      EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
+    }
+  }
+  EDL_HIP(hipStreamSynchronize(st));
+  double up = 0.0, dw = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    float a = 0.f, b = 0.f;
+    EDL_HIP(hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
+    EDL_HIP(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
+    up += a;
+    dw += b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  ms_pass[0] = up / reps;
+  ms_pass[1] = two ? dw / reps : 0.0;
+  const int32_t out[8] = {two ? 1 : 0, ld, cf.C, cf.threads_up, cf.grid_up, cf.R, cf.threads_dw, cf.nchunk};
+  for (int k = 0; k < 8; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
+}
+
+extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {
+  API_BEGIN
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_bench_passes: prec must be EDL_FP64 or EDL_FP32");
+  EDL_CHECK(reps >= 1 && ms_pass && cfg, "edl_bench_passes: bad arguments");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& s = get_sector(c, isector);
+  if (prec == EDL_FP64)
+    bench_passes_impl<double>(c, s, reps, ms_pass, cfg);
+  else
+    bench_passes_impl<float>(c, s, reps, ms_pass, cfg);
   API_END
 }

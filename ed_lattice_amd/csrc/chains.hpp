@@ -42,9 +42,10 @@ struct ChainDev {
   int64_t tile0;           // first global tile of the chain
   int ntiles;              // = dimdw
   // state
+  int ld;                  // column stride of q0/q1 (>= dimup)
   void* q0;                // stored vectors; q_k lives in q[k & 1]
   void* q1;
-  double* x;               // fp64 eigenvector accumulator (replay), or null
+  double* x;               // fp64 eigenvector accumulator (replay, natural layout), or null
   const double* ccoef;     // replay coefficients c_k, or null
   double* part;            // [ntiles] tile partial sums
   unsigned* counter;       // tiles completed in the current launch
@@ -92,8 +93,14 @@ void fill_sector_view(const Sector& s, bool fp32, ChainDev& cd);
 void launch_start_vector(double* d_x, int64_t n, uint64_t seed, hipStream_t st);
 template <typename T>
 void launch_convert(const double* src, T* dst, int64_t n, hipStream_t st);
+// src in the natural layout (rows x cols); dst with column stride ld.
+template <typename T>
+void launch_convert_ld(const double* src, T* dst, int rows, int cols, int ld, hipStream_t st);
 template <typename T>
 void launch_to_fp64(const T* src, double* dst, int64_t n, hipStream_t st);
+// src with column stride ld; dst in the natural layout (rows x cols).
+template <typename T>
+void launch_to_fp64_ld(const T* src, double* dst, int rows, int cols, int ld, hipStream_t st);
 double device_dot(Ctx& c, const double* a, const double* b, int64_t n, hipStream_t st);
 template <typename T>
 double device_dot_t(const T* a, const T* b, int64_t n, hipStream_t st);

@@ -88,7 +88,7 @@ static __device__ __forceinline__ void column_hq(const ChainDev& C, const InterP
                                                  int idw, Emit emit) {
   const T* __restrict__ cu = reinterpret_cast<const T*>(C.cu);
   const T* __restrict__ cd = reinterpret_cast<const T*>(C.cd);
-  const size_t col = (size_t)idw * C.dimup;
+  const size_t col = (size_t)idw * C.ld;
   const uint32_t sd = (uint32_t)C.map_dw[idw];
   const double dgd = C.dg_dw[idw];
   const int ndw = C.cnt_dw[idw];
@@ -116,7 +116,7 @@ static __device__ __forceinline__ void column_hq(const ChainDev& C, const InterP
         const bool on = k0 + j < ndw;
         const size_t o = (size_t)(k0 + j) * C.dimdw + idw;
         c[j] = on ? cd[o] : (T)0;
-        x[j] = on ? q[(size_t)C.nbr_dw[o] * C.dimup + iup] : (T)0;
+        x[j] = on ? q[(size_t)C.nbr_dw[o] * C.ld + iup] : (T)0;
       }
 #pragma unroll
       for (int j = 0; j < KCH; ++j) acc += c[j] * x[j];
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(1024) void k_lz_b(const ChainDev* __restrict__ chai
     const T am = (T)(sc.alpha_k * inv_s);
     double* __restrict__ x = C.x;
     const double cx = x ? C.ccoef[sc.k] * inv_s : 0.0;
-    const size_t col = (size_t)idw * C.dimup;
+    const size_t col = (size_t)idw * C.ld, colx = (size_t)idw * C.dimup;
     double acc = 0.0;
     for (int iup = threadIdx.x; iup < C.dimup; iup += blockDim.x) {
       const size_t i = col + iup;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(1024) void k_lz_b(const ChainDev* __restrict__ chai
       const T r = qo[i] - am * qv;
       qo[i] = r;
       acc += (double)r * (double)r;
-      if (x) x[i] += cx * (double)qv;
+      if (x) x[colx + iup] += cx * (double)qv;
     }
     if (tile_done(C, idw, acc, sh, &flag)) {
       const double s2 = chain_total(C, sh);
@@ -266,6 +266,18 @@ template <typename T>
 __global__ void k_convert(const double* __restrict__ s, T* __restrict__ d, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     d[i] = (T)s[i];
+}
+// Natural layout (rows x cols, column-major) to column stride ld >= rows.
+template <typename T>
+__global__ void k_convert_ld(const double* __restrict__ s, T* __restrict__ d, int rows, int64_t n, int64_t ld) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[(i / rows) * ld + i % rows] = (T)s[i];
+}
+// Column stride ld to the natural layout, widened to fp64.
+template <typename T>
+__global__ void k_to64_ld(const T* __restrict__ s, double* __restrict__ d, int rows, int64_t n, int64_t ld) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (double)s[(i / rows) * ld + i % rows];
 }
 template <typename T>
 __global__ void k_to64(const T* __restrict__ s, double* __restrict__ d, int64_t n) {
@@ -319,6 +331,22 @@ void launch_start_vector(double* d_x, int64_t n, uint64_t seed, hipStream_t st) 
 template <typename T>
 void launch_convert(const double* src, T* dst, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, src, dst, n);
+  EDL_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_convert_ld(const double* src, T* dst, int rows, int cols, int ld, hipStream_t st) {
+  EDL_CHECK(ld >= rows && rows > 0, "launch_convert_ld: bad column stride");
+  const int64_t n = (int64_t)rows * cols;
+  if (ld == rows) return launch_convert<T>(src, dst, n, st);
+  hipLaunchKernelGGL(k_convert_ld<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, src, dst, rows, n, (int64_t)ld);
+  EDL_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_to_fp64_ld(const T* src, double* dst, int rows, int cols, int ld, hipStream_t st) {
+  EDL_CHECK(ld >= rows && rows > 0, "launch_to_fp64_ld: bad column stride");
+  const int64_t n = (int64_t)rows * cols;
+  if (ld == rows) return launch_to_fp64<T>(src, dst, n, st);
+  hipLaunchKernelGGL(k_to64_ld<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, src, dst, rows, n, (int64_t)ld);
   EDL_HIP(hipGetLastError());
 }
 template <typename T>
@@ -402,6 +430,7 @@ void fill_sector_view(const Sector& s, bool fp32, ChainDev& cd) {
   cd.dg_up = s.sp[0].diag;
   cd.dg_dw = s.sp[1].diag;
   cd.ntiles = cd.dimdw;
+  cd.ld = cd.dimup;
 }
 
 static int pick_grid(Ctx& c, int64_t total_tiles, const Schedule& w) {
@@ -484,7 +513,11 @@ template void hxv_operator<double>(Ctx&, const Sector&, const double*, double*, 
 template void hxv_operator<float>(Ctx&, const Sector&, const float*, float*, hipStream_t, const Schedule&);
 template void launch_convert<double>(const double*, double*, int64_t, hipStream_t);
 template void launch_convert<float>(const double*, float*, int64_t, hipStream_t);
+template void launch_convert_ld<double>(const double*, double*, int, int, int, hipStream_t);
+template void launch_convert_ld<float>(const double*, float*, int, int, int, hipStream_t);
 template void launch_to_fp64<double>(const double*, double*, int64_t, hipStream_t);
 template void launch_to_fp64<float>(const float*, double*, int64_t, hipStream_t);
+template void launch_to_fp64_ld<double>(const double*, double*, int, int, int, hipStream_t);
+template void launch_to_fp64_ld<float>(const float*, double*, int, int, int, hipStream_t);
 
 }  // namespace edl

@@ -6,11 +6,12 @@
 //     HxV_local.f90); the packed hop word of a row is read once for all C
 //     columns.
 //   pass D (row panels):     out(r, :) = T(r, :) + [H_dw v](r, :)
-//     XCD x walks the row panels p = x, x+8, ...; all blocks of that XCD
-//     sweep the columns of the same panel at the same time, so the panel
-//     (R rows x DimDw columns) is read from HBM once and its down-hop
-//     neighbour segments (R contiguous rows of another column,
-//     HxV_dw.f90) are served from that XCD's L2.
+//     XCD x takes the row panels p = x, x+8, ... in order, one block per
+//     (panel, column chunk); the blocks resident on an XCD work on the same
+//     panel, so the panel (R rows x DimDw columns) is read from HBM once
+//     and its down-hop neighbour segments (R contiguous rows of another
+//     column, HxV_dw.f90) are served from that XCD's L2.  With the padded
+//     column stride a 32-row fp32 (16-row fp64) segment is one cache line.
 //
 // Hop tables are packed words nbr | cid << 24 with the signed coefficient
 // lut[cid] kept in LDS.  Pass D carries the Lanczos phase-A fusion
@@ -74,6 +75,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   }
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
   const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
   const int npan = (dimdw + C - 1) / C;
   const T ts = (T)scale;
   for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
       VT x;
 #pragma unroll
       for (int j = 0; j < C; ++j)
-        Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * dimup + r]) : (T)0);
+        Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0);
       P[r] = x;
     }
     __syncthreads();
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
         if (h == 1 && !okb) break;
 #pragma unroll
         for (int j = 0; j < C; ++j)
-          if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * dimup + row[h]]);
+          if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * ld + row[h]]);
       }
     }
   }
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
 // q_{k-1}) = tin + H_dw q / s_k - beta_k q_{k-1} / s_{k-1}, alpha partials.
 template <typename T, int R, int MODE>
 __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
-                                                  const T* __restrict__ tin, T* outp, LzA A) {
+                                                  const T* __restrict__ tin, T* outp, LzA A, int nchunk) {
   __shared__ T lut[256];
   __shared__ double sh[16];
   __shared__ int flag;
@@ -163,19 +165,26 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   }
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_dw[k];
   const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  // Block b = 8*i + x runs on XCD x (the hardware deals blocks to the 8 XCDs
+  // round robin) and takes column chunk i % nchunk of panel p = 8*(i /
+  // nchunk) + x.  Each XCD receives its blocks in increasing i, so the
+  // dispatcher itself keeps the blocks resident on an XCD within one or two
+  // consecutive panels of that XCD: the panel's neighbour segments stay in
+  // the XCD's L2 without any inter-block synchronisation.  Blocks past the
+  // last panel only contribute a zero partial.
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int xcd = (G % 8 == 0) ? b % 8 : 0;
-  const int nbx = (G % 8 == 0) ? G / 8 : G;
-  const int jb = (G % 8 == 0) ? b / 8 : b;
+  const int xcd = b % 8, ib = b / 8;
+  const int jb = ib % nchunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int SLOTS = 64 / R;
   const int r = lane % R, slot = lane / R;
   const int nwaves = blockDim.x >> 6;
   const int npan = (dimup + R - 1) / R;
-  const int cb = (int)((int64_t)jb * dimdw / nbx), ce = (int)((int64_t)(jb + 1) * dimdw / nbx);
-  // The block's columns are fixed for every panel: stage their hop words
-  // and exact hop counts in LDS once.
+  const int p0 = 8 * (ib / nchunk) + xcd, pend = p0 < npan ? p0 + 1 : p0;
+  const int cb = (int)((int64_t)jb * dimdw / nchunk), ce = (int)((int64_t)(jb + 1) * dimdw / nchunk);
+  // stage the chunk's hop words and exact hop counts in LDS
   extern __shared__ __align__(16) unsigned char dsm[];
   uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);           // [ncol][kdw]
   const int ncol = ce - cb, kdw = V.kdw;
@@ -190,7 +199,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   const T ts = (T)inv_s;
   const T bo = (MODE == 1) ? (T)(s.beta_k / s.s_old) : (T)0;
   double part = 0.0;
-  for (int p = xcd; p < npan; p += ((G % 8 == 0) ? 8 : 1)) {
+  for (int p = p0; p < pend; ++p) {
     const int iup = p * R + r;
     const bool ok = iup < dimup;
     // Two columns per lane; the neighbour segments of a chunk of KCH hops of
@@ -221,7 +230,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
         for (int h = 0; h < 2; ++h)
 #pragma unroll
           for (int j = 0; j < KCH; ++j)
-            x[h][j] = (k0 + j < (h ? n1 : n0)) ? q[(size_t)(w[h][j] & 0xFFFFFFu) * dimup + iuc] : (T)0;
+            x[h][j] = (k0 + j < (h ? n1 : n0)) ? q[(size_t)(w[h][j] & 0xFFFFFFu) * ld + iuc] : (T)0;
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -231,7 +240,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           if (h == 1 && !okb) break;
-          const size_t i = (size_t)col[h] * dimup + iup;
+          const size_t i = (size_t)col[h] * ld + iup;
           if (MODE == 0) {
             __builtin_nontemporal_store(__builtin_nontemporal_load(&tin[i]) + acc[h], &out[i]);
           } else {
@@ -292,10 +301,14 @@ bool two_pass_ok(const Sector& s, size_t elem) {
   return s.sp[0].packed_ok && s.sp[1].packed_ok && up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes();
 }
 
-TwoPassView two_pass_view(const Sector& s) {
+int padded_ld(int dimup) { return (dimup + 31) / 32 * 32; }
+
+TwoPassView two_pass_view(const Sector& s, int ld) {
+  EDL_CHECK(ld >= s.sp[0].dim, "two_pass_view: column stride below DimUp");
   TwoPassView v{};
   v.dimup = s.sp[0].dim;
   v.dimdw = s.sp[1].dim;
+  v.ld = ld;
   v.kup = s.sp[0].kmax;
   v.kdw = s.sp[1].kmax;
   v.map_up = s.sp[0].map;
@@ -336,25 +349,35 @@ void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, co
   }
 }
 
+int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
+  const int npan = (V.dimup + cfg.R - 1) / cfg.R;
+  return 8 * cfg.nchunk * ((npan + 7) / 8);
+}
+
 template <typename T, int MODE>
 static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA& A,
                         const TwoPassCfg& cfg, hipStream_t st) {
-  const int nbx = (cfg.grid_dw % 8 == 0) ? cfg.grid_dw / 8 : cfg.grid_dw;
-  const int ncol = (V.dimdw + nbx - 1) / nbx + 1;
+  EDL_CHECK(cfg.nchunk >= 1 && cfg.nchunk <= V.dimdw, "launch_pass_dw: bad column chunk count");
+  const int ncol = (V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1;
   const size_t lds = sizeof(uint32_t) * (size_t)ncol * (V.kdw + 1);
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 4096, "launch_pass_dw: column table exceeds LDS");
+  const int grid = pass_dw_grid(V, cfg);
   static bool attr = false;
   if (!attr) {
-    for (auto f : {(const void*)k_pass_dw<T, 32, MODE>, (const void*)k_pass_dw<T, 64, MODE>})
+    for (auto f : {(const void*)k_pass_dw<T, 16, MODE>, (const void*)k_pass_dw<T, 32, MODE>,
+                   (const void*)k_pass_dw<T, 64, MODE>})
       EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
     attr = true;
   }
-  if (cfg.R == 32)
-    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
-                       tin, out, A);
+  if (cfg.R == 16)
+    hipLaunchKernelGGL((k_pass_dw<T, 16, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
+                       A, cfg.nchunk);
+  else if (cfg.R == 32)
+    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
+                       A, cfg.nchunk);
   else if (cfg.R == 64)
-    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(cfg.grid_dw), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
-                       tin, out, A);
+    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
+                       A, cfg.nchunk);
   else
     throw Error("launch_pass_dw: unsupported panel height");
   EDL_HIP(hipGetLastError());

@@ -6,6 +6,7 @@ namespace edl {
 
 struct TwoPassView {
   int dimup, dimdw, kup, kdw;
+  int ld;                  // column stride of the vectors (>= dimup)
   const int32_t* map_up;
   const int32_t* map_dw;
   const uint32_t* pk_up;
@@ -20,7 +21,7 @@ struct TwoPassView {
 struct LzA {
   LzScalars* sc;
   double* alpha;
-  double* part;       // [grid_dw]
+  double* part;       // [pass_dw_grid]
   unsigned* counter;
 };
 
@@ -30,11 +31,20 @@ struct TwoPassCfg {
   int grid_up = 256;
   int R = 32;           // rows per down-pass panel
   int threads_dw = 256;
-  int grid_dw = 2048;   // multiple of 8
+  int nchunk = 32;      // column chunks per down-pass panel (one block each)
 };
 
+// Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)); the size of
+// LzA::part.
+int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg);
+
 bool two_pass_ok(const Sector& s, size_t elem);
-TwoPassView two_pass_view(const Sector& s);
+// Column stride of the engine's own vectors: DimUp rounded up to a whole
+// number of 128-byte cache lines of fp32 (32 elements), so that a 32-row
+// fp32 panel segment (16-row fp64) of any column is exactly one line.
+int padded_ld(int dimup);
+// ld = DimUp for vectors in the natural layout i = iup + idw * DimUp.
+TwoPassView two_pass_view(const Sector& s, int ld);
 
 // Operator mode: sc = null, input qa.  Chain mode: input q_k = (k odd ? qb : qa).
 template <typename T>

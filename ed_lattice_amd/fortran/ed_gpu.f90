@@ -21,7 +21,7 @@ module ED_GPU
   public :: edl_device_alloc, edl_copy_h2d, edl_copy_d2h, edl_vec_random, edl_vec_dot
   public :: edl_lanczos_gs, edl_lanczos_stats, edl_set_state, edl_get_state
   public :: edl_tridiag, edl_tridiag_batched, edl_lanczos_poles, edl_observables
-  public :: edl_last_timing, edl_bench_hxv
+  public :: edl_last_timing, edl_bench_hxv, edl_bench_passes
   public :: edl_check, edl_error_message
 
   interface
@@ -230,6 +230,14 @@ module ED_GPU
        integer(c_int), value :: isector, prec, reps
        real(c_double) :: ms_per_hxv
      end function edl_bench_hxv
+
+     integer(c_int) function edl_bench_passes(ctx, isector, prec, reps, ms_pass, cfg) bind(C, name="edl_bench_passes")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: isector, prec, reps
+       real(c_double) :: ms_pass(2)
+       integer(c_int32_t) :: cfg(8)
+     end function edl_bench_passes
 
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr

@@ -123,6 +123,23 @@ def test_hxv_fp64_and_fp32_vs_oracle(ed, orc, name, mk, sec):
     assert _relerr(ed.direct_matvec(isec, vc, "fp64"), refc) < 1e-13
 
 
+@pytest.mark.parametrize("prec,dt,tol", [("fp64", np.float64, 1e-13), ("fp32", np.float32, 2e-6)])
+def test_hxv_device_buffers_natural_layout(ed, orc, prec, dt, tol):
+    """edl_hxv on caller buffers (natural layout i = iup + idw*DimUp, no
+    padding) against the oracle on the same vector."""
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(7, 6)
+    dim, _, _ = ed.build_sector(isec)
+    nb = dim * np.dtype(dt).itemsize
+    v, hv = ed.alloc(nb), ed.alloc(nb)
+    ed.vec_random(prec, v, dim, 77)
+    ed.hxv_device(isec, prec, v, hv)
+    vh = ed.copy_to_host(v, np.empty(dim, dt)).astype(np.float64)
+    got = ed.copy_to_host(hv, np.empty(dim, dt)).astype(np.float64)
+    ref = orc.matvec(m, 7, 6, vh.astype(np.complex128), nthreads=8).real
+    assert _relerr(got, ref) < tol
+
+
 def test_hxv_rect3x4_gs_sector(ed, orc):
     m = _use(ed, orc, lattice.square(3, 4), [6.0])
     isec = ed.sector_index(6, 6)
@@ -256,10 +273,20 @@ def test_tridiag_seed_vs_oracle(ed, orc):
     np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8 * n2)
 
 
-@pytest.mark.parametrize("prec,tol,conv_tol,lead", [("fp64", 1e-8, 1e-10, 1e-9), ("fp32", 2e-4, 1e-5, 1e-4)])
-def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead):
+@pytest.mark.parametrize("prec,tol,conv_tol,lead,nlead,window", [("fp64", 1e-8, 1e-10, 1e-9, 10, 1e-6),
+                                                                 ("fp32", 2e-4, 1e-5, 1e-4, 6, 1e-4)])
+def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, window):
     """Config ring8: all diagonal + off-diagonal GF seeds of spin up in one
-    batched launch, continued fractions vs the oracle's seeds + Lanczos."""
+    batched launch, continued fractions vs the oracle's seeds + Lanczos.
+
+    The GF seeds of this small cluster put their weight on a few poles that
+    converge within a few steps; after that, loss of orthogonality grows the
+    fp32 rounding by about 10x per step (measured: alpha agrees to 1e-6 up to
+    step 6, 2e-3 at step 9), so fp32 is held to the first 6 coefficients,
+    the converged poles and G(z) at broadening 1.  fp32 pole positions carry
+    ~1e-6 absolute error (eps_fp32 * |H|), so their match window is 1e-4: a
+    numpy fp32 Lanczos on the same seeds misses the oracle's weight by 0.3
+    with a 1e-6 window and by 2e-6 with windows 1e-5 .. 1e-3."""
     m = _use(ed, orc, lattice.ring(8), [4.0])
     isec = ed.sector_index(4, 4)
     r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
@@ -282,9 +309,10 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead):
             assert abs(w.sum() - s @ s) < 1e-10
             k = [kk for kk, key in enumerate(gf) if key == (ispin, io, jo)][0]
             sidx = 2 * k + ch
-            _assert_leading(raw["alpha"][sidx], raw["beta"][sidx], ao, bo, 10, lead)
+            _assert_leading(raw["alpha"][sidx], raw["beta"][sidx], ao, bo, nlead, lead)
             g = _ritz(raw["alpha"][sidx], raw["beta"][sidx], raw["norm2"][sidx], e0, isign)
-            _assert_converged_poles_match(g, _ritz(ao, bo, s @ s, e0, isign), tol, conv_tol=conv_tol)
+            _assert_converged_poles_match(g, _ritz(ao, bo, s @ s, e0, isign), tol, window=window,
+                                          conv_tol=conv_tol)
 
 
 def test_gf_batched_equals_sequential_rect3x4(ed, orc):
