@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "../../include/edlat_gpu.h"
+#include "batched2.hpp"
 #include "chains.hpp"
 #include "hxv2.hpp"
 #include "seeds.hpp"
@@ -65,6 +66,25 @@ struct DevBuf {
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// Bump allocator over one device allocation: the buffers of a chain group
+// cost one hipMalloc.
+struct Arena {
+  DevBuf buf;
+  size_t off = 0, cap = 0;
+  static size_t round(size_t b) { return (b + 255) & ~(size_t)255; }
+  void reserve(size_t bytes) {
+    buf = DevBuf(bytes);
+    cap = bytes;
+    off = 0;
+  }
+  void* take(size_t bytes) {
+    EDL_CHECK(off + round(bytes) <= cap, "arena: out of reserved memory");
+    void* p = static_cast<char*>(buf.p) + off;
+    off += round(bytes);
+    return p;
+  }
 };
 
 struct EventTimer {
@@ -136,8 +156,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R, &p.threads_dw,
                               &p.nchunk);
     EDL_CHECK(n == 6, "EDL_TWO_PASS_CFG_F32/_F64: expected C,threads_up,grid_up,R,threads_dw,nchunk");
-    EDL_CHECK((p.C == 1 || p.C == 2 || p.C == 4) && (p.R == 16 || p.R == 32 || p.R == 64),
-              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1, 2 or 4 and R 16, 32 or 64");
+    EDL_CHECK(p.C >= 1 && p.C <= 4 && (p.R == 16 || p.R == 32 || p.R == 64),
+              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1 to 4 and R 16, 32 or 64");
     EDL_CHECK(p.threads_up % 64 == 0 && p.threads_up >= 64 && p.threads_up <= 1024 && p.threads_dw % 64 == 0 &&
                   p.threads_dw >= 64 && p.threads_dw <= 1024 && p.grid_up >= 1 && p.nchunk >= 1 &&
                   p.nchunk <= V.dimdw,
@@ -158,7 +178,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   best.nchunk = std::min(32, V.dimdw);
   double best_up = 1e300, best_dw = 1e300;
   const int lds = 160 * 1024;
-  for (int C : {1, 2, 4})
+  for (int C : {1, 2, 3, 4})
     for (int th : {512, 1024})
       for (int gm : {1, 2}) {
         const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
@@ -216,7 +236,8 @@ static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, int ld, hipStre
     const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
     const TwoPassView V = two_pass_view(s, ld);
-    launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v, v, tmp, nullptr, cfg, st);
+    T* vin = const_cast<T*>(v);  // operator mode only reads it
+    launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, cfg, st);
     launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), const_cast<T*>(v), const_cast<T*>(v), tmp, out, nullptr, cfg, st);
   } else {
     EDL_CHECK(ld == s.sp[0].dim, "apply_h: the single-pass operator takes the natural layout");
@@ -239,6 +260,17 @@ struct ChainOut {
 template <typename T>
 struct ChainAlloc {
   DevBuf q0, q1, sc, alpha, beta, counter;
+  Arena* ar = nullptr;  // carve from this instead of allocating, when set
+  void* get(DevBuf& slot, size_t bytes) {
+    if (ar) return ar->take(bytes);
+    slot = DevBuf(bytes);
+    return slot.p;
+  }
+  // arena bytes of one chain of D stored elements and nmax steps
+  static size_t arena_bytes(int64_t D, int nmax) {
+    return 2 * Arena::round(sizeof(T) * D) + Arena::round(sizeof(LzScalars)) +
+           2 * Arena::round(sizeof(double) * nmax) + Arena::round(sizeof(unsigned));
+  }
 };
 
 // d_start is in the natural layout; the stored vectors get column stride ld
@@ -250,19 +282,19 @@ static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, 
   const int64_t D = (int64_t)ld * S.sp[1].dim;
   fill_sector_view(S, sizeof(T) == 4, cd);
   cd.ld = ld;
-  A.q0 = DevBuf(sizeof(T) * D);
-  A.q1 = DevBuf(sizeof(T) * D);
-  A.sc = DevBuf(sizeof(LzScalars));
-  A.alpha = DevBuf(sizeof(double) * nmax);
-  A.beta = DevBuf(sizeof(double) * nmax);
-  A.counter = DevBuf(sizeof(unsigned));
-  if (ld != S.sp[0].dim) launch_zero_bytes(A.q0.p, sizeof(T) * D, st);
-  launch_convert_ld<T>(d_start, A.q0.template as<T>(), S.sp[0].dim, S.sp[1].dim, ld, st);
-  launch_zero_bytes(A.q1.p, sizeof(T) * D, st);
-  launch_zero_bytes(A.alpha.p, sizeof(double) * nmax, st);
-  launch_zero_bytes(A.beta.p, sizeof(double) * nmax, st);
-  launch_zero_bytes(A.counter.p, 4, st);
-  const double ss = device_sumsq<T>(A.q0.template as<T>(), D, st);
+  T* q0 = static_cast<T*>(A.get(A.q0, sizeof(T) * D));
+  T* q1 = static_cast<T*>(A.get(A.q1, sizeof(T) * D));
+  LzScalars* sc = static_cast<LzScalars*>(A.get(A.sc, sizeof(LzScalars)));
+  double* alpha = static_cast<double*>(A.get(A.alpha, sizeof(double) * nmax));
+  double* beta = static_cast<double*>(A.get(A.beta, sizeof(double) * nmax));
+  unsigned* counter = static_cast<unsigned*>(A.get(A.counter, sizeof(unsigned)));
+  if (ld != S.sp[0].dim) launch_zero_bytes(q0, sizeof(T) * D, st);
+  launch_convert_ld<T>(d_start, q0, S.sp[0].dim, S.sp[1].dim, ld, st);
+  launch_zero_bytes(q1, sizeof(T) * D, st);
+  launch_zero_bytes(alpha, sizeof(double) * nmax, st);
+  launch_zero_bytes(beta, sizeof(double) * nmax, st);
+  launch_zero_bytes(counter, 4, st);
+  const double ss = device_sumsq<T>(q0, D, st);
   LzScalars s{};
   s.k = 0;
   s.stopped = (norm2 == 0.0 || ss == 0.0) ? 1 : 0;
@@ -272,15 +304,15 @@ static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, 
   s.beta_k = 0.0;
   s.alpha_k = 0.0;
   s.norm2 = norm2;
-  EDL_HIP(hipMemcpyAsync(A.sc.p, &s, sizeof(s), hipMemcpyHostToDevice, st));
-  cd.q0 = A.q0.p;
-  cd.q1 = A.q1.p;
+  EDL_HIP(hipMemcpyAsync(sc, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  cd.q0 = q0;
+  cd.q1 = q1;
   cd.x = nullptr;
   cd.ccoef = nullptr;
-  cd.counter = A.counter.template as<unsigned>();
-  cd.sc = A.sc.template as<LzScalars>();
-  cd.alpha = A.alpha.template as<double>();
-  cd.beta = A.beta.template as<double>();
+  cd.counter = counter;
+  cd.sc = sc;
+  cd.alpha = alpha;
+  cd.beta = beta;
   cd.nmax = nmax;
   cd.brk_tol = 1e-14;
 }
@@ -347,7 +379,7 @@ struct TwoPassStep {
   const T* lut_up;
   const T* lut_dw;
   T* tmp;
-  LzA A;
+  LzChain L;
 };
 
 template <typename T>
@@ -357,14 +389,17 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
   hipGraphExec_t ge = nullptr;
   EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   if (tp) {
+    // up (+ phase B of the previous step), beta, down (+ phase A), alpha
     T* q0 = static_cast<T*>(cs.h[0].q0);
     T* q1 = static_cast<T*>(cs.h[0].q1);
-    launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, cs.h[0].sc, tp->cfg, st);
-    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->A, tp->cfg, st);
+    launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
+    launch_lz_beta(tp->L, tp->V, tp->cfg, st);
+    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
+    launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
   } else {
     lz_phase_a(c, cs, st);
+    lz_phase_b(c, cs, st);
   }
-  lz_phase_b(c, cs, st);
   EDL_HIP(hipStreamEndCapture(st, &g));
   EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
   double e_prev = 0.0;
@@ -394,6 +429,37 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
   (void)hipGraphExecDestroy(ge);
   (void)hipGraphDestroy(g);
   return done;
+}
+
+// EDL_BATCHED_KERNELS=single keeps batched chains on the single-pass kernels
+// (tests run both paths on the same seeds).
+static bool batched_single_pass_forced() {
+  const char* e = std::getenv("EDL_BATCHED_KERNELS");
+  return e && std::strcmp(e, "single") == 0;
+}
+
+// EDL_VERBOSE=1: phase timings of the drivers on stderr.
+static bool verbose() {
+  const char* e = std::getenv("EDL_VERBOSE");
+  return e && *e && *e != '0';
+}
+
+// `steps` Lanczos steps of every chain: up pass (with the previous step's
+// phase B), down pass + phase A; captured once as a graph and replayed.
+template <typename T>
+static int run_steps_bt(Ctx& c, const BtSet& bs, int steps, hipStream_t st) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  bt_launch_up<T>(bs, c.model.inter, st);
+  bt_launch_dw<T>(bs, st);
+  EDL_HIP(hipStreamEndCapture(st, &g));
+  EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  for (int k = 0; k < steps; ++k) EDL_HIP(hipGraphLaunch(ge, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return steps;
 }
 
 template <typename T>
@@ -437,11 +503,10 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   const int nmax = (int)std::min<int64_t>(niter, S.dim);
   GsRun r;
   Schedule sch;
-  // Large sectors: phase A by the two-pass operator (its partial sums and
-  // counter are separate from phase B's).
+  // Sectors whose columns fit in LDS: the two-pass step (LzChain).
   const bool two = two_pass_ok(S, sizeof(T));
   TwoPassStep<T> tps{};
-  DevBuf tp_part, tp_counter;
+  DevBuf tp_up, tp_dw;
   // Two-pass chains keep their vectors with the padded column stride.
   const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
@@ -450,15 +515,22 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     tps.lut_up = lut_dev<T>(S.sp[0]);
     tps.lut_dw = lut_dev<T>(S.sp[1]);
     tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * S.sp[1].dim));
-    tp_part = DevBuf(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
-    tp_counter = DevBuf(sizeof(unsigned));
-    launch_zero_bytes(tp_counter.p, 4, st);
-    tps.A.part = tp_part.as<double>();
-    tps.A.counter = tp_counter.as<unsigned>();
+    tp_up = DevBuf(sizeof(double) * pass_up_grid(tps.V, tps.cfg));
+    tp_dw = DevBuf(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
+    tps.L.part_up = tp_up.as<double>();
+    tps.L.part_dw = tp_dw.as<double>();
     sch.block_threads = 256;
-
     sch.grid = c.num_cu * 8;
   }
+  auto bind = [&](const ChainDev& cd, int n, double* x, const double* ccoef) {
+    tps.L.sc = cd.sc;
+    tps.L.alpha = cd.alpha;
+    tps.L.beta = cd.beta;
+    tps.L.nmax = n;
+    tps.L.brk_tol = cd.brk_tol;
+    tps.L.x = x;
+    tps.L.ccoef = ccoef;
+  };
   {
     ChainSet<T> cs;
     ChainAlloc<T> A;
@@ -467,8 +539,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     if (!two) sch = autotune<T>(c, {cd}, st);
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
-    tps.A.sc = cd.sc;
-    tps.A.alpha = cd.alpha;
+    bind(cd, nmax, nullptr, nullptr);
     r.hxv += run_steps(c, cs, nmax, check, tol, st, two ? &tps : nullptr);
     r.out = read_chain<T>(cs.h[0], nmax, st);
   }
@@ -487,9 +558,11 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     cd.ccoef = dc.as<double>();
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
-    tps.A.sc = cd.sc;
-    tps.A.alpha = cd.alpha;
-    r.hxv += run_steps(c, cs, m, 0, 0.0, st, two ? &tps : nullptr);
+    bind(cd, m, d_x, dc.as<double>());
+    // two-pass: one more launch, whose up pass adds the replay term of the
+    // last step (its phase B is otherwise not needed)
+    r.hxv += m;
+    run_steps(c, cs, two ? m + 1 : m, 0, 0.0, st, two ? &tps : nullptr);
     const double n2 = device_dot(c, d_x, d_x, S.dim, st);
     launch_scale(d_x, S.dim, 1.0 / std::sqrt(n2), st);
     EDL_HIP(hipStreamSynchronize(st));
@@ -940,6 +1013,90 @@ static int target_sector(const Ctx& c, int isector, const SeedSpec& s) {
   return 1 + u * (Ns + 1) + w;
 }
 
+static std::map<std::tuple<int, int, int, int>, BtShape>& bt_cache() {
+  static std::map<std::tuple<int, int, int, int>, BtShape> m;
+  return m;
+}
+
+static std::vector<BtChain> bt_resized(std::vector<BtChain> v, const BtShape& s) {
+  for (BtChain& b : v) {
+    b.nslab = (b.V.dimup + s.RS - 1) / s.RS;
+    b.npanel = (b.V.dimdw + s.C - 1) / s.C;
+  }
+  return v;
+}
+
+// Panel width, slab height and block sizes of the batched two-pass kernels,
+// by timing the up and down launches on the chain set itself at step 0,
+// where both are repeatable (the up pass has no pending phase B; the down
+// pass multiplies q_{-1} by beta_0 = 0); the scalars are put back after.
+template <typename T>
+static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, int max_dw, hipStream_t st) {
+  const auto key = std::make_tuple((int)sizeof(T), max_up, max_dw, (int)chains.size());
+  auto it = bt_cache().find(key);
+  if (it != bt_cache().end()) return it->second;
+  BtShape best = bt_shape(max_up, max_dw, sizeof(T));
+  const size_t head = 256 * sizeof(T) + 16, budget = 150 * 1024;
+  std::vector<LzScalars> snap(chains.size());
+  for (size_t k = 0; k < chains.size(); ++k)
+    EDL_HIP(hipMemcpyAsync(&snap[k], chains[k].sc, sizeof(LzScalars), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  auto time = [&](const BtShape& shp, bool up) {
+    BtSet bs;
+    bs.shape = shp;
+    bs.h = bt_resized(chains, shp);
+    bt_finalize<T>(bs);
+    for (int r = 0; r < 3; ++r) {
+      if (r == 1) EDL_HIP(hipStreamSynchronize(st));
+      EventTimer tm(st);
+      if (up)
+        bt_launch_up<T>(bs, c.model.inter, st);
+      else
+        bt_launch_dw<T>(bs, st);
+      const double ms = tm.stop();
+      if (r == 2) return ms;
+    }
+    return 0.0;
+  };
+  double bu = 1e300, bd = 1e300;
+  for (int C : {1, 2, 4}) {
+    if (head + (size_t)C * max_up * sizeof(T) > budget) continue;
+    for (int th : {256, 512, 1024}) {
+      BtShape w = best;
+      w.C = C;
+      w.threads_up = th;
+      const double ms = time(w, true);
+      if (ms < bu) {
+        bu = ms;
+        best.C = C;
+        best.threads_up = th;
+      }
+    }
+  }
+  for (int RS : {32, 16, 8, 4, 2, 1}) {
+    if ((size_t)RS * sizeof(T) > 128 || head + (size_t)RS * max_dw * sizeof(T) > budget) continue;
+    for (int th : {256, 512, 1024}) {
+      BtShape w = best;
+      w.RS = RS;
+      w.threads_dw = th;
+      const double ms = time(w, false);
+      if (ms < bd) {
+        bd = ms;
+        best.RS = RS;
+        best.threads_dw = th;
+      }
+    }
+  }
+  for (size_t k = 0; k < chains.size(); ++k)
+    EDL_HIP(hipMemcpyAsync(chains[k].sc, &snap[k], sizeof(LzScalars), hipMemcpyHostToDevice, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  if (verbose())
+    std::fprintf(stderr, "[edl] batched two-pass shape for %zu chains: C %d x %d threads (%.3f ms), RS %d x %d (%.3f ms)\n",
+                 chains.size(), best.C, best.threads_up, bu, best.RS, best.threads_dw, bd);
+  bt_cache()[key] = best;
+  return best;
+}
+
 template <typename T>
 static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const std::vector<int>& idx,
                              const std::vector<int>& targets, int nlanc, double* alpha, double* beta, double* norm2,
@@ -947,10 +1104,36 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   hipStream_t st = c.stream;
   std::vector<ChainAlloc<T>> allocs(idx.size());
   std::vector<ChainDev> chains;
+  std::vector<BtChain> btchains;
   std::vector<int> nmaxs;
   std::vector<int> which;
   int64_t maxdim = 0;
-  for (size_t k = 0; k < idx.size(); ++k) maxdim = std::max<int64_t>(maxdim, get_sector(c, targets[idx[k]]).dim);
+  int max_up = 1, max_dw = 1;
+  bool packed = true;
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const Sector& J = get_sector(c, targets[idx[k]]);
+    maxdim = std::max<int64_t>(maxdim, J.dim);
+    max_up = std::max(max_up, J.sp[0].dim);
+    max_dw = std::max(max_dw, J.sp[1].dim);
+    packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok;
+  }
+  // batched two-pass kernels when every target sector fits them
+  const bool bt = packed && bt_shape(max_up, max_dw, sizeof(T)).RS > 0 && !batched_single_pass_forced();
+  // every buffer of the group in one allocation
+  Arena ar;
+  {
+    size_t bytes = 0;
+    for (size_t k = 0; k < idx.size(); ++k) {
+      const Sector& J = get_sector(c, targets[idx[k]]);
+      const int ld = bt ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
+      const int64_t D = (int64_t)ld * J.sp[1].dim;
+      bytes += ChainAlloc<T>::arena_bytes(D, (int)std::min<int64_t>(nlanc, J.dim));
+      if (bt)
+        bytes += Arena::round(sizeof(T) * D) + Arena::round(sizeof(double) * J.sp[0].dim) +
+                 Arena::round(sizeof(double) * J.sp[1].dim);
+    }
+    ar.reserve(bytes);
+  }
   DevBuf tmp(sizeof(double) * maxdim);
   for (size_t k = 0; k < idx.size(); ++k) {
     const int s = idx[k];
@@ -964,7 +1147,26 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     const double n2 = device_dot(c, tmp.as<double>(), tmp.as<double>(), J.dim, st);
     const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
     ChainDev cd{};
-    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), n2, nmax, st);
+    const int ld = bt ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
+    allocs[k].ar = &ar;
+    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), n2, nmax, st, ld);
+    if (bt) {
+      BtChain b{};
+      b.V = two_pass_view(J, ld);
+      b.lut_up = lut_dev<T>(J.sp[0]);
+      b.lut_dw = lut_dev<T>(J.sp[1]);
+      b.q0 = cd.q0;
+      b.q1 = cd.q1;
+      b.nmax = nmax;
+      b.brk_tol = cd.brk_tol;
+      b.tb = ar.take(sizeof(T) * (size_t)ld * J.sp[1].dim);
+      b.partD = static_cast<double*>(ar.take(sizeof(double) * J.sp[0].dim));  // room for any slab height
+      b.partU = static_cast<double*>(ar.take(sizeof(double) * J.sp[1].dim));  // and any panel width
+      b.sc = cd.sc;
+      b.alpha = cd.alpha;
+      b.beta = cd.beta;
+      btchains.push_back(b);
+    }
     EDL_HIP(hipStreamSynchronize(st));  // tmp is reused by the next seed
     norm2[s] = n2;
     chains.push_back(cd);
@@ -972,12 +1174,31 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     which.push_back(s);
   }
   if (chains.empty()) return 0;
-  Schedule sch = autotune<T>(c, chains, st);
+  const int steps = *std::max_element(nmaxs.begin(), nmaxs.end());
   ChainSet<T> cs;
   cs.h = chains;
-  chainset_finalize(c, cs, sch);
-  const int steps = *std::max_element(nmaxs.begin(), nmaxs.end());
-  run_steps(c, cs, steps, 0, 0.0, st);
+  BtShape shp{};
+  BtSet bs;
+  if (bt) {
+    shp = bt_tune<T>(c, btchains, max_up, max_dw, st);
+    bs.shape = shp;
+    bs.h = bt_resized(btchains, shp);
+    bt_finalize<T>(bs);
+  }
+  EventTimer run_tm(st);
+  if (bt) {
+    run_steps_bt<T>(c, bs, steps, st);
+  } else {
+    Schedule sch = autotune<T>(c, chains, st);
+    chainset_finalize(c, cs, sch);
+    run_steps(c, cs, steps, 0, 0.0, st);
+  }
+  if (verbose()) {
+    const double ms = run_tm.stop();
+    std::fprintf(stderr, "[edl] batched group: %zu chains, %s kernels (C %d, RS %d), %d steps in %.3f ms (%.1f us/step)\n",
+                 chains.size(), bt ? "two-pass" : "single-pass", bt ? shp.C : 0, bt ? shp.RS : 0, steps, ms,
+                 1e3 * ms / steps);
+  }
   int64_t hxv = 0;
   for (size_t k = 0; k < chains.size(); ++k) {
     ChainOut o = read_chain<T>(cs.h[k], nmaxs[k], st);
@@ -1050,7 +1271,8 @@ extern "C" int edl_tridiag_batched(edl_ctx* ctx, int nseeds, const edl_seed_desc
     EDL_HIP(hipMemGetInfo(&free_b, &total_b));
     int64_t maxdim = 0;
     for (int s : active) maxdim = std::max<int64_t>(maxdim, get_sector(c, targets[s]).dim);
-    const double per_chain = 2.0 * (double)maxdim * (prec == EDL_FP64 ? 8.0 : 4.0);
+    // q0, q1 and the two-pass intermediate, with the padded column stride
+    const double per_chain = 3.0 * 1.03 * (double)maxdim * (prec == EDL_FP64 ? 8.0 : 4.0);
     const double budget = 0.7 * (double)free_b - 8.0 * (double)maxdim;
     const int fit = (int)std::max(1.0, budget / per_chain);
     g = std::min<int>(g, fit);

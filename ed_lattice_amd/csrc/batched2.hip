@@ -1,0 +1,341 @@
+// Batched two-pass Lanczos steps for small sectors (the GF / chi seeds).
+//
+// Up tiles (chain, panel of C columns) run the same LDS column pass as the
+// single-sector operator.  Down tiles (chain, slab of RS rows) stage the
+// slab -- RS rows of every column, i.e. RS contiguous elements per column,
+// one cache line when RS * sizeof(T) = 128 -- in LDS and gather every
+// down-hop neighbour there, so both passes stream q, T and q_{k-1} through
+// HBM once and keep all gathers on chip.  The down tile fuses Lanczos
+// phase A, the up tile the previous step's phase B; their per-tile partial
+// sums are added in tile order by one block per chain in a following
+// launch (deterministic).
+#include <algorithm>
+
+#include "batched2.hpp"
+
+namespace edl {
+
+// Block b of a grid of 8*n8 blocks takes tile (b % 8) * n8 + b / 8: the
+// blocks of one XCD take a contiguous range of the tile list, so the slabs
+// of one chain that share cache lines run on the same XCD.
+static __device__ __forceinline__ int64_t xcd_tile(int64_t n8) {
+  return (int64_t)(blockIdx.x % 8) * n8 + blockIdx.x / 8;
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
+                                                const int32_t* __restrict__ tchain, InterParams I, int64_t Tall,
+                                                int64_t n8) {
+  typedef typename Vec<T, C>::type VT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut = reinterpret_cast<T*>(smem);
+  VT* P = reinterpret_cast<VT*>(smem + lds_head_bytes(sizeof(T)));
+  __shared__ double sh[16];
+  const int64_t t = xcd_tile(n8);
+  if (t >= Tall) return;
+  const int c = tchain[t];
+  const BtChain& B = chains[c];
+  const LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const TwoPassView V = B.V;
+  const int pan = (int)(t - tile0[c]);
+  const int c0 = pan * C;
+  const T* lsrc = reinterpret_cast<const T*>(B.lut_up);
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  // Phase B of step k pending: r = t - alpha_k q_k / s_k, stored over t (the
+  // buffer that becomes q_{k+1}); this panel of r is then the up-pass input.
+  // Otherwise (first launch) the input is q_0.
+  const bool pend = s.pad != 0;
+  T* __restrict__ qn = reinterpret_cast<T*>(((pend ? s.k + 1 : s.k) & 1) ? B.q1 : B.q0);
+  const T* __restrict__ qk = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+  const T am = (T)(s.alpha_k / s.s_cur);
+  __syncthreads();
+  const double nrm = stage_panel<T, C>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
+  __syncthreads();
+  up_rows<T, C>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
+  if (!pend) return;
+  const double v = block_sum(nrm, sh);
+  if (threadIdx.x == 0) B.partU[pan] = v;
+}
+
+// Deterministic sum of part[0..n) by one block (contiguous slice per thread,
+// fixed tree); thread 0 holds the result.
+static __device__ double ordered_sum(const double* __restrict__ part, int n, double* sh) {
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int a = threadIdx.x * per, e = min(a + per, n);
+  double x = 0.0;
+  for (int k = a; k < e; ++k) x += part[k];
+  return block_sum(x, sh);
+}
+
+// One block per chain, after the up launch: beta_{k+1} = |r| from the
+// panel partials, and the step's scalars advance.  The kernel boundary
+// makes every panel's partial visible (no fence inside the hot kernels).
+__global__ __launch_bounds__(256) void k_bt_beta(const BtChain* __restrict__ chains) {
+  __shared__ double sh[16];
+  const BtChain& B = chains[blockIdx.x];
+  const LzScalars s = *B.sc;
+  if (s.stopped || !s.pad) return;
+  const double s2 = ordered_sum(B.partU, B.npanel, sh);
+  if (threadIdx.x == 0) {
+    const double b = sqrt(s2);
+    LzScalars n = s;
+    n.pad = 0;
+    const bool more = (s.k + 1 < B.nmax);
+    if (more) B.beta[s.k + 1] = b;
+    const bool brk = !(b > B.brk_tol * (fabs(s.alpha_k) + s.beta_k));
+    if (!more || brk) {
+      n.stopped = 1;
+    } else {
+      n.s_old = s.s_cur;
+      n.s_cur = b;
+      n.beta_k = b;
+      n.k = s.k + 1;
+    }
+    *B.sc = n;
+  }
+}
+
+// One block per chain, after the down launch: alpha_k from the slab
+// partials; phase B of step k becomes pending.
+__global__ __launch_bounds__(256) void k_bt_alpha(const BtChain* __restrict__ chains) {
+  __shared__ double sh[16];
+  const BtChain& B = chains[blockIdx.x];
+  const LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const double a = ordered_sum(B.partD, B.nslab, sh);
+  if (threadIdx.x == 0) {
+    B.alpha[s.k] = a;
+    LzScalars n = s;
+    n.alpha_k = a;
+    n.nsteps = s.k + 1;
+    n.pad = 1;
+    if (s.k + 1 >= B.nmax) n.stopped = 1;
+    *B.sc = n;
+  }
+}
+
+template <typename T, int RS>
+__global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
+                                                const int32_t* __restrict__ tchain, int64_t Tall, int64_t n8) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut = reinterpret_cast<T*>(smem);
+  T* S = reinterpret_cast<T*>(smem + lds_head_bytes(sizeof(T)));  // [dimdw][RS]
+  __shared__ double sh[16];
+  const int64_t t = xcd_tile(n8);
+  if (t >= Tall) return;
+  const int c = tchain[t];
+  const BtChain& B = chains[c];
+  const LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const TwoPassView V = B.V;
+  const T* __restrict__ q = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+  T* __restrict__ qo = reinterpret_cast<T*>((s.k & 1) ? B.q0 : B.q1);
+  const T* __restrict__ tb = reinterpret_cast<const T*>(B.tb);
+  const T* lsrc = reinterpret_cast<const T*>(B.lut_dw);
+  const int sl = (int)(t - tile0[c]);
+  const int r0 = sl * RS;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+#pragma unroll 4
+  for (int i = threadIdx.x; i < dimdw * RS; i += blockDim.x) {
+    const int j = i / RS, r = i % RS;
+    S[i] = (r0 + r < dimup) ? q[(size_t)j * ld + r0 + r] : (T)0;
+  }
+  __syncthreads();
+  const double inv_s = 1.0 / s.s_cur;
+  const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
+  // lane = row of the slab (fastest) x column: the global accesses of a wave
+  // are 64 / RS whole slab segments
+  const int r = threadIdx.x % RS, jl = threadIdx.x / RS, jstep = blockDim.x / RS;
+  const bool rok = r0 + r < dimup;
+  double part = 0.0;
+  constexpr int KCH = 8;
+  // Two columns per thread; their T and q_{k-1} elements and 2*KCH hop
+  // words are requested before the LDS gathers that need them.
+  for (int j0 = jl; j0 < dimdw; j0 += 2 * jstep) {
+    int col[2] = {j0, j0 + jstep};
+    const bool okb = col[1] < dimdw;
+    if (!okb) col[1] = j0;
+    const int nh[2] = {V.cnt_dw[col[0]], okb ? V.cnt_dw[col[1]] : 0};
+    size_t ii[2];
+    T tin[2], qold[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      ii[h] = (size_t)col[h] * ld + r0 + r;
+      tin[h] = rok ? tb[ii[h]] : (T)0;
+      qold[h] = rok ? qo[ii[h]] : (T)0;
+    }
+    T acc[2] = {(T)0, (T)0};
+    const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
+    for (int k0 = 0; k0 < nmax; k0 += KCH) {
+      uint32_t w[2][KCH];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < KCH; ++jj)
+          w[h][jj] = (k0 + jj < nh[h]) ? V.pk_dw[(size_t)(k0 + jj) * dimdw + col[h]] : (255u << 24);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jj = 0; jj < KCH; ++jj) acc[h] += lut[w[h][jj] >> 24] * S[(int)(w[h][jj] & 0xFFFFFFu) * RS + r];
+    }
+    if (rok) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !okb) break;
+        const T tv = (tin[h] + acc[h]) * ts - bo * qold[h];
+        qo[ii[h]] = tv;
+        part += (double)tv * (double)S[col[h] * RS + r];
+      }
+    }
+  }
+  const double v = block_sum(part * inv_s, sh);
+  if (threadIdx.x == 0) B.partD[sl] = v;
+}
+
+// ---------------------------------------------------------------- host side
+static constexpr size_t kBtUpBudget = 64 * 1024;   // two or more up tiles per CU
+static constexpr size_t kBtDwBudget = 150 * 1024;  // one slab per CU at most
+
+BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem) {
+  BtShape b;
+  b.RS = 0;
+  const size_t head = lds_head_bytes(elem);
+  int C = 0;
+  for (int c : {4, 2, 1})
+    if (head + (size_t)c * max_dimup * elem <= kBtUpBudget) {
+      C = c;
+      break;
+    }
+  if (C == 0 && head + (size_t)max_dimup * elem <= kBtDwBudget) C = 1;
+  if (C == 0) return b;
+  int RS = 0;
+  for (int rs : {32, 16, 8, 4, 2, 1})
+    if ((size_t)rs * elem <= 128 && head + (size_t)rs * max_dimdw * elem <= kBtDwBudget) {
+      RS = rs;
+      break;
+    }
+  if (RS == 0) return b;
+  b.C = C;
+  b.RS = RS;
+  const size_t up_lds = head + (size_t)C * max_dimup * elem;
+  b.threads_up = up_lds > 40 * 1024 ? 1024 : 256;
+  const size_t dw_lds = head + (size_t)RS * max_dimdw * elem;
+  b.threads_dw = dw_lds > 40 * 1024 ? 1024 : 256;
+  return b;
+}
+
+BtSet::~BtSet() {
+  if (d) (void)hipFree(d);
+  if (d_tile_up) (void)hipFree(d_tile_up);
+  if (d_tile_dw) (void)hipFree(d_tile_dw);
+  if (d_chain_up) (void)hipFree(d_chain_up);
+  if (d_chain_dw) (void)hipFree(d_chain_dw);
+}
+
+// LDS limits of every instantiation, set once and outside any graph capture.
+template <typename T>
+static void bt_attributes() {
+  static bool done = false;
+  if (done) return;
+  for (auto f : {(const void*)k_bt_up<T, 1>, (const void*)k_bt_up<T, 2>, (const void*)k_bt_up<T, 4>,
+                 (const void*)k_bt_dw<T, 1>, (const void*)k_bt_dw<T, 2>, (const void*)k_bt_dw<T, 4>,
+                 (const void*)k_bt_dw<T, 8>, (const void*)k_bt_dw<T, 16>, (const void*)k_bt_dw<T, 32>})
+    EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBtDwBudget));
+  done = true;
+}
+
+template <typename T>
+void bt_finalize(BtSet& bs) {
+  bt_attributes<T>();
+  const size_t n = bs.h.size();
+  std::vector<int64_t> tu(n + 1, 0), td(n + 1, 0);
+  for (size_t k = 0; k < n; ++k) {
+    const BtChain& b = bs.h[k];
+    EDL_CHECK(b.nslab == (b.V.dimup + bs.shape.RS - 1) / bs.shape.RS, "bt_finalize: slab count mismatch");
+    tu[k + 1] = tu[k] + (b.V.dimdw + bs.shape.C - 1) / bs.shape.C;
+    td[k + 1] = td[k] + b.nslab;
+  }
+  bs.tiles_up = tu[n];
+  bs.tiles_dw = td[n];
+  EDL_HIP(hipMalloc(&bs.d, sizeof(BtChain) * n));
+  EDL_HIP(hipMalloc(&bs.d_tile_up, sizeof(int64_t) * (n + 1)));
+  EDL_HIP(hipMalloc(&bs.d_tile_dw, sizeof(int64_t) * (n + 1)));
+  EDL_HIP(hipMemcpy(bs.d, bs.h.data(), sizeof(BtChain) * n, hipMemcpyHostToDevice));
+  EDL_HIP(hipMemcpy(bs.d_tile_up, tu.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+  EDL_HIP(hipMemcpy(bs.d_tile_dw, td.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+  std::vector<int32_t> cu((size_t)bs.tiles_up), cdw((size_t)bs.tiles_dw);
+  for (size_t k = 0; k < n; ++k) {
+    for (int64_t t = tu[k]; t < tu[k + 1]; ++t) cu[(size_t)t] = (int32_t)k;
+    for (int64_t t = td[k]; t < td[k + 1]; ++t) cdw[(size_t)t] = (int32_t)k;
+  }
+  EDL_HIP(hipMalloc(&bs.d_chain_up, sizeof(int32_t) * std::max<size_t>(cu.size(), 1)));
+  EDL_HIP(hipMalloc(&bs.d_chain_dw, sizeof(int32_t) * std::max<size_t>(cdw.size(), 1)));
+  EDL_HIP(hipMemcpy(bs.d_chain_up, cu.data(), sizeof(int32_t) * cu.size(), hipMemcpyHostToDevice));
+  EDL_HIP(hipMemcpy(bs.d_chain_dw, cdw.data(), sizeof(int32_t) * cdw.size(), hipMemcpyHostToDevice));
+}
+
+static int max_dim(const BtSet& bs, int which) {
+  int m = 1;
+  for (const BtChain& b : bs.h) m = std::max(m, which == 0 ? b.V.dimup : b.V.dimdw);
+  return m;
+}
+
+template <typename T, int C>
+static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)C * max_dim(bs, 0) * sizeof(T);
+  EDL_CHECK(lds <= kBtDwBudget, "bt_launch_up: panel exceeds LDS");
+  const int64_t n8 = (bs.tiles_up + 7) / 8;
+  hipLaunchKernelGGL((k_bt_up<T, C>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st, bs.d,
+                     bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bt_beta, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int RS>
+static void bt_dw_r(const BtSet& bs, hipStream_t st) {
+  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)RS * max_dim(bs, 1) * sizeof(T);
+  EDL_CHECK(lds <= kBtDwBudget, "bt_launch_dw: slab exceeds LDS");
+  EDL_CHECK(bs.shape.threads_dw % RS == 0, "bt_launch_dw: block size not a multiple of the slab height");
+  const int64_t n8 = (bs.tiles_dw + 7) / 8;
+  hipLaunchKernelGGL((k_bt_dw<T, RS>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
+                     bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bt_alpha, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  switch (bs.shape.C) {
+    case 1: return bt_up_c<T, 1>(bs, I, st);
+    case 2: return bt_up_c<T, 2>(bs, I, st);
+    case 4: return bt_up_c<T, 4>(bs, I, st);
+    default: throw Error("bt_launch_up: unsupported panel width");
+  }
+}
+
+template <typename T>
+void bt_launch_dw(const BtSet& bs, hipStream_t st) {
+  switch (bs.shape.RS) {
+    case 1: return bt_dw_r<T, 1>(bs, st);
+    case 2: return bt_dw_r<T, 2>(bs, st);
+    case 4: return bt_dw_r<T, 4>(bs, st);
+    case 8: return bt_dw_r<T, 8>(bs, st);
+    case 16: return bt_dw_r<T, 16>(bs, st);
+    case 32: return bt_dw_r<T, 32>(bs, st);
+    default: throw Error("bt_launch_dw: unsupported slab height");
+  }
+}
+
+template void bt_finalize<double>(BtSet&);
+template void bt_finalize<float>(BtSet&);
+template void bt_launch_up<double>(const BtSet&, const InterParams&, hipStream_t);
+template void bt_launch_up<float>(const BtSet&, const InterParams&, hipStream_t);
+template void bt_launch_dw<double>(const BtSet&, hipStream_t);
+template void bt_launch_dw<float>(const BtSet&, hipStream_t);
+
+}  // namespace edl

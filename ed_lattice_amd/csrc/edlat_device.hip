@@ -4,3 +4,4 @@
 #include "hxv.hip"
 #include "seeds.hip"
 #include "hxv2.hip"
+#include "batched2.hip"

@@ -38,6 +38,16 @@ struct Vec<T, 2> {
   }
 };
 template <typename T>
+struct Tri {
+  T v[3];
+};
+template <typename T>
+struct Vec<T, 3> {  // packed (3 * sizeof(T) bytes): an fp32 panel of 3 columns of 12870 rows fits in LDS
+  typedef Tri<T> type;
+  static __device__ __forceinline__ T get(const type& v, int j) { return v.v[j]; }
+  static __device__ __forceinline__ void set(type& v, int j, T x) { v.v[j] = x; }
+};
+template <typename T>
 struct Vec<T, 4> {
   typedef T type __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ T get(const type& v, int j) { return v[j]; }
@@ -56,92 +66,185 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
 }
 
 // ------------------------------------------------------------------ pass U
-// With a chain (sc != null) the input is q_k = (k odd ? qb : qa), scaled by 1/s_k.
+// One panel of C columns starting at c0: stage them in LDS (interleaved by
+// row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
+// the single-sector kernel and the batched chain kernel run it.
 template <typename T, int C>
-__global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up,
-                                                  const T* qa, const T* qb, T* __restrict__ tout,
-                                                  const LzScalars* __restrict__ sc) {
+static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
+                                               const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
+                                               T ts);
+
+template <typename T, int C>
+static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
+                                                typename Vec<T, C>::type* P, const T* __restrict__ q,
+                                                T* __restrict__ tout, int c0, T ts) {
+  typedef typename Vec<T, C>::type VT;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  __syncthreads();
+  for (int r = threadIdx.x; r < dimup; r += blockDim.x) {
+    VT x;
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+      Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0);
+    P[r] = x;
+  }
+  __syncthreads();
+  up_rows<T, C>(V, I, lut, P, tout, c0, ts);
+}
+
+// The row loop of a panel already staged in P.
+template <typename T, int C>
+static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
+                                               const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
+                                               T ts) {
+  typedef typename Vec<T, C>::type VT;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  uint32_t sd[C];
+  double dgd[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    const int c = (c0 + j < dimdw) ? c0 + j : c0;
+    sd[j] = (uint32_t)V.map_dw[c];
+    dgd[j] = V.dg_dw[c];
+  }
+  // Two rows per thread, hop words fetched KCH at a time, so that 2*KCH
+  // table loads are in flight before the LDS gathers that depend on them.
+  constexpr int KCH = 8;
+  const int stride = 2 * blockDim.x;
+  for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
+    int row[2] = {i0, i0 + (int)blockDim.x};
+    const bool okb = row[1] < dimup;
+    if (!okb) row[1] = i0;
+    T acc[2][C];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t su = (uint32_t)V.map_up[row[h]];
+      const double dgu = V.dg_up[row[h]];
+      const VT self = P[row[h]];
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+    }
+    const int nh[2] = {V.cnt_up[row[0]], okb ? V.cnt_up[row[1]] : 0};
+    const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
+    for (int k0 = 0; k0 < nmax; k0 += KCH) {
+      uint32_t w[2][KCH];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < KCH; ++j)
+          w[h][j] = (k0 + j < nh[h]) ? V.pk_up[(size_t)(k0 + j) * dimup + row[h]] : (255u << 24);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < KCH; ++j) {
+          const T cf = lut[w[h][j] >> 24];
+          const VT x = P[w[h][j] & 0xFFFFFFu];
+#pragma unroll
+          for (int c = 0; c < C; ++c) acc[h][c] += cf * Vec<T, C>::get(x, c);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && !okb) break;
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * ld + row[h]]);
+    }
+  }
+}
+
+static __host__ __device__ constexpr size_t lds_head_bytes(size_t sz) { return 256 * sz + (16 - (256 * sz) % 16) % 16; }
+
+// Stage columns c0 .. c0+C-1 of the up-pass input in P.  Without phase B
+// the input is q.  With phase B of step k pending it is r = t - am * q_k
+// (t read from qn, r stored back over it unless `keep_t`), and the block's
+// share of |r|^2 is returned; x (natural layout) accumulates cx * q_k for
+// the eigenvector replay when given.  Two rows per thread, every load of
+// both rows in flight first.
+template <typename T, int C>
+static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typename Vec<T, C>::type* P, T* qn,
+                                                     const T* __restrict__ qk, int c0, bool pend, bool keep_t,
+                                                     T am, double* __restrict__ x, double cx) {
+  typedef typename Vec<T, C>::type VT;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  double nrm = 0.0;
+  for (int r0 = threadIdx.x; r0 < dimup; r0 += 2 * blockDim.x) {
+    const int rr[2] = {r0, r0 + (int)blockDim.x};
+    T a[2][C], b[2][C];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const bool ok = rr[h] < dimup && c0 + j < dimdw;
+        const size_t i = (size_t)(c0 + j) * ld + rr[h];
+        a[h][j] = ok ? qn[i] : (T)0;
+        b[h][j] = (ok && pend) ? qk[i] : (T)0;
+      }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (rr[h] >= dimup) break;
+      VT v;
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        T e = a[h][j];
+        if (pend && c0 + j < dimdw) {
+          e = a[h][j] - am * b[h][j];
+          if (!keep_t) qn[(size_t)(c0 + j) * ld + rr[h]] = e;
+          nrm += (double)e * (double)e;
+          if (x) x[(size_t)(c0 + j) * dimup + rr[h]] += cx * (double)b[h][j];
+        }
+        Vec<T, C>::set(v, j, e);
+      }
+      P[rr[h]] = v;
+    }
+  }
+  return nrm;
+}
+
+// Operator mode (chain = 0): T = [D + H_up] qa.  Chain mode: phase B of the
+// previous step where one is pending (r = t - alpha_k v_k over t, block
+// partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
+// T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
+// has run its nmax steps only the replay term remains.
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
+                                                  T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
   extern __shared__ __align__(16) unsigned char smem[];
   T* lut = reinterpret_cast<T*>(smem);
-  VT* P = reinterpret_cast<VT*>(smem + 256 * sizeof(T) + (16 - (256 * sizeof(T)) % 16) % 16);
-  double scale = 1.0;
-  const T* __restrict__ q = qa;
-  if (sc) {
-    const LzScalars s = *sc;
-    if (s.stopped) return;
-    scale = 1.0 / s.s_cur;
-    q = (s.k & 1) ? qb : qa;
+  VT* P = reinterpret_cast<VT*>(smem + lds_head_bytes(sizeof(T)));
+  __shared__ double sh[16];
+  const int npan = (V.dimdw + C - 1) / C;
+  if (!chain) {
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+    for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) up_panel<T, C>(V, I, lut, P, qa, tout, pan * C, (T)1);
+    return;
   }
+  const LzScalars s = *L.sc;
+  if (s.stopped) return;
+  const bool pend = s.pad != 0;
+  const bool last = pend && s.k + 1 >= L.nmax;
+  if (last && !L.x) return;
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
-  const int dimup = V.dimup, dimdw = V.dimdw;
-  const size_t ld = (size_t)V.ld;
-  const int npan = (dimdw + C - 1) / C;
-  const T ts = (T)scale;
+  T* qn = ((pend ? s.k + 1 : s.k) & 1) ? qb : qa;
+  const T* qk = (s.k & 1) ? qb : qa;
+  const T am = (T)(s.alpha_k / s.s_cur);
+  double* x = pend ? L.x : nullptr;
+  const double cx = x ? L.ccoef[s.k] / s.s_cur : 0.0;
+  double nrm = 0.0;
   for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
-    const int c0 = pan * C;
     __syncthreads();
-    for (int r = threadIdx.x; r < dimup; r += blockDim.x) {
-      VT x;
-#pragma unroll
-      for (int j = 0; j < C; ++j)
-        Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0);
-      P[r] = x;
-    }
+    nrm += stage_panel<T, C>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
     __syncthreads();
-    uint32_t sd[C];
-    double dgd[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-      const int c = (c0 + j < dimdw) ? c0 + j : c0;
-      sd[j] = (uint32_t)V.map_dw[c];
-      dgd[j] = V.dg_dw[c];
-    }
-    // Two rows per thread, hop words fetched KCH at a time, so that 2*KCH
-    // table loads are in flight before the LDS gathers that depend on them.
-    constexpr int KCH = 8;
-    const int stride = 2 * blockDim.x;
-    for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
-      int row[2] = {i0, i0 + (int)blockDim.x};
-      const bool okb = row[1] < dimup;
-      if (!okb) row[1] = i0;
-      T acc[2][C];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t su = (uint32_t)V.map_up[row[h]];
-        const double dgu = V.dg_up[row[h]];
-        const VT self = P[row[h]];
-#pragma unroll
-        for (int j = 0; j < C; ++j)
-          acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
-      }
-      const int nh[2] = {V.cnt_up[row[0]], okb ? V.cnt_up[row[1]] : 0};
-      const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
-      for (int k0 = 0; k0 < nmax; k0 += KCH) {
-        uint32_t w[2][KCH];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-          for (int j = 0; j < KCH; ++j)
-            w[h][j] = (k0 + j < nh[h]) ? V.pk_up[(size_t)(k0 + j) * dimup + row[h]] : (255u << 24);
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-          for (int j = 0; j < KCH; ++j) {
-            const T cf = lut[w[h][j] >> 24];
-            const VT x = P[w[h][j] & 0xFFFFFFu];
-#pragma unroll
-            for (int c = 0; c < C; ++c) acc[h][c] += cf * Vec<T, C>::get(x, c);
-          }
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (h == 1 && !okb) break;
-#pragma unroll
-        for (int j = 0; j < C; ++j)
-          if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * ld + row[h]]);
-      }
-    }
+    if (!last) up_rows<T, C>(V, I, lut, P, tout, pan * C, (T)1);
+  }
+  if (pend && !last) {
+    const double v = block_sum(nrm, sh);
+    if (threadIdx.x == 0) L.part_up[blockIdx.x] = v;
   }
 }
 
@@ -150,10 +253,9 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
 // q_{k-1}) = tin + H_dw q / s_k - beta_k q_{k-1} / s_{k-1}, alpha partials.
 template <typename T, int R, int MODE>
 __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
-                                                  const T* __restrict__ tin, T* outp, LzA A, int nchunk) {
+                                                  const T* __restrict__ tin, T* outp, LzChain A, int nchunk) {
   __shared__ T lut[256];
   __shared__ double sh[16];
-  __shared__ int flag;
   LzScalars s{};
   const T* __restrict__ q = qa;
   T* __restrict__ out = outp;
@@ -173,7 +275,6 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   // consecutive panels of that XCD: the panel's neighbour segments stay in
   // the XCD's L2 without any inter-block synchronisation.  Blocks past the
   // last panel only contribute a zero partial.
-  const int G = gridDim.x;
   const int b = blockIdx.x;
   const int xcd = b % 8, ib = b / 8;
   const int jb = ib % nchunk;
@@ -244,7 +345,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
           if (MODE == 0) {
             __builtin_nontemporal_store(__builtin_nontemporal_load(&tin[i]) + acc[h], &out[i]);
           } else {
-            const T tv = __builtin_nontemporal_load(&tin[i]) + acc[h] * ts - bo * __builtin_nontemporal_load(&out[i]);
+            const T tv = (__builtin_nontemporal_load(&tin[i]) + acc[h]) * ts - bo * __builtin_nontemporal_load(&out[i]);
             __builtin_nontemporal_store(tv, &out[i]);
             part += (double)tv * (double)q[i];
           }
@@ -253,39 +354,63 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
     }
   }
   if (MODE == 1) {
-    // deterministic partial per block, last block reduces (alpha_k)
-    double v = part * inv_s;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if (lane == 0) sh[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < nwaves; ++w) t += sh[w];
-      A.part[b] = t;
-      __threadfence();
-      const unsigned old = atomicAdd(A.counter, 1u);
-      flag = (old + 1u == (unsigned)G) ? 1 : 0;
+    // block partial of alpha_k; summed in block order by k_lz_alpha
+    const double v = block_sum(part * inv_s, sh);
+    if (threadIdx.x == 0) A.part_dw[b] = v;
+  }
+}
+
+// ------------------------------------------------------ chain reductions
+// One block each, between the passes: the kernel boundary makes the block
+// partials visible to it (a fence inside the passes would write back and
+// invalidate the XCD's L2).  Sums run in block order (deterministic).
+static __device__ double ordered_sum_blocks(const double* __restrict__ part, int n, double* sh) {
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int a = threadIdx.x * per, e = min(a + per, n);
+  double x = 0.0;
+  for (int k = a; k < e; ++k) x += part[k];
+  return block_sum(x, sh);
+}
+
+// After the up pass: beta_{k+1} = |r|; the step's scalars advance, or the
+// chain stops after nmax steps or on breakdown.
+__global__ __launch_bounds__(256) void k_lz_beta(LzChain L, int nparts) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (s.stopped || !s.pad) return;
+  const bool more = (s.k + 1 < L.nmax);
+  const double s2 = more ? ordered_sum_blocks(L.part_up, nparts, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    const double b = sqrt(s2);
+    LzScalars n = s;
+    n.pad = 0;
+    if (more) L.beta[s.k + 1] = b;
+    const bool brk = !(b > L.brk_tol * (fabs(s.alpha_k) + s.beta_k));
+    if (!more || brk) {
+      n.stopped = 1;
+    } else {
+      n.s_old = s.s_cur;
+      n.s_cur = b;
+      n.beta_k = b;
+      n.k = s.k + 1;
     }
-    __syncthreads();
-    if (flag) {
-      __threadfence();
-      double a = 0.0;
-      for (int k = threadIdx.x; k < G; k += blockDim.x)
-        a += __hip_atomic_load(&A.part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      __syncthreads();
-      if (lane == 0) sh[wave] = a;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < nwaves; ++w) t += sh[w];
-        A.alpha[s.k] = t;
-        A.sc->alpha_k = t;
-        *A.counter = 0u;
-        __threadfence();
-      }
-    }
+    *L.sc = n;
+  }
+}
+
+// After the down pass: alpha_k; phase B of step k becomes pending.
+__global__ __launch_bounds__(256) void k_lz_alpha(LzChain L, int nparts) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (s.stopped) return;
+  const double a = ordered_sum_blocks(L.part_dw, nparts, sh);
+  if (threadIdx.x == 0) {
+    L.alpha[s.k] = a;
+    LzScalars n = s;
+    n.alpha_k = a;
+    n.nsteps = s.k + 1;
+    n.pad = 1;
+    *L.sc = n;
   }
 }
 
@@ -298,7 +423,8 @@ static size_t up_lds_bytes(int dimup, int C, size_t sz) {
 }
 
 bool two_pass_ok(const Sector& s, size_t elem) {
-  return s.sp[0].packed_ok && s.sp[1].packed_ok && up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes();
+  return s.sp[0].packed_ok && s.sp[1].packed_ok &&
+         up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes() - 256;
 }
 
 int padded_ld(int dimup) { return (dimup + 31) / 32 * 32; }
@@ -323,30 +449,46 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
 }
 
 template <typename T, int C>
-static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, const T* qa, const T* qb, T* tout,
-                        const LzScalars* sc, int grid, int threads, hipStream_t st) {
+static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                        const LzChain* L, int grid, int threads, hipStream_t st) {
   const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
   static bool attr = false;
   if (!attr) {
     EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                lds_limit_bytes()));
+                                lds_limit_bytes() - 256));
     attr = true;
   }
-  hipLaunchKernelGGL((k_pass_up<T, C>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout, sc);
+  hipLaunchKernelGGL((k_pass_up<T, C>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
 }
 
-template <typename T>
-void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, const T* qa, const T* qb, T* tout,
-                    const LzScalars* sc, const TwoPassCfg& cfg, hipStream_t st) {
+int pass_up_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
   const int npan = (V.dimdw + cfg.C - 1) / cfg.C;
-  const int grid = cfg.grid_up < npan ? cfg.grid_up : npan;
+  return cfg.grid_up < npan ? cfg.grid_up : npan;
+}
+
+template <typename T>
+void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout, const LzChain* L,
+                    const TwoPassCfg& cfg, hipStream_t st) {
+  const int grid = pass_up_grid(V, cfg);
   switch (cfg.C) {
-    case 1: launch_up_c<T, 1>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
-    case 2: launch_up_c<T, 2>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
-    case 4: launch_up_c<T, 4>(V, I, lut, qa, qb, tout, sc, grid, cfg.threads_up, st); break;
+    case 1: launch_up_c<T, 1>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
+    case 2: launch_up_c<T, 2>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
+    case 3: launch_up_c<T, 3>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
+    case 4: launch_up_c<T, 4>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
     default: throw Error("launch_pass_up: unsupported panel width");
   }
+}
+
+void launch_lz_beta(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(256), 0, st, L, pass_up_grid(V, cfg));
+  EDL_HIP(hipGetLastError());
+}
+
+void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_alpha, dim3(1), dim3(256), 0, st, L, pass_dw_grid(V, cfg));
+  EDL_HIP(hipGetLastError());
 }
 
 int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
@@ -355,7 +497,7 @@ int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
 }
 
 template <typename T, int MODE>
-static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA& A,
+static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
                         const TwoPassCfg& cfg, hipStream_t st) {
   EDL_CHECK(cfg.nchunk >= 1 && cfg.nchunk <= V.dimdw, "launch_pass_dw: bad column chunk count");
   const int ncol = (V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1;
@@ -384,19 +526,19 @@ static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
 }
 
 template <typename T>
-void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA* A,
+void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain* A,
                     const TwoPassCfg& cfg, hipStream_t st) {
   if (A) launch_dw_r<T, 1>(V, lut, qa, qb, tin, out, *A, cfg, st);
-  else launch_dw_r<T, 0>(V, lut, qa, qb, tin, out, LzA{}, cfg, st);
+  else launch_dw_r<T, 0>(V, lut, qa, qb, tin, out, LzChain{}, cfg, st);
 }
 
-template void launch_pass_up<double>(const TwoPassView&, const InterParams&, const double*, const double*,
-                                     const double*, double*, const LzScalars*, const TwoPassCfg&, hipStream_t);
-template void launch_pass_up<float>(const TwoPassView&, const InterParams&, const float*, const float*, const float*,
-                                    float*, const LzScalars*, const TwoPassCfg&, hipStream_t);
+template void launch_pass_up<double>(const TwoPassView&, const InterParams&, const double*, double*, double*,
+                                     double*, const LzChain*, const TwoPassCfg&, hipStream_t);
+template void launch_pass_up<float>(const TwoPassView&, const InterParams&, const float*, float*, float*, float*,
+                                    const LzChain*, const TwoPassCfg&, hipStream_t);
 template void launch_pass_dw<double>(const TwoPassView&, const double*, double*, double*, const double*, double*,
-                                     const LzA*, const TwoPassCfg&, hipStream_t);
+                                     const LzChain*, const TwoPassCfg&, hipStream_t);
 template void launch_pass_dw<float>(const TwoPassView&, const float*, float*, float*, const float*, float*,
-                                    const LzA*, const TwoPassCfg&, hipStream_t);
+                                    const LzChain*, const TwoPassCfg&, hipStream_t);
 
 }  // namespace edl

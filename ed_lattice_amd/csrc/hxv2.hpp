@@ -17,12 +17,26 @@ struct TwoPassView {
   const double* dg_dw;
 };
 
-// Lanczos phase-A fusion of the down pass (single chain).
-struct LzA {
+// One Lanczos chain driven by the two passes.  A step is four launches:
+//   up     phase B of the previous step where pending (r = t - alpha v over
+//          t, block partials of |r|^2, replay term x += c v), then
+//          T = [D + H_up] q, unscaled;
+//   beta   one block: beta = |r|, scalars advance (the chain stops after
+//          nmax steps or on breakdown);
+//   down   t = (T + H_dw q) / s_k - beta_k q_{k-1} / s_{k-1} over q_{k-1},
+//          block partials of alpha_k;
+//   alpha  one block: alpha_k, phase B of step k pending.
+// No fence inside the passes: the kernel boundaries publish the partials.
+struct LzChain {
   LzScalars* sc;
-  double* alpha;
-  double* part;       // [pass_dw_grid]
-  unsigned* counter;
+  double* alpha;          // [nmax]
+  double* beta;           // [nmax]; beta[0] = 0
+  double* part_up;        // [pass_up_grid]
+  double* part_dw;        // [pass_dw_grid]
+  double* x;              // fp64 replay accumulator (natural layout), or null
+  const double* ccoef;    // replay coefficients c_k, or null
+  int nmax;
+  double brk_tol;         // relative breakdown threshold on beta
 };
 
 struct TwoPassCfg {
@@ -34,9 +48,10 @@ struct TwoPassCfg {
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
 };
 
-// Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)); the size of
-// LzA::part.
+// Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
+// LzChain::part_dw; blocks of an up-pass launch, the size of part_up.
 int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg);
+int pass_up_grid(const TwoPassView& V, const TwoPassCfg& cfg);
 
 bool two_pass_ok(const Sector& s, size_t elem);
 // Column stride of the engine's own vectors: DimUp rounded up to a whole
@@ -46,14 +61,16 @@ int padded_ld(int dimup);
 // ld = DimUp for vectors in the natural layout i = iup + idw * DimUp.
 TwoPassView two_pass_view(const Sector& s, int ld);
 
-// Operator mode: sc = null, input qa.  Chain mode: input q_k = (k odd ? qb : qa).
+// Operator mode (L = null): tout = [D + H_up] qa.  Chain mode: see LzChain;
+// the stored vectors q_k alternate between qa and qb by parity.
 template <typename T>
-void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, const T* qa, const T* qb, T* tout,
-                    const LzScalars* sc, const TwoPassCfg& cfg, hipStream_t st);
-// Operator mode (A = null): out = tin + H_dw qa.  Chain mode: q_k and the
-// q_{k-1} buffer (overwritten with the phase-A vector) alternate by parity.
-template <typename T>
-void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzA* A,
+void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout, const LzChain* L,
                     const TwoPassCfg& cfg, hipStream_t st);
+// Operator mode (A = null): out = tin + H_dw qa.  Chain mode: see LzChain.
+template <typename T>
+void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain* A,
+                    const TwoPassCfg& cfg, hipStream_t st);
+void launch_lz_beta(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
+void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
 
 }  // namespace edl

@@ -252,7 +252,19 @@ def _assert_converged_poles_match(gpu, ora, tol, window=1e-6, conv_tol=1e-10):
             assert abs((p[sg] * w[sg]).sum() / wg - (po[so] * wo[so]).sum() / wo_) < tol * max(1.0, 1e-6 / wg)
 
 
-def test_tridiag_seed_vs_oracle(ed, orc):
+@pytest.fixture(params=["two_pass", "single"])
+def batched_kernels(request, monkeypatch):
+    """Kernels of the batched chains: the two-pass LDS kernels (default
+    where the sectors fit) or the single-pass gather kernels."""
+    if request.param == "single":
+        monkeypatch.setenv("EDL_BATCHED_KERNELS", "single")
+    else:
+        monkeypatch.delenv("EDL_BATCHED_KERNELS", raising=False)
+    print(f"batched kernels: {request.param}", flush=True)
+    return request.param
+
+
+def test_tridiag_seed_vs_oracle(ed, orc, batched_kernels):
     m = _use(ed, orc, lattice.ring(8), [4.0])
     isec = ed.sector_index(5, 4)
     dim, _, _ = ed.build_sector(isec)
@@ -275,7 +287,7 @@ def test_tridiag_seed_vs_oracle(ed, orc):
 
 @pytest.mark.parametrize("prec,tol,conv_tol,lead,nlead,window", [("fp64", 1e-8, 1e-10, 1e-9, 10, 1e-6),
                                                                  ("fp32", 2e-4, 1e-5, 1e-4, 6, 1e-4)])
-def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, window):
+def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, window, batched_kernels):
     """Config ring8: all diagonal + off-diagonal GF seeds of spin up in one
     batched launch, continued fractions vs the oracle's seeds + Lanczos.
 
@@ -354,7 +366,7 @@ def test_gf_batched_equals_sequential_rect3x4(ed, orc):
                                       _ritz(ao, bo, s @ s, r["e0"], isign), 1e-8)
 
 
-def test_chi_batched_vs_oracle(ed, orc):
+def test_chi_batched_vs_oracle(ed, orc, batched_kernels):
     m = _use(ed, orc, lattice.ring(8), [4.0])
     isec = ed.sector_index(4, 4)
     r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
