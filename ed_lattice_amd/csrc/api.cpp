@@ -132,6 +132,12 @@ const double* lut_dev<double>(const SpinTable& t) { return t.d_lut64; }
 template <>
 const float* lut_dev<float>(const SpinTable& t) { return t.d_lut32; }
 
+// EDL_VERBOSE=1: phase timings of the drivers on stderr.
+static bool verbose() {
+  const char* e = std::getenv("EDL_VERBOSE");
+  return e && *e && *e != '0';
+}
+
 // Tuned two-pass schedule per (sector shape, precision).
 static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg>& tp_cache() {
   static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg> m;
@@ -193,6 +199,9 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
         for (int r = 0; r < 3; ++r)
           launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
         const double ms = tm.stop();
+        if (verbose())
+          std::fprintf(stderr, "[edl] tune up  fp%d C %d threads %4d grid %5d: %.4f ms\n", 8 * (int)sizeof(T), C, th,
+                       w.grid_up, ms / 3);
         if (ms < best_up) {
           best_up = ms;
           best.C = C;
@@ -215,6 +224,9 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
         for (int r = 0; r < 3; ++r)
           launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
         const double ms = tm.stop();
+        if (verbose())
+          std::fprintf(stderr, "[edl] tune dw  fp%d R %d threads %4d chunks %3d: %.4f ms\n", 8 * (int)sizeof(T), R, th,
+                       nch, ms / 3);
         if (ms < best_dw) {
           best_dw = ms;
           best.R = R;
@@ -436,12 +448,6 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
 static bool batched_single_pass_forced() {
   const char* e = std::getenv("EDL_BATCHED_KERNELS");
   return e && std::strcmp(e, "single") == 0;
-}
-
-// EDL_VERBOSE=1: phase timings of the drivers on stderr.
-static bool verbose() {
-  const char* e = std::getenv("EDL_VERBOSE");
-  return e && *e && *e != '0';
 }
 
 // `steps` Lanczos steps of every chain: up pass (with the previous step's
@@ -1074,7 +1080,7 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
     }
   }
   for (int RS : {32, 16, 8, 4, 2, 1}) {
-    if ((size_t)RS * sizeof(T) > 128 || head + (size_t)RS * max_dw * sizeof(T) > budget) continue;
+    if ((size_t)RS * sizeof(T) > 128 || bt_dw_lds_bytes(max_dw, RS, sizeof(T)) > budget) continue;
     for (int th : {256, 512, 1024}) {
       BtShape w = best;
       w.RS = RS;

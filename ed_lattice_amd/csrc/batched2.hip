@@ -138,11 +138,13 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  int* cntS = reinterpret_cast<int*>(S + (size_t)dimdw * RS);  // [dimdw] hop counts
 #pragma unroll 4
   for (int i = threadIdx.x; i < dimdw * RS; i += blockDim.x) {
     const int j = i / RS, r = i % RS;
     S[i] = (r0 + r < dimup) ? q[(size_t)j * ld + r0 + r] : (T)0;
   }
+  for (int j = threadIdx.x; j < dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
   __syncthreads();
   const double inv_s = 1.0 / s.s_cur;
   const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
@@ -152,39 +154,44 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
   const bool rok = r0 + r < dimup;
   double part = 0.0;
   constexpr int KCH = 8;
-  // Two columns per thread; their T and q_{k-1} elements and 2*KCH hop
-  // words are requested before the LDS gathers that need them.
-  for (int j0 = jl; j0 < dimdw; j0 += 2 * jstep) {
-    int col[2] = {j0, j0 + jstep};
-    const bool okb = col[1] < dimdw;
-    if (!okb) col[1] = j0;
-    const int nh[2] = {V.cnt_dw[col[0]], okb ? V.cnt_dw[col[1]] : 0};
-    size_t ii[2];
-    T tin[2], qold[2];
+  // NC columns per thread; their T and q_{k-1} elements and NC*KCH hop
+  // words are requested before the LDS gathers that need them (the hop
+  // counts come from LDS).
+  constexpr int NC = 2;
+  for (int j0 = jl; j0 < dimdw; j0 += NC * jstep) {
+    int col[NC], nh[NC];
+    bool okc[NC];
+    size_t ii[NC];
+    T tin[NC], qold[NC], acc[NC];
+    int nmax = 0;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NC; ++h) {
+      col[h] = j0 + h * jstep;
+      okc[h] = col[h] < dimdw;
+      if (!okc[h]) col[h] = j0;
+      nh[h] = okc[h] ? cntS[col[h]] : 0;
+      nmax = nh[h] > nmax ? nh[h] : nmax;
       ii[h] = (size_t)col[h] * ld + r0 + r;
-      tin[h] = rok ? tb[ii[h]] : (T)0;
-      qold[h] = rok ? qo[ii[h]] : (T)0;
+      tin[h] = (rok && okc[h]) ? tb[ii[h]] : (T)0;
+      qold[h] = (rok && okc[h]) ? qo[ii[h]] : (T)0;
+      acc[h] = (T)0;
     }
-    T acc[2] = {(T)0, (T)0};
-    const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
     for (int k0 = 0; k0 < nmax; k0 += KCH) {
-      uint32_t w[2][KCH];
+      uint32_t w[NC][KCH];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < NC; ++h)
 #pragma unroll
         for (int jj = 0; jj < KCH; ++jj)
           w[h][jj] = (k0 + jj < nh[h]) ? V.pk_dw[(size_t)(k0 + jj) * dimdw + col[h]] : (255u << 24);
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < NC; ++h)
 #pragma unroll
         for (int jj = 0; jj < KCH; ++jj) acc[h] += lut[w[h][jj] >> 24] * S[(int)(w[h][jj] & 0xFFFFFFu) * RS + r];
     }
     if (rok) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (h == 1 && !okb) break;
+      for (int h = 0; h < NC; ++h) {
+        if (!okc[h]) continue;
         const T tv = (tin[h] + acc[h]) * ts - bo * qold[h];
         qo[ii[h]] = tv;
         part += (double)tv * (double)S[col[h] * RS + r];
@@ -198,6 +205,10 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
 // ---------------------------------------------------------------- host side
 static constexpr size_t kBtUpBudget = 64 * 1024;   // two or more up tiles per CU
 static constexpr size_t kBtDwBudget = 150 * 1024;  // one slab per CU at most
+
+size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem) {
+  return lds_head_bytes(elem) + (size_t)RS * dimdw * elem + sizeof(int) * (size_t)dimdw;
+}
 
 BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem) {
   BtShape b;
@@ -213,7 +224,7 @@ BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem) {
   if (C == 0) return b;
   int RS = 0;
   for (int rs : {32, 16, 8, 4, 2, 1})
-    if ((size_t)rs * elem <= 128 && head + (size_t)rs * max_dimdw * elem <= kBtDwBudget) {
+    if ((size_t)rs * elem <= 128 && bt_dw_lds_bytes(max_dimdw, rs, elem) <= kBtDwBudget) {
       RS = rs;
       break;
     }
@@ -222,7 +233,7 @@ BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem) {
   b.RS = RS;
   const size_t up_lds = head + (size_t)C * max_dimup * elem;
   b.threads_up = up_lds > 40 * 1024 ? 1024 : 256;
-  const size_t dw_lds = head + (size_t)RS * max_dimdw * elem;
+  const size_t dw_lds = bt_dw_lds_bytes(max_dimdw, RS, elem);
   b.threads_dw = dw_lds > 40 * 1024 ? 1024 : 256;
   return b;
 }
@@ -297,7 +308,7 @@ static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
 
 template <typename T, int RS>
 static void bt_dw_r(const BtSet& bs, hipStream_t st) {
-  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)RS * max_dim(bs, 1) * sizeof(T);
+  const size_t lds = bt_dw_lds_bytes(max_dim(bs, 1), RS, sizeof(T));
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_dw: slab exceeds LDS");
   EDL_CHECK(bs.shape.threads_dw % RS == 0, "bt_launch_dw: block size not a multiple of the slab height");
   const int64_t n8 = (bs.tiles_dw + 7) / 8;

@@ -47,6 +47,8 @@ struct BtShape {
 // RS = 0 when some sector does not fit (the caller then takes the
 // single-pass kernels).
 BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem);
+// LDS bytes of a down tile: coefficient table, RS-row slab, hop counts.
+size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem);
 
 struct BtSet {
   std::vector<BtChain> h;
