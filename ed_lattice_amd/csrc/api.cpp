@@ -154,16 +154,19 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   // timed on the engine's own (padded) layout
   const int ld = padded_ld(s.sp[0].dim);
   const TwoPassView V = two_pass_view(s, ld);
-  // EDL_TWO_PASS_CFG_F32 / _F64 = "C,threads_up,grid_up,R,threads_dw,nchunk"
+  // EDL_TWO_PASS_CFG_F32 / _F64 = "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol[,kch]]"
   // pins the schedule of that precision (reproducible profiles of one
   // configuration); no tuning then.
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R, &p.threads_dw,
-                              &p.nchunk);
-    EDL_CHECK(n == 6, "EDL_TWO_PASS_CFG_F32/_F64: expected C,threads_up,grid_up,R,threads_dw,nchunk");
-    EDL_CHECK(p.C >= 1 && p.C <= 4 && (p.R == 16 || p.R == 32 || p.R == 64),
-              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1 to 4 and R 16, 32 or 64");
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R, &p.threads_dw,
+                              &p.nchunk, &p.ncol, &p.kch);
+    EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected C,threads_up,grid_up,R,threads_dw,nchunk[,ncol[,kch]]");
+    EDL_CHECK((p.ncol == 1 && (p.kch == 8 || p.kch == 16 || p.kch == 32)) ||
+                  (p.ncol == 2 && (p.kch == 8 || p.kch == 16)),
+              "EDL_TWO_PASS_CFG_F32/_F64: (ncol, kch) must be (1, 8|16|32) or (2, 8|16)");
+    EDL_CHECK(p.C >= 1 && p.C <= 4 && (p.R == 32 || p.R == 64),
+              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1 to 4 and R 32 or 64");
     EDL_CHECK(p.threads_up % 64 == 0 && p.threads_up >= 64 && p.threads_up <= 1024 && p.threads_dw % 64 == 0 &&
                   p.threads_dw >= 64 && p.threads_dw <= 1024 && p.grid_up >= 1 && p.nchunk >= 1 &&
                   p.nchunk <= V.dimdw,
@@ -209,12 +212,16 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best.grid_up = w.grid_up;
         }
       }
-  for (int R : {16, 32, 64})
-    for (int th : {256, 512, 1024})
-      for (int nch : {8, 16, 32, 64, 128, 256}) {
+  for (int R : {32, 64})
+    for (int nk : {108, 116, 132, 208, 216})
+      for (int th : {256, 512, 1024})
+        for (int nch : {32, 64, 128, 256}) {
         if (nch > V.dimdw) continue;
+        const int nc = nk / 100, kc = nk % 100;
         TwoPassCfg w = best;
         w.R = R;
+        w.ncol = nc;
+        w.kch = kc;
         w.threads_dw = th;
         w.nchunk = nch;
         const size_t tbl = sizeof(uint32_t) * (size_t)((V.dimdw + nch - 1) / nch + 1) * (V.kdw + 1);
@@ -225,11 +232,13 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
         const double ms = tm.stop();
         if (verbose())
-          std::fprintf(stderr, "[edl] tune dw  fp%d R %d threads %4d chunks %3d: %.4f ms\n", 8 * (int)sizeof(T), R, th,
-                       nch, ms / 3);
+          std::fprintf(stderr, "[edl] tune dw  fp%d R %d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
+                       8 * (int)sizeof(T), R, nc, kc, th, nch, ms / 3);
         if (ms < best_dw) {
           best_dw = ms;
           best.R = R;
+          best.ncol = nc;
+          best.kch = kc;
           best.threads_dw = th;
           best.nchunk = nch;
         }
@@ -1409,8 +1418,9 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
   ms_pass[1] = two ? dw / reps : 0.0;
-  const int32_t out[8] = {two ? 1 : 0, ld, cf.C, cf.threads_up, cf.grid_up, cf.R, cf.threads_dw, cf.nchunk};
-  for (int k = 0; k < 8; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
+  const int32_t out[10] = {two ? 1 : 0, ld,         cf.C,      cf.threads_up, cf.grid_up,
+                           cf.R,        cf.threads_dw, cf.nchunk, cf.ncol,       cf.kch};
+  for (int k = 0; k < 10; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
 // ------------------------------------------------------------------ pass D
 // MODE 0: out = tin + H_dw q.   MODE 1 (Lanczos phase A): out (the buffer of
 // q_{k-1}) = tin + H_dw q / s_k - beta_k q_{k-1} / s_{k-1}, alpha partials.
-template <typename T, int R, int MODE>
+template <typename T, int R, int MODE, int NCOL, int KCH>
 __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
                                                   const T* __restrict__ tin, T* outp, LzChain A, int nchunk) {
   __shared__ T lut[256];
@@ -303,51 +303,62 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   for (int p = p0; p < pend; ++p) {
     const int iup = p * R + r;
     const bool ok = iup < dimup;
-    // Two columns per lane; the neighbour segments of a chunk of KCH hops of
-    // both columns are requested together (2*KCH loads in flight), the
-    // chunk count following the exact hop counts.
-    constexpr int KCH = 16;
+    // NCOL columns per lane.  Their T (and q_{k-1}, q) elements are requested
+    // first, then the neighbour segments of a chunk of KCH hops of every
+    // column together (NCOL*KCH loads in flight); the chunk count follows the
+    // exact hop counts.
     const int cstep = nwaves * SLOTS;
     const int iuc = ok ? iup : 0;
-    for (int c0 = cb + wave * SLOTS + slot; c0 < ce; c0 += 2 * cstep) {
-      int col[2] = {c0, c0 + cstep};
-      const bool okb = col[1] < ce;
-      if (!okb) col[1] = c0;
-      const int n0 = tcnt[col[0] - cb], n1 = okb ? tcnt[col[1] - cb] : 0;
-      const int nmax = n0 > n1 ? n0 : n1;
-      T acc[2] = {(T)0, (T)0};
+    for (int c0 = cb + wave * SLOTS + slot; c0 < ce; c0 += NCOL * cstep) {
+      int col[NCOL], nh[NCOL];
+      bool okc[NCOL];
+      T t0[NCOL], o0[NCOL], q0v[NCOL], acc[NCOL];
+      int nmax = 0;
+#pragma unroll
+      for (int h = 0; h < NCOL; ++h) {
+        col[h] = c0 + h * cstep;
+        okc[h] = col[h] < ce;
+        if (!okc[h]) col[h] = c0;
+        nh[h] = okc[h] ? tcnt[col[h] - cb] : 0;
+        nmax = nh[h] > nmax ? nh[h] : nmax;
+        const size_t i = (size_t)col[h] * ld + iup;
+        const bool live = ok && okc[h];
+        t0[h] = live ? __builtin_nontemporal_load(&tin[i]) : (T)0;
+        o0[h] = (MODE == 1 && live) ? __builtin_nontemporal_load(&out[i]) : (T)0;
+        q0v[h] = (MODE == 1 && live) ? q[i] : (T)0;
+        acc[h] = (T)0;
+      }
       for (int k0 = 0; k0 < nmax; k0 += KCH) {
-        uint32_t w[2][KCH];
+        uint32_t w[NCOL][KCH];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < NCOL; ++h) {
           const uint32_t* tw = tbl + (size_t)(col[h] - cb) * kdw;
-          const int nh = h ? n1 : n0;
 #pragma unroll
           for (int j = 0; j < KCH; ++j)
-            w[h][j] = (k0 + j < nh) ? tw[k0 + j] : ((uint32_t)col[h] | (255u << 24));
+            w[h][j] = (k0 + j < nh[h]) ? tw[k0 + j] : ((uint32_t)col[h] | (255u << 24));
         }
-        T x[2][KCH];
+        T x[NCOL][KCH];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < NCOL; ++h)
 #pragma unroll
           for (int j = 0; j < KCH; ++j)
-            x[h][j] = (k0 + j < (h ? n1 : n0)) ? q[(size_t)(w[h][j] & 0xFFFFFFu) * ld + iuc] : (T)0;
+            x[h][j] = (k0 + j < nh[h]) ? q[(size_t)(w[h][j] & 0xFFFFFFu) * ld + iuc] : (T)0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < NCOL; ++h)
 #pragma unroll
           for (int j = 0; j < KCH; ++j) acc[h] += lut[w[h][j] >> 24] * x[h][j];
       }
       if (ok) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          if (h == 1 && !okb) break;
+        for (int h = 0; h < NCOL; ++h) {
+          if (!okc[h]) continue;
           const size_t i = (size_t)col[h] * ld + iup;
           if (MODE == 0) {
-            __builtin_nontemporal_store(__builtin_nontemporal_load(&tin[i]) + acc[h], &out[i]);
+            __builtin_nontemporal_store(t0[h] + acc[h], &out[i]);
           } else {
-            const T tv = (__builtin_nontemporal_load(&tin[i]) + acc[h]) * ts - bo * __builtin_nontemporal_load(&out[i]);
+            const T tv = (t0[h] + acc[h]) * ts - bo * o0[h];
             __builtin_nontemporal_store(tv, &out[i]);
-            part += (double)tv * (double)q[i];
+            part += (double)tv * (double)q0v[h];
           }
         }
       }
@@ -496,6 +507,26 @@ int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
   return 8 * cfg.nchunk * ((npan + 7) / 8);
 }
 
+template <typename T, int MODE, int NCOL, int KCH>
+static void launch_dw_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
+                        const TwoPassCfg& cfg, size_t lds, int grid, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    for (auto f : {(const void*)k_pass_dw<T, 32, MODE, NCOL, KCH>, (const void*)k_pass_dw<T, 64, MODE, NCOL, KCH>})
+      EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    attr = true;
+  }
+  if (cfg.R == 32)
+    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE, NCOL, KCH>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
+                       tin, out, A, cfg.nchunk);
+  else if (cfg.R == 64)
+    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE, NCOL, KCH>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
+                       tin, out, A, cfg.nchunk);
+  else
+    throw Error("launch_pass_dw: unsupported panel height");
+  EDL_HIP(hipGetLastError());
+}
+
 template <typename T, int MODE>
 static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
                         const TwoPassCfg& cfg, hipStream_t st) {
@@ -504,25 +535,15 @@ static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
   const size_t lds = sizeof(uint32_t) * (size_t)ncol * (V.kdw + 1);
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 4096, "launch_pass_dw: column table exceeds LDS");
   const int grid = pass_dw_grid(V, cfg);
-  static bool attr = false;
-  if (!attr) {
-    for (auto f : {(const void*)k_pass_dw<T, 16, MODE>, (const void*)k_pass_dw<T, 32, MODE>,
-                   (const void*)k_pass_dw<T, 64, MODE>})
-      EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
-    attr = true;
+  const int code = cfg.ncol * 100 + cfg.kch;
+  switch (code) {
+    case 108: return launch_dw_n<T, MODE, 1, 8>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 116: return launch_dw_n<T, MODE, 1, 16>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 132: return launch_dw_n<T, MODE, 1, 32>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 208: return launch_dw_n<T, MODE, 2, 8>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 216: return launch_dw_n<T, MODE, 2, 16>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    default: throw Error("launch_pass_dw: unsupported columns per lane / hop chunk");
   }
-  if (cfg.R == 16)
-    hipLaunchKernelGGL((k_pass_dw<T, 16, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
-                       A, cfg.nchunk);
-  else if (cfg.R == 32)
-    hipLaunchKernelGGL((k_pass_dw<T, 32, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
-                       A, cfg.nchunk);
-  else if (cfg.R == 64)
-    hipLaunchKernelGGL((k_pass_dw<T, 64, MODE>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb, tin, out,
-                       A, cfg.nchunk);
-  else
-    throw Error("launch_pass_dw: unsupported panel height");
-  EDL_HIP(hipGetLastError());
 }
 
 template <typename T>

@@ -43,9 +43,11 @@ struct TwoPassCfg {
   int C = 2;            // columns per up-pass panel
   int threads_up = 1024;
   int grid_up = 256;
-  int R = 32;           // rows per down-pass panel
+  int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
+  int ncol = 1;         // columns per lane in the down pass (1 or 2)
+  int kch = 16;         // hop chunk of the down pass (8, 16, 32 with 1 column; 8, 16 with 2)
 };
 
 // Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
