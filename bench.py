@@ -62,7 +62,8 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
     out = {}
     if sched["two_pass"]:
         for name, ms, nbytes, k in (
-                ("pass_up", up_ms, 2 * dim * elem, f"k_pass_up<{t}, {sched['up_cols']}>"),
+                ("pass_up", up_ms, 2 * dim * elem,
+                 f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}>"),
                 ("pass_dw", dw_ms, 3 * dim * elem,
                  f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>")):
             out[name] = {"kernel": k, "ms": ms, "algorithmic_bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}

@@ -154,14 +154,18 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   // timed on the engine's own (padded) layout
   const int ld = padded_ld(s.sp[0].dim);
   const TwoPassView V = two_pass_view(s, ld);
-  // EDL_TWO_PASS_CFG_F32 / _F64 = "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol[,kch]]"
+  // EDL_TWO_PASS_CFG_F32 / _F64 =
+  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch]]"
   // pins the schedule of that precision (reproducible profiles of one
   // configuration); no tuning then.
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R, &p.threads_dw,
-                              &p.nchunk, &p.ncol, &p.kch);
-    EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected C,threads_up,grid_up,R,threads_dw,nchunk[,ncol[,kch]]");
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
+                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch);
+    EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
+                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch]]");
+    EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
+              "EDL_TWO_PASS_CFG_F32/_F64: (up_rpt, up_kch) must be (1, 8|16) or (2, 4|8)");
     EDL_CHECK((p.ncol == 1 && (p.kch == 8 || p.kch == 16 || p.kch == 32)) ||
                   (p.ncol == 2 && (p.kch == 8 || p.kch == 16)),
               "EDL_TWO_PASS_CFG_F32/_F64: (ncol, kch) must be (1, 8|16|32) or (2, 8|16)");
@@ -188,6 +192,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   double best_up = 1e300, best_dw = 1e300;
   const int lds = 160 * 1024;
   for (int C : {1, 2, 3, 4})
+    for (int rk : {208, 204, 108, 116})
     for (int th : {512, 1024})
       for (int gm : {1, 2}) {
         const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
@@ -195,6 +200,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
         if (gm == 2 && 2 * need > (size_t)lds) continue;
         TwoPassCfg w = best;
         w.C = C;
+        w.up_rpt = rk / 100;
+        w.up_kch = rk % 100;
         w.threads_up = th;
         w.grid_up = c.num_cu * gm;
         launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
@@ -203,11 +210,13 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
         const double ms = tm.stop();
         if (verbose())
-          std::fprintf(stderr, "[edl] tune up  fp%d C %d threads %4d grid %5d: %.4f ms\n", 8 * (int)sizeof(T), C, th,
-                       w.grid_up, ms / 3);
+          std::fprintf(stderr, "[edl] tune up  fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
+                       8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
         if (ms < best_up) {
           best_up = ms;
           best.C = C;
+          best.up_rpt = w.up_rpt;
+          best.up_kch = w.up_kch;
           best.threads_up = th;
           best.grid_up = w.grid_up;
         }
@@ -1418,9 +1427,9 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
   ms_pass[1] = two ? dw / reps : 0.0;
-  const int32_t out[10] = {two ? 1 : 0, ld,         cf.C,      cf.threads_up, cf.grid_up,
-                           cf.R,        cf.threads_dw, cf.nchunk, cf.ncol,       cf.kch};
-  for (int k = 0; k < 10; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
+  const int32_t out[12] = {two ? 1 : 0, ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+                           cf.threads_dw, cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch};
+  for (int k = 0; k < 12; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

@@ -69,12 +69,12 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
-template <typename T, int C>
+template <typename T, int C, int RPT = 2, int UKCH = 8>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
 
-template <typename T, int C>
+template <typename T, int C, int RPT = 2, int UKCH = 8>
 static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
                                                 typename Vec<T, C>::type* P, const T* __restrict__ q,
                                                 T* __restrict__ tout, int c0, T ts) {
@@ -90,11 +90,13 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
     P[r] = x;
   }
   __syncthreads();
-  up_rows<T, C>(V, I, lut, P, tout, c0, ts);
+  up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, c0, ts);
 }
 
-// The row loop of a panel already staged in P.
-template <typename T, int C>
+// The row loop of a panel already staged in P: RPT rows per thread, hop
+// words fetched UKCH at a time so that RPT*UKCH table loads are in flight
+// before the LDS gathers that depend on them.
+template <typename T, int C, int RPT, int UKCH>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts) {
@@ -109,17 +111,19 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
     sd[j] = (uint32_t)V.map_dw[c];
     dgd[j] = V.dg_dw[c];
   }
-  // Two rows per thread, hop words fetched KCH at a time, so that 2*KCH
-  // table loads are in flight before the LDS gathers that depend on them.
-  constexpr int KCH = 8;
-  const int stride = 2 * blockDim.x;
+  const int stride = RPT * blockDim.x;
   for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
-    int row[2] = {i0, i0 + (int)blockDim.x};
-    const bool okb = row[1] < dimup;
-    if (!okb) row[1] = i0;
-    T acc[2][C];
+    int row[RPT], nh[RPT];
+    bool okr[RPT];
+    int nmax = 0;
+    T acc[RPT][C];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < RPT; ++h) {
+      row[h] = i0 + h * (int)blockDim.x;
+      okr[h] = row[h] < dimup;
+      if (!okr[h]) row[h] = i0;
+      nh[h] = okr[h] ? V.cnt_up[row[h]] : 0;
+      nmax = nh[h] > nmax ? nh[h] : nmax;
       const uint32_t su = (uint32_t)V.map_up[row[h]];
       const double dgu = V.dg_up[row[h]];
       const VT self = P[row[h]];
@@ -127,19 +131,17 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       for (int j = 0; j < C; ++j)
         acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
     }
-    const int nh[2] = {V.cnt_up[row[0]], okb ? V.cnt_up[row[1]] : 0};
-    const int nmax = nh[0] > nh[1] ? nh[0] : nh[1];
-    for (int k0 = 0; k0 < nmax; k0 += KCH) {
-      uint32_t w[2][KCH];
+    for (int k0 = 0; k0 < nmax; k0 += UKCH) {
+      uint32_t w[RPT][UKCH];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < RPT; ++h)
 #pragma unroll
-        for (int j = 0; j < KCH; ++j)
+        for (int j = 0; j < UKCH; ++j)
           w[h][j] = (k0 + j < nh[h]) ? V.pk_up[(size_t)(k0 + j) * dimup + row[h]] : (255u << 24);
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < RPT; ++h)
 #pragma unroll
-        for (int j = 0; j < KCH; ++j) {
+        for (int j = 0; j < UKCH; ++j) {
           const T cf = lut[w[h][j] >> 24];
           const VT x = P[w[h][j] & 0xFFFFFFu];
 #pragma unroll
@@ -147,8 +149,8 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
         }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (h == 1 && !okb) break;
+    for (int h = 0; h < RPT; ++h) {
+      if (!okr[h]) continue;
 #pragma unroll
       for (int j = 0; j < C; ++j)
         if (c0 + j < dimdw) __builtin_nontemporal_store(acc[h][j] * ts, &tout[(size_t)(c0 + j) * ld + row[h]]);
@@ -210,7 +212,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C>
+template <typename T, int C, int RPT, int UKCH>
 __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
@@ -221,7 +223,8 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   const int npan = (V.dimdw + C - 1) / C;
   if (!chain) {
     for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
-    for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) up_panel<T, C>(V, I, lut, P, qa, tout, pan * C, (T)1);
+    for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
+      up_panel<T, C, RPT, UKCH>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
   }
   const LzScalars s = *L.sc;
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     __syncthreads();
     nrm += stage_panel<T, C>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
     __syncthreads();
-    if (!last) up_rows<T, C>(V, I, lut, P, tout, pan * C, (T)1);
+    if (!last) up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, pan * C, (T)1);
   }
   if (pend && !last) {
     const double v = block_sum(nrm, sh);
@@ -459,19 +462,32 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   return v;
 }
 
-template <typename T, int C>
-static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+template <typename T, int C, int RPT, int UKCH>
+static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
   const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
   static bool attr = false;
   if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C, RPT, UKCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 lds_limit_bytes() - 256));
     attr = true;
   }
-  hipLaunchKernelGGL((k_pass_up<T, C>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
+  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
                      L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int C>
+static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                        const LzChain* L, int grid, const TwoPassCfg& cfg, hipStream_t st) {
+  const int th = cfg.threads_up;
+  switch (cfg.up_rpt * 100 + cfg.up_kch) {
+    case 108: return launch_up_k<T, C, 1, 8>(V, I, lut, qa, qb, tout, L, grid, th, st);
+    case 116: return launch_up_k<T, C, 1, 16>(V, I, lut, qa, qb, tout, L, grid, th, st);
+    case 208: return launch_up_k<T, C, 2, 8>(V, I, lut, qa, qb, tout, L, grid, th, st);
+    case 204: return launch_up_k<T, C, 2, 4>(V, I, lut, qa, qb, tout, L, grid, th, st);
+    default: throw Error("launch_pass_up: unsupported rows per thread / hop chunk");
+  }
 }
 
 int pass_up_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
@@ -484,10 +500,10 @@ void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, T*
                     const TwoPassCfg& cfg, hipStream_t st) {
   const int grid = pass_up_grid(V, cfg);
   switch (cfg.C) {
-    case 1: launch_up_c<T, 1>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
-    case 2: launch_up_c<T, 2>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
-    case 3: launch_up_c<T, 3>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
-    case 4: launch_up_c<T, 4>(V, I, lut, qa, qb, tout, L, grid, cfg.threads_up, st); break;
+    case 1: launch_up_c<T, 1>(V, I, lut, qa, qb, tout, L, grid, cfg, st); break;
+    case 2: launch_up_c<T, 2>(V, I, lut, qa, qb, tout, L, grid, cfg, st); break;
+    case 3: launch_up_c<T, 3>(V, I, lut, qa, qb, tout, L, grid, cfg, st); break;
+    case 4: launch_up_c<T, 4>(V, I, lut, qa, qb, tout, L, grid, cfg, st); break;
     default: throw Error("launch_pass_up: unsupported panel width");
   }
 }

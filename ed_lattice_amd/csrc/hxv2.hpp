@@ -43,6 +43,8 @@ struct TwoPassCfg {
   int C = 2;            // columns per up-pass panel
   int threads_up = 1024;
   int grid_up = 256;
+  int up_rpt = 2;       // rows per thread in the up pass (1 or 2)
+  int up_kch = 8;       // hop chunk of the up pass ((1, 8|16) or (2, 4|8))
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
