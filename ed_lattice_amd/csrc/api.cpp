@@ -306,8 +306,11 @@ struct ChainAlloc {
 // d_start is in the natural layout; the stored vectors get column stride ld
 // (the padding rows are zero and no kernel writes them).
 template <typename T>
+// With d_ss set, the start vector's sum of squares goes to *d_ss on the
+// device (partials in `part`) and the scalars are left to
+// launch_init_scalars: no host round trip.
 static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, const double* d_start, double norm2,
-                       int nmax, hipStream_t st, int ld = 0) {
+                       int nmax, hipStream_t st, int ld = 0, double* d_ss = nullptr, double* part = nullptr) {
   if (ld == 0) ld = S.sp[0].dim;
   const int64_t D = (int64_t)ld * S.sp[1].dim;
   fill_sector_view(S, sizeof(T) == 4, cd);
@@ -324,17 +327,21 @@ static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, 
   launch_zero_bytes(alpha, sizeof(double) * nmax, st);
   launch_zero_bytes(beta, sizeof(double) * nmax, st);
   launch_zero_bytes(counter, 4, st);
-  const double ss = device_sumsq<T>(q0, D, st);
-  LzScalars s{};
-  s.k = 0;
-  s.stopped = (norm2 == 0.0 || ss == 0.0) ? 1 : 0;
-  s.nsteps = 0;
-  s.s_cur = ss > 0.0 ? std::sqrt(ss) : 1.0;
-  s.s_old = 1.0;
-  s.beta_k = 0.0;
-  s.alpha_k = 0.0;
-  s.norm2 = norm2;
-  EDL_HIP(hipMemcpyAsync(sc, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  if (d_ss) {
+    device_sumsq_to<T>(q0, D, part, d_ss, st);
+  } else {
+    const double ss = device_sumsq<T>(q0, D, st);
+    LzScalars s{};
+    s.k = 0;
+    s.stopped = (norm2 == 0.0 || ss == 0.0) ? 1 : 0;
+    s.nsteps = 0;
+    s.s_cur = ss > 0.0 ? std::sqrt(ss) : 1.0;
+    s.s_old = 1.0;
+    s.beta_k = 0.0;
+    s.alpha_k = 0.0;
+    s.norm2 = norm2;
+    EDL_HIP(hipMemcpyAsync(sc, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  }
   cd.q0 = q0;
   cd.q1 = q1;
   cd.x = nullptr;
@@ -1085,38 +1092,49 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
   double bu = 1e300, bd = 1e300;
   for (int C : {1, 2, 4}) {
     if (head + (size_t)C * max_up * sizeof(T) > budget) continue;
-    for (int th : {256, 512, 1024}) {
-      BtShape w = best;
-      w.C = C;
-      w.threads_up = th;
-      const double ms = time(w, true);
-      if (ms < bu) {
-        bu = ms;
-        best.C = C;
-        best.threads_up = th;
+    for (int rk : {208, 204, 108})
+      for (int th : {256, 512, 1024}) {
+        BtShape w = best;
+        w.C = C;
+        w.up_rpt = rk / 100;
+        w.up_kch = rk % 100;
+        w.threads_up = th;
+        const double ms = time(w, true);
+        if (ms < bu) {
+          bu = ms;
+          best.C = C;
+          best.up_rpt = w.up_rpt;
+          best.up_kch = w.up_kch;
+          best.threads_up = th;
+        }
       }
-    }
   }
   for (int RS : {32, 16, 8, 4, 2, 1}) {
     if ((size_t)RS * sizeof(T) > 128 || bt_dw_lds_bytes(max_dw, RS, sizeof(T)) > budget) continue;
-    for (int th : {256, 512, 1024}) {
-      BtShape w = best;
-      w.RS = RS;
-      w.threads_dw = th;
-      const double ms = time(w, false);
-      if (ms < bd) {
-        bd = ms;
-        best.RS = RS;
-        best.threads_dw = th;
+    for (int nc : {1, 2})
+      for (int th : {256, 512, 1024}) {
+        BtShape w = best;
+        w.RS = RS;
+        w.dw_nc = nc;
+        w.threads_dw = th;
+        const double ms = time(w, false);
+        if (ms < bd) {
+          bd = ms;
+          best.RS = RS;
+          best.dw_nc = nc;
+          best.threads_dw = th;
+        }
       }
-    }
   }
   for (size_t k = 0; k < chains.size(); ++k)
     EDL_HIP(hipMemcpyAsync(chains[k].sc, &snap[k], sizeof(LzScalars), hipMemcpyHostToDevice, st));
   EDL_HIP(hipStreamSynchronize(st));
   if (verbose())
-    std::fprintf(stderr, "[edl] batched two-pass shape for %zu chains: C %d x %d threads (%.3f ms), RS %d x %d (%.3f ms)\n",
-                 chains.size(), best.C, best.threads_up, bu, best.RS, best.threads_dw, bd);
+    std::fprintf(stderr,
+                 "[edl] batched two-pass shape for %zu chains: C %d rpt %d kch %d x %d threads (%.3f ms), "
+                 "RS %d nc %d x %d (%.3f ms)\n",
+                 chains.size(), best.C, best.up_rpt, best.up_kch, best.threads_up, bu, best.RS, best.dw_nc,
+                 best.threads_dw, bd);
   bt_cache()[key] = best;
   return best;
 }
@@ -1158,8 +1176,14 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     }
     ar.reserve(bytes);
   }
+  // Seeds are generated one after another into tmp (stream order keeps the
+  // reuse safe); their norms stay on the device until every chain is set up.
   DevBuf tmp(sizeof(double) * maxdim);
-  for (size_t k = 0; k < idx.size(); ++k) {
+  const size_t nc = idx.size();
+  DevBuf sums(sizeof(double) * 2 * nc), part(sizeof(double) * (size_t)partials_size(std::max<int64_t>(maxdim, 1) * 2));
+  double* d_n2 = sums.as<double>();
+  double* d_ss = d_n2 + nc;
+  for (size_t k = 0; k < nc; ++k) {
     const int s = idx[k];
     Sector& J = get_sector(c, targets[s]);
     if (host_seed) {
@@ -1168,12 +1192,12 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       const Sector& I = get_sector(c, c.gs_sector);
       launch_seed(seeds[s], c.d_gs, I.sp[0].dim, J, tmp.as<double>(), st);
     }
-    const double n2 = device_dot(c, tmp.as<double>(), tmp.as<double>(), J.dim, st);
+    device_dot_to(tmp.as<double>(), tmp.as<double>(), J.dim, part.as<double>(), d_n2 + k, st);
     const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
     ChainDev cd{};
     const int ld = bt ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
     allocs[k].ar = &ar;
-    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), n2, nmax, st, ld);
+    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), 0.0, nmax, st, ld, d_ss + k, part.as<double>());
     if (bt) {
       BtChain b{};
       b.V = two_pass_view(J, ld);
@@ -1191,13 +1215,22 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       b.beta = cd.beta;
       btchains.push_back(b);
     }
-    EDL_HIP(hipStreamSynchronize(st));  // tmp is reused by the next seed
-    norm2[s] = n2;
     chains.push_back(cd);
     nmaxs.push_back(nmax);
     which.push_back(s);
   }
   if (chains.empty()) return 0;
+  {
+    std::vector<LzScalars*> scp(nc);
+    for (size_t k = 0; k < nc; ++k) scp[k] = chains[k].sc;
+    DevBuf d_scp(sizeof(LzScalars*) * nc);
+    EDL_HIP(hipMemcpyAsync(d_scp.p, scp.data(), sizeof(LzScalars*) * nc, hipMemcpyHostToDevice, st));
+    launch_init_scalars(d_scp.as<LzScalars*>(), d_n2, d_ss, (int)nc, st);
+    std::vector<double> n2h(nc);
+    EDL_HIP(hipMemcpyAsync(n2h.data(), d_n2, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+    EDL_HIP(hipStreamSynchronize(st));
+    for (size_t k = 0; k < nc; ++k) norm2[which[k]] = n2h[k];
+  }
   const int steps = *std::max_element(nmaxs.begin(), nmaxs.end());
   ChainSet<T> cs;
   cs.h = chains;

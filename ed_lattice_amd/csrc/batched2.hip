@@ -22,7 +22,7 @@ static __device__ __forceinline__ int64_t xcd_tile(int64_t n8) {
   return (int64_t)(blockIdx.x % 8) * n8 + blockIdx.x / 8;
 }
 
-template <typename T, int C>
+template <typename T, int C, int RPT, int UKCH>
 __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
                                                 const int32_t* __restrict__ tchain, InterParams I, int64_t Tall,
                                                 int64_t n8) {
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chai
   __syncthreads();
   const double nrm = stage_panel<T, C>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
   __syncthreads();
-  up_rows<T, C>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
+  up_rows<T, C, RPT, UKCH>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
   if (!pend) return;
   const double v = block_sum(nrm, sh);
   if (threadIdx.x == 0) B.partU[pan] = v;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_bt_alpha(const BtChain* __restrict__ ch
   }
 }
 
-template <typename T, int RS>
+template <typename T, int RS, int NC>
 __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
                                                 const int32_t* __restrict__ tchain, int64_t Tall, int64_t n8) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -157,7 +157,6 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
   // NC columns per thread; their T and q_{k-1} elements and NC*KCH hop
   // words are requested before the LDS gathers that need them (the hop
   // counts come from LDS).
-  constexpr int NC = 2;
   for (int j0 = jl; j0 < dimdw; j0 += NC * jstep) {
     int col[NC], nh[NC];
     bool okc[NC];
@@ -251,9 +250,13 @@ template <typename T>
 static void bt_attributes() {
   static bool done = false;
   if (done) return;
-  for (auto f : {(const void*)k_bt_up<T, 1>, (const void*)k_bt_up<T, 2>, (const void*)k_bt_up<T, 4>,
-                 (const void*)k_bt_dw<T, 1>, (const void*)k_bt_dw<T, 2>, (const void*)k_bt_dw<T, 4>,
-                 (const void*)k_bt_dw<T, 8>, (const void*)k_bt_dw<T, 16>, (const void*)k_bt_dw<T, 32>})
+  for (auto f : {(const void*)k_bt_up<T, 1, 2, 8>, (const void*)k_bt_up<T, 2, 2, 8>, (const void*)k_bt_up<T, 4, 2, 8>,
+                 (const void*)k_bt_up<T, 1, 2, 4>, (const void*)k_bt_up<T, 2, 2, 4>, (const void*)k_bt_up<T, 4, 2, 4>,
+                 (const void*)k_bt_up<T, 1, 1, 8>, (const void*)k_bt_up<T, 2, 1, 8>, (const void*)k_bt_up<T, 4, 1, 8>,
+                 (const void*)k_bt_dw<T, 1, 1>, (const void*)k_bt_dw<T, 2, 1>, (const void*)k_bt_dw<T, 4, 1>,
+                 (const void*)k_bt_dw<T, 8, 1>, (const void*)k_bt_dw<T, 16, 1>, (const void*)k_bt_dw<T, 32, 1>,
+                 (const void*)k_bt_dw<T, 1, 2>, (const void*)k_bt_dw<T, 2, 2>, (const void*)k_bt_dw<T, 4, 2>,
+                 (const void*)k_bt_dw<T, 8, 2>, (const void*)k_bt_dw<T, 16, 2>, (const void*)k_bt_dw<T, 32, 2>})
     EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBtDwBudget));
   done = true;
 }
@@ -294,29 +297,39 @@ static int max_dim(const BtSet& bs, int which) {
   return m;
 }
 
-template <typename T, int C>
-static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
+template <typename T, int C, int RPT, int UKCH>
+static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st) {
   const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)C * max_dim(bs, 0) * sizeof(T);
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_up: panel exceeds LDS");
   const int64_t n8 = (bs.tiles_up + 7) / 8;
-  hipLaunchKernelGGL((k_bt_up<T, C>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st, bs.d,
+  hipLaunchKernelGGL((k_bt_up<T, C, RPT, UKCH>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st, bs.d,
                      bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
   EDL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bt_beta, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
   EDL_HIP(hipGetLastError());
 }
 
-template <typename T, int RS>
+template <typename T, int RS, int NC>
 static void bt_dw_r(const BtSet& bs, hipStream_t st) {
   const size_t lds = bt_dw_lds_bytes(max_dim(bs, 1), RS, sizeof(T));
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_dw: slab exceeds LDS");
   EDL_CHECK(bs.shape.threads_dw % RS == 0, "bt_launch_dw: block size not a multiple of the slab height");
   const int64_t n8 = (bs.tiles_dw + 7) / 8;
-  hipLaunchKernelGGL((k_bt_dw<T, RS>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
+  hipLaunchKernelGGL((k_bt_dw<T, RS, NC>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
                      bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
   EDL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_bt_alpha, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
   EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int C>
+static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  switch (bs.shape.up_rpt * 100 + bs.shape.up_kch) {
+    case 208: return bt_up_k<T, C, 2, 8>(bs, I, st);
+    case 204: return bt_up_k<T, C, 2, 4>(bs, I, st);
+    case 108: return bt_up_k<T, C, 1, 8>(bs, I, st);
+    default: throw Error("bt_launch_up: unsupported rows per thread / hop chunk");
+  }
 }
 
 template <typename T>
@@ -329,17 +342,27 @@ void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st) {
   }
 }
 
-template <typename T>
-void bt_launch_dw(const BtSet& bs, hipStream_t st) {
+template <typename T, int NC>
+static void bt_dw_n(const BtSet& bs, hipStream_t st) {
   switch (bs.shape.RS) {
-    case 1: return bt_dw_r<T, 1>(bs, st);
-    case 2: return bt_dw_r<T, 2>(bs, st);
-    case 4: return bt_dw_r<T, 4>(bs, st);
-    case 8: return bt_dw_r<T, 8>(bs, st);
-    case 16: return bt_dw_r<T, 16>(bs, st);
-    case 32: return bt_dw_r<T, 32>(bs, st);
+    case 1: return bt_dw_r<T, 1, NC>(bs, st);
+    case 2: return bt_dw_r<T, 2, NC>(bs, st);
+    case 4: return bt_dw_r<T, 4, NC>(bs, st);
+    case 8: return bt_dw_r<T, 8, NC>(bs, st);
+    case 16: return bt_dw_r<T, 16, NC>(bs, st);
+    case 32: return bt_dw_r<T, 32, NC>(bs, st);
     default: throw Error("bt_launch_dw: unsupported slab height");
   }
+}
+
+template <typename T>
+void bt_launch_dw(const BtSet& bs, hipStream_t st) {
+  if (bs.shape.dw_nc == 1)
+    bt_dw_n<T, 1>(bs, st);
+  else if (bs.shape.dw_nc == 2)
+    bt_dw_n<T, 2>(bs, st);
+  else
+    throw Error("bt_launch_dw: unsupported columns per thread");
 }
 
 template void bt_finalize<double>(BtSet&);

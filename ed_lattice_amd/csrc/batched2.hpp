@@ -40,6 +40,9 @@ struct BtShape {
   int C = 1;             // columns per up tile
   int RS = 1;            // rows per down slab
   int threads_up = 256;
+  int up_rpt = 2;        // rows per thread / hop chunk of the up row loop
+  int up_kch = 8;
+  int dw_nc = 2;         // columns per thread in the down tile
   int threads_dw = 256;
 };
 

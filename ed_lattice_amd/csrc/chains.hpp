@@ -105,6 +105,14 @@ double device_dot(Ctx& c, const double* a, const double* b, int64_t n, hipStream
 template <typename T>
 double device_dot_t(const T* a, const T* b, int64_t n, hipStream_t st);
 void launch_scale(double* x, int64_t n, double s, hipStream_t st);
+// Device-side reductions into device memory (no host round trip); `part`
+// holds partials_size(n) doubles.
+int partials_size(int64_t n);
+void device_dot_to(const double* a, const double* b, int64_t n, double* part, double* out, hipStream_t st);
+template <typename T>
+void device_sumsq_to(const T* a, int64_t n, double* part, double* out, hipStream_t st);
+// *d_sc[k] = initial Lanczos scalars from n2[k], ss[k] (k < n).
+void launch_init_scalars(LzScalars* const* d_sc, const double* n2, const double* ss, int n, hipStream_t st);
 void launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 
 }  // namespace edl

@@ -366,6 +366,58 @@ void launch_zero_bytes(void* p, size_t bytes, hipStream_t st) {
   EDL_HIP(hipGetLastError());
 }
 
+// One block: deterministic total of g block partials (contiguous slice per
+// thread, fixed tree) into *out.
+__global__ void k_sum_partials(const double* __restrict__ part, int g, double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int per = (g + blockDim.x - 1) / blockDim.x;
+  const int a = threadIdx.x * per, e = min(a + per, g);
+  double x = 0.0;
+  for (int k = a; k < e; ++k) x += part[k];
+  const double t = block_sum(x, sh);
+  if (threadIdx.x == 0) *out = t;
+}
+
+int partials_size(int64_t n) { return grid_for(n); }
+
+void device_dot_to(const double* a, const double* b, int64_t n, double* part, double* out, hipStream_t st) {
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, st, a, b, n, part);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, part, g, out);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void device_sumsq_to(const T* a, int64_t n, double* part, double* out, hipStream_t st) {
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_sumsq_partial<T>, dim3(g), dim3(kBlock), 0, st, a, n, part);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, part, g, out);
+  EDL_HIP(hipGetLastError());
+}
+template void device_sumsq_to<double>(const double*, int64_t, double*, double*, hipStream_t);
+template void device_sumsq_to<float>(const float*, int64_t, double*, double*, hipStream_t);
+
+// Initial scalars of chain k from its seed norm n2[k] and the sum of squares
+// ss[k] of its stored start vector (nothing runs when either is zero).
+__global__ void k_init_scalars(LzScalars* const* __restrict__ sc, const double* __restrict__ n2,
+                               const double* __restrict__ ss, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  LzScalars s{};
+  s.stopped = (n2[k] == 0.0 || ss[k] == 0.0) ? 1 : 0;
+  s.s_cur = ss[k] > 0.0 ? sqrt(ss[k]) : 1.0;
+  s.s_old = 1.0;
+  s.norm2 = n2[k];
+  *sc[k] = s;
+}
+
+void launch_init_scalars(LzScalars* const* d_sc, const double* n2, const double* ss, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_scalars, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d_sc, n2, ss, n);
+  EDL_HIP(hipGetLastError());
+}
+
 static double host_sum_partials(const double* part, int g, hipStream_t st) {
   std::vector<double> h(g);
   EDL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, st));
