@@ -1068,10 +1068,23 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
   if (it != bt_cache().end()) return it->second;
   BtShape best = bt_shape(max_up, max_dw, sizeof(T));
   const size_t head = 256 * sizeof(T) + 16, budget = 150 * 1024;
+  // The tile kernels are timed alone, in the state of a middle step: phase B
+  // pending (pad = 1) with alpha_k = 0, so that each up launch reads t and
+  // q_k and writes r = t and T as in a real step yet leaves its input as it
+  // found it.  The chain scalars are restored afterwards; the vectors the
+  // timing launches wrote are rewritten by the real first step before they
+  // are read (q_{-1} enters step 0 with weight beta_0 = 0 and stays finite).
   std::vector<LzScalars> snap(chains.size());
   for (size_t k = 0; k < chains.size(); ++k)
     EDL_HIP(hipMemcpyAsync(&snap[k], chains[k].sc, sizeof(LzScalars), hipMemcpyDeviceToHost, st));
   EDL_HIP(hipStreamSynchronize(st));
+  std::vector<LzScalars> mid(snap);
+  for (LzScalars& m : mid) {
+    m.pad = 1;
+    m.alpha_k = 0.0;
+  }
+  for (size_t k = 0; k < chains.size(); ++k)
+    EDL_HIP(hipMemcpyAsync(chains[k].sc, &mid[k], sizeof(LzScalars), hipMemcpyHostToDevice, st));
   auto time = [&](const BtShape& shp, bool up) {
     BtSet bs;
     bs.shape = shp;
@@ -1081,9 +1094,9 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
       if (r == 1) EDL_HIP(hipStreamSynchronize(st));
       EventTimer tm(st);
       if (up)
-        bt_launch_up<T>(bs, c.model.inter, st);
+        bt_launch_up<T>(bs, c.model.inter, st, false);
       else
-        bt_launch_dw<T>(bs, st);
+        bt_launch_dw<T>(bs, st, false);
       const double ms = tm.stop();
       if (r == 2) return ms;
     }

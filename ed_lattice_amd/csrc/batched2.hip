@@ -298,19 +298,20 @@ static int max_dim(const BtSet& bs, int which) {
 }
 
 template <typename T, int C, int RPT, int UKCH>
-static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st) {
+static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st, bool scalars) {
   const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)C * max_dim(bs, 0) * sizeof(T);
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_up: panel exceeds LDS");
   const int64_t n8 = (bs.tiles_up + 7) / 8;
   hipLaunchKernelGGL((k_bt_up<T, C, RPT, UKCH>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st, bs.d,
                      bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
   EDL_HIP(hipGetLastError());
+  if (!scalars) return;
   hipLaunchKernelGGL(k_bt_beta, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
   EDL_HIP(hipGetLastError());
 }
 
 template <typename T, int RS, int NC>
-static void bt_dw_r(const BtSet& bs, hipStream_t st) {
+static void bt_dw_r(const BtSet& bs, hipStream_t st, bool scalars) {
   const size_t lds = bt_dw_lds_bytes(max_dim(bs, 1), RS, sizeof(T));
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_dw: slab exceeds LDS");
   EDL_CHECK(bs.shape.threads_dw % RS == 0, "bt_launch_dw: block size not a multiple of the slab height");
@@ -318,58 +319,59 @@ static void bt_dw_r(const BtSet& bs, hipStream_t st) {
   hipLaunchKernelGGL((k_bt_dw<T, RS, NC>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
                      bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
   EDL_HIP(hipGetLastError());
+  if (!scalars) return;
   hipLaunchKernelGGL(k_bt_alpha, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
   EDL_HIP(hipGetLastError());
 }
 
 template <typename T, int C>
-static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
+static void bt_up_c(const BtSet& bs, const InterParams& I, hipStream_t st, bool scalars) {
   switch (bs.shape.up_rpt * 100 + bs.shape.up_kch) {
-    case 208: return bt_up_k<T, C, 2, 8>(bs, I, st);
-    case 204: return bt_up_k<T, C, 2, 4>(bs, I, st);
-    case 108: return bt_up_k<T, C, 1, 8>(bs, I, st);
+    case 208: return bt_up_k<T, C, 2, 8>(bs, I, st, scalars);
+    case 204: return bt_up_k<T, C, 2, 4>(bs, I, st, scalars);
+    case 108: return bt_up_k<T, C, 1, 8>(bs, I, st, scalars);
     default: throw Error("bt_launch_up: unsupported rows per thread / hop chunk");
   }
 }
 
 template <typename T>
-void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st) {
+void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st, bool scalars) {
   switch (bs.shape.C) {
-    case 1: return bt_up_c<T, 1>(bs, I, st);
-    case 2: return bt_up_c<T, 2>(bs, I, st);
-    case 4: return bt_up_c<T, 4>(bs, I, st);
+    case 1: return bt_up_c<T, 1>(bs, I, st, scalars);
+    case 2: return bt_up_c<T, 2>(bs, I, st, scalars);
+    case 4: return bt_up_c<T, 4>(bs, I, st, scalars);
     default: throw Error("bt_launch_up: unsupported panel width");
   }
 }
 
 template <typename T, int NC>
-static void bt_dw_n(const BtSet& bs, hipStream_t st) {
+static void bt_dw_n(const BtSet& bs, hipStream_t st, bool scalars) {
   switch (bs.shape.RS) {
-    case 1: return bt_dw_r<T, 1, NC>(bs, st);
-    case 2: return bt_dw_r<T, 2, NC>(bs, st);
-    case 4: return bt_dw_r<T, 4, NC>(bs, st);
-    case 8: return bt_dw_r<T, 8, NC>(bs, st);
-    case 16: return bt_dw_r<T, 16, NC>(bs, st);
-    case 32: return bt_dw_r<T, 32, NC>(bs, st);
+    case 1: return bt_dw_r<T, 1, NC>(bs, st, scalars);
+    case 2: return bt_dw_r<T, 2, NC>(bs, st, scalars);
+    case 4: return bt_dw_r<T, 4, NC>(bs, st, scalars);
+    case 8: return bt_dw_r<T, 8, NC>(bs, st, scalars);
+    case 16: return bt_dw_r<T, 16, NC>(bs, st, scalars);
+    case 32: return bt_dw_r<T, 32, NC>(bs, st, scalars);
     default: throw Error("bt_launch_dw: unsupported slab height");
   }
 }
 
 template <typename T>
-void bt_launch_dw(const BtSet& bs, hipStream_t st) {
+void bt_launch_dw(const BtSet& bs, hipStream_t st, bool scalars) {
   if (bs.shape.dw_nc == 1)
-    bt_dw_n<T, 1>(bs, st);
+    bt_dw_n<T, 1>(bs, st, scalars);
   else if (bs.shape.dw_nc == 2)
-    bt_dw_n<T, 2>(bs, st);
+    bt_dw_n<T, 2>(bs, st, scalars);
   else
     throw Error("bt_launch_dw: unsupported columns per thread");
 }
 
 template void bt_finalize<double>(BtSet&);
 template void bt_finalize<float>(BtSet&);
-template void bt_launch_up<double>(const BtSet&, const InterParams&, hipStream_t);
-template void bt_launch_up<float>(const BtSet&, const InterParams&, hipStream_t);
-template void bt_launch_dw<double>(const BtSet&, hipStream_t);
-template void bt_launch_dw<float>(const BtSet&, hipStream_t);
+template void bt_launch_up<double>(const BtSet&, const InterParams&, hipStream_t, bool);
+template void bt_launch_up<float>(const BtSet&, const InterParams&, hipStream_t, bool);
+template void bt_launch_dw<double>(const BtSet&, hipStream_t, bool);
+template void bt_launch_dw<float>(const BtSet&, hipStream_t, bool);
 
 }  // namespace edl

@@ -67,9 +67,13 @@ struct BtSet {
 
 template <typename T>
 void bt_finalize(BtSet& bs);
+// One pass over every tile, then (scalars = true) the per-chain scalar
+// update that completes the step's half (beta after the up pass, alpha after
+// the down pass).  scalars = false launches the tile kernel alone and leaves
+// the chain state as it was (schedule timing).
 template <typename T>
-void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st);
+void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st, bool scalars = true);
 template <typename T>
-void bt_launch_dw(const BtSet& bs, hipStream_t st);
+void bt_launch_dw(const BtSet& bs, hipStream_t st, bool scalars = true);
 
 }  // namespace edl
