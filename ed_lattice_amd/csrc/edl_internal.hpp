@@ -63,6 +63,10 @@ struct SpinTable {
   double lut64[256] = {};
   float lut32[256] = {};
   int32_t* cnt = nullptr;      // [dim] number of hops of each string
+  // The packed words grouped by string, without padding: the hops of string
+  // a are csr[off[a] .. off[a+1]) in table order (with the packed table).
+  uint32_t* csr = nullptr;
+  int32_t* off = nullptr;      // [dim+1]
   double* d_lut64 = nullptr;   // device copies of the coefficient lut
   float* d_lut32 = nullptr;
 };

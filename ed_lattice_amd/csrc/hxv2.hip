@@ -288,16 +288,16 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   const int npan = (dimup + R - 1) / R;
   const int p0 = 8 * (ib / nchunk) + xcd, pend = p0 < npan ? p0 + 1 : p0;
   const int cb = (int)((int64_t)jb * dimdw / nchunk), ce = (int)((int64_t)(jb + 1) * dimdw / nchunk);
-  // stage the chunk's hop words and exact hop counts in LDS
+  // Stage the chunk's hop words (one contiguous run of the column-grouped
+  // table: no padding, so the table's share of L2 is the exact hop count)
+  // and their per-column offsets in LDS.
   extern __shared__ __align__(16) unsigned char dsm[];
-  uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);           // [ncol][kdw]
-  const int ncol = ce - cb, kdw = V.kdw;
-  int* tcnt = reinterpret_cast<int*>(tbl + (size_t)ncol * kdw);  // [ncol]
-  for (int i = threadIdx.x; i < ncol * kdw; i += blockDim.x) {
-    const int c = i / kdw, k = i % kdw;
-    tbl[i] = V.pk_dw[(size_t)k * dimdw + cb + c];
-  }
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) tcnt[c] = V.cnt_dw[cb + c];
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);
+  const int ncol = ce - cb;
+  const int w0 = V.off_dw[cb], nw = V.off_dw[ce] - w0;
+  int* toff = reinterpret_cast<int*>(tbl + nw);  // [ncol+1]
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = V.csr_dw[w0 + i];
+  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = V.off_dw[cb + c] - w0;
   __syncthreads();
   const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
   const T ts = (T)inv_s;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
         col[h] = c0 + h * cstep;
         okc[h] = col[h] < ce;
         if (!okc[h]) col[h] = c0;
-        nh[h] = okc[h] ? tcnt[col[h] - cb] : 0;
+        nh[h] = okc[h] ? toff[col[h] - cb + 1] - toff[col[h] - cb] : 0;
         nmax = nh[h] > nmax ? nh[h] : nmax;
         const size_t i = (size_t)col[h] * ld + iup;
         const bool live = ok && okc[h];
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
         uint32_t w[NCOL][KCH];
 #pragma unroll
         for (int h = 0; h < NCOL; ++h) {
-          const uint32_t* tw = tbl + (size_t)(col[h] - cb) * kdw;
+          const uint32_t* tw = tbl + toff[col[h] - cb];
 #pragma unroll
           for (int j = 0; j < KCH; ++j)
             w[h][j] = (k0 + j < nh[h]) ? tw[k0 + j] : ((uint32_t)col[h] | (255u << 24));
@@ -459,6 +459,8 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   v.cnt_dw = s.sp[1].cnt;
   v.dg_up = s.sp[0].diag;
   v.dg_dw = s.sp[1].diag;
+  v.csr_dw = s.sp[1].csr;
+  v.off_dw = s.sp[1].off;
   return v;
 }
 

@@ -15,6 +15,8 @@ struct TwoPassView {
   const int32_t* cnt_dw;
   const double* dg_up;
   const double* dg_dw;
+  const uint32_t* csr_dw;  // down-spin packed words grouped by column (no padding)
+  const int32_t* off_dw;   // [dimdw+1] their offsets
 };
 
 // One Lanczos chain driven by the two passes.  A step is four launches:

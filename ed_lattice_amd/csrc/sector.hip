@@ -78,6 +78,15 @@ __global__ void k_hop_count(const int32_t* map, int dim, const HopPair* pairs, i
   cnt[a] = c;
 }
 
+// Packed words of string a, k < cnt[a], to csr[off[a] + k].
+__global__ void k_ell_to_csr(const uint32_t* packed, const int32_t* cnt, const int32_t* off, int dim,
+                             uint32_t* csr) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= dim) return;
+  const int n = cnt[a], o = off[a];
+  for (int k = 0; k < n; ++k) csr[o + k] = packed[(size_t)k * dim + a];
+}
+
 __global__ void k_hop_fill(const int32_t* map, int dim, const HopPair* pairs, int npairs, int kmax,
                            int32_t* nbr, double* coef64, float* coef32, uint32_t* packed) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,6 +207,23 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
   hipLaunchKernelGGL(k_hop_fill, dim3(blocks_for(t.dim)), dim3(kBlock), 0, st, t.map, t.dim, d_pairs, np, kmax,
                      t.nbr, t.coef64, t.coef32, t.packed);
   EDL_HIP(hipGetLastError());
+  if (t.packed_ok) {
+    std::vector<int32_t> off((size_t)t.dim + 1, 0);
+    int64_t tot = 0;
+    for (int a = 0; a < t.dim; ++a) {
+      off[a] = (int32_t)tot;
+      tot += cnt[a];
+    }
+    EDL_CHECK(tot < ((int64_t)1 << 31), "build_sector: hop table exceeds 2^31 words");
+    off[t.dim] = (int32_t)tot;
+    EDL_HIP(hipMalloc(&t.off, sizeof(int32_t) * ((size_t)t.dim + 1)));
+    EDL_HIP(hipMalloc(&t.csr, sizeof(uint32_t) * (size_t)(tot > 0 ? tot : 1)));
+    EDL_HIP(hipMemcpyAsync(t.off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ell_to_csr, dim3(blocks_for(t.dim)), dim3(kBlock), 0, st, t.packed, d_cnt, t.off, t.dim,
+                       t.csr);
+    EDL_HIP(hipGetLastError());
+    EDL_HIP(hipStreamSynchronize(st));  // off (host) must outlive the copy
+  }
 
   // one-spin diagonal (HxV_local.f90:16-18, 47-60, 63-91)
   DiagParams P{};
@@ -259,6 +285,8 @@ void free_sector(Sector& s) {
     if (t.diag) (void)hipFree(t.diag);
     if (t.packed) (void)hipFree(t.packed);
     if (t.cnt) (void)hipFree(t.cnt);
+    if (t.csr) (void)hipFree(t.csr);
+    if (t.off) (void)hipFree(t.off);
     if (t.d_lut64) (void)hipFree(t.d_lut64);
     if (t.d_lut32) (void)hipFree(t.d_lut32);
     t = SpinTable{};
