@@ -260,9 +260,24 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
 // otherwise the single-pass gather kernel.
 // v and out have column stride ld (DimUp: natural layout; padded_ld: the
 // engine's layout, two-pass sectors only).
+// Column stride of the engine's own vectors on sector s: padded to whole
+// cache lines wherever the down pass runs (two-pass and segmented sectors).
 template <typename T>
-static void apply_h(Ctx& c, const Sector& s, const T* v, T* out, int ld, hipStream_t st) {
-  if (two_pass_ok(s, sizeof(T))) {
+static int engine_ld(const Sector& s) {
+  return (two_pass_ok(s, sizeof(T)) || seg_path(s, sizeof(T))) ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+}
+
+template <typename T>
+static void apply_h(Ctx& c, Sector& s, const T* v, T* out, int ld, hipStream_t st) {
+  if (seg_path(s, sizeof(T))) {
+    // columns larger than LDS: segmented up pass, then the down pass
+    seg_bounds(s, sizeof(T));
+    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
+    const TwoPassView V = two_pass_view(s, ld);
+    T* vin = const_cast<T*>(v);  // operator mode only reads it
+    launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, st);
+    launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, out, nullptr, seg_dw_cfg(s.sp[1].dim), st);
+  } else if (two_pass_ok(s, sizeof(T))) {
     const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
     const TwoPassView V = two_pass_view(s, ld);
@@ -382,6 +397,26 @@ static Schedule autotune(Ctx& c, const std::vector<ChainDev>& chains, hipStream_
   best.block_threads = 256;
   best.grid = c.num_cu * 8;
   double best_ms = 1e300;
+  {
+    // The default schedule first: where one launch already takes longer
+    // than kTuneCapMs (sectors of ~1e9 states and more, memory bound), the
+    // 80 timing launches of the search would cost minutes for a few percent.
+    constexpr double kTuneCapMs = 25.0;
+    ChainSet<T> cs;
+    cs.h = chains;
+    chainset_finalize(c, cs, best);
+    EventTimer tm(st);
+    lz_phase_a(c, cs, st);
+    const double ms = tm.stop();
+    if (ms > kTuneCapMs) {
+      best = cs.sch;
+      if (verbose())
+        std::fprintf(stderr, "[edl] single-pass schedule: default (%d threads, grid %d); one launch %.1f ms\n",
+                     best.block_threads, best.grid, ms);
+      tune_cache()[key] = best;
+      return best;
+    }
+  }
   for (int th : threads)
     for (int gm : gmul) {
         Schedule w;
@@ -417,6 +452,7 @@ struct TwoPassStep {
   const T* lut_dw;
   T* tmp;
   LzChain L;
+  const Sector* seg = nullptr;  // set: segmented up pass, phase B as its own launch
 };
 
 template <typename T>
@@ -425,7 +461,16 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  if (tp) {
+  if (tp && tp->seg) {
+    // phase B of the previous step, beta, segmented up, down (+ phase A), alpha
+    T* q0 = static_cast<T*>(cs.h[0].q0);
+    T* q1 = static_cast<T*>(cs.h[0].q1);
+    launch_phase_b<T>(tp->V, q0, q1, tp->L, st);
+    launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
+    launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, st);
+    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
+    launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
+  } else if (tp) {
     // up (+ phase B of the previous step), beta, down (+ phase A), alpha
     T* q0 = static_cast<T*>(cs.h[0].q0);
     T* q1 = static_cast<T*>(cs.h[0].q1);
@@ -534,19 +579,25 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   const int nmax = (int)std::min<int64_t>(niter, S.dim);
   GsRun r;
   Schedule sch;
-  // Sectors whose columns fit in LDS: the two-pass step (LzChain).
-  const bool two = two_pass_ok(S, sizeof(T));
+  // Sectors whose columns fit in LDS: the two-pass step (LzChain); larger
+  // columns: the same step with the segmented up pass.
+  const bool seg = seg_path(S, sizeof(T));
+  const bool two = seg || two_pass_ok(S, sizeof(T));
   TwoPassStep<T> tps{};
   DevBuf tp_up, tp_dw;
   // Two-pass chains keep their vectors with the padded column stride.
   const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
-    tps.cfg = two_pass_cfg<T>(c, S);
+    if (seg) {
+      seg_bounds(S, sizeof(T));
+      tps.seg = &S;
+    }
+    tps.cfg = seg ? seg_dw_cfg(S.sp[1].dim) : two_pass_cfg<T>(c, S);
     tps.V = two_pass_view(S, ld);
     tps.lut_up = lut_dev<T>(S.sp[0]);
     tps.lut_dw = lut_dev<T>(S.sp[1]);
     tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * S.sp[1].dim));
-    tp_up = DevBuf(sizeof(double) * pass_up_grid(tps.V, tps.cfg));
+    tp_up = DevBuf(sizeof(double) * (seg ? phase_b_grid(tps.V) : pass_up_grid(tps.V, tps.cfg)));
     tp_dw = DevBuf(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
     tps.L.part_up = tp_up.as<double>();
     tps.L.part_dw = tp_dw.as<double>();
@@ -590,8 +641,8 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
     bind(cd, m, d_x, dc.as<double>());
-    // two-pass: one more launch, whose up pass adds the replay term of the
-    // last step (its phase B is otherwise not needed)
+    // two-pass: one more launch, whose phase B (in the up pass, or its own
+    // launch on segmented sectors) adds the replay term of the last step
     r.hxv += m;
     run_steps(c, cs, two ? m + 1 : m, 0, 0.0, st, two ? &tps : nullptr);
     const double n2 = device_dot(c, d_x, d_x, S.dim, st);
@@ -850,7 +901,7 @@ static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_
   std::vector<double> tmp(D);
   for (int64_t i = 0; i < D; ++i) tmp[i] = v[i * stride];
   // through the engine's own layout (padded columns on two-pass sectors)
-  const int ld = two_pass_ok(s, sizeof(T)) ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const int ld = engine_ld<T>(s);
   const int64_t Dp = (int64_t)ld * s.sp[1].dim;
   DevBuf d64(sizeof(double) * D), dv(sizeof(T) * Dp), dh(sizeof(T) * Dp);
   EDL_HIP(hipMemcpyAsync(d64.p, tmp.data(), sizeof(double) * D, hipMemcpyHostToDevice, st));
@@ -1402,7 +1453,7 @@ static double bench_impl(Ctx& c, Sector& s, int reps) {
   hipStream_t st = c.stream;
   const int64_t D = s.dim;
   // the engine's own layout (padded columns on two-pass sectors)
-  const int ld = two_pass_ok(s, sizeof(T)) ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const int ld = engine_ld<T>(s);
   const int64_t Dp = (int64_t)ld * s.sp[1].dim;
   DevBuf d64(sizeof(double) * D), v(sizeof(T) * Dp), h(sizeof(T) * Dp);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
@@ -1431,8 +1482,9 @@ extern "C" int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, doub
 template <typename T>
 static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int32_t* cfg) {
   hipStream_t st = c.stream;
-  const bool two = two_pass_ok(s, sizeof(T));
-  const int ld = two ? padded_ld(s.sp[0].dim) : s.sp[0].dim;
+  const bool seg = seg_path(s, sizeof(T));
+  const bool two = !seg && two_pass_ok(s, sizeof(T));
+  const int ld = engine_ld<T>(s);
   const int64_t D = s.dim, Dp = (int64_t)ld * s.sp[1].dim;
   DevBuf d64(sizeof(double) * D), v(sizeof(T) * Dp), h(sizeof(T) * Dp);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
@@ -1442,7 +1494,19 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   std::vector<hipEvent_t> ev(3 * (size_t)reps);
   for (auto& e : ev) EDL_HIP(hipEventCreate(&e));
   TwoPassCfg cf{};
-  if (two) {
+  if (seg) {
+    cf = seg_dw_cfg(s.sp[1].dim);
+    cf.C = 0;
+    const TwoPassView V = two_pass_view(s, ld);
+    T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)Dp));
+    for (int r = 0; r < reps; ++r) {
+      EDL_HIP(hipEventRecord(ev[3 * r], st));
+      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, st);
+      EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
+      launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
+      EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
+    }
+  } else if (two) {
     cf = two_pass_cfg<T>(c, s);
     const TwoPassView V = two_pass_view(s, ld);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)Dp));
@@ -1472,10 +1536,10 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
-  ms_pass[1] = two ? dw / reps : 0.0;
-  const int32_t out[12] = {two ? 1 : 0, ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
-                           cf.threads_dw, cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch};
-  for (int k = 0; k < 12; ++k) cfg[k] = two ? out[k] : (k < 2 ? out[k] : 0);
+  ms_pass[1] = (two || seg) ? dw / reps : 0.0;
+  const int32_t out[12] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+                           cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch};
+  for (int k = 0; k < 12; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

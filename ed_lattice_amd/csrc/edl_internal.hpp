@@ -75,6 +75,12 @@ struct Sector {
   int isector = 0, Nup = 0, Ndw = 0;
   int64_t dim = 0;
   SpinTable sp[2];
+  // Row segments of the segmented up pass (columns larger than LDS): bounds
+  // seg_h[0..n] of up index ranges that share their top sites' occupation,
+  // each at most seg_cap rows; device copy d_seg.  Built on first use.
+  std::vector<int> seg_h;
+  int* d_seg = nullptr;
+  int seg_cap = 0;
 };
 
 struct Model {

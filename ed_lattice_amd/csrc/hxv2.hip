@@ -296,8 +296,9 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   const int ncol = ce - cb;
   const int w0 = V.off_dw[cb], nw = V.off_dw[ce] - w0;
   int* toff = reinterpret_cast<int*>(tbl + nw);  // [ncol+1]
-  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = V.csr_dw[w0 + i];
-  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = V.off_dw[cb + c] - w0;
+  // (non-temporal: the table should not displace the panel's lines in L2)
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = __builtin_nontemporal_load(&V.csr_dw[w0 + i]);
+  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = __builtin_nontemporal_load(&V.off_dw[cb + c]) - w0;
   __syncthreads();
   const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
   const T ts = (T)inv_s;
@@ -570,6 +571,218 @@ void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* t
   if (A) launch_dw_r<T, 1>(V, lut, qa, qb, tin, out, *A, cfg, st);
   else launch_dw_r<T, 0>(V, lut, qa, qb, tin, out, LzChain{}, cfg, st);
 }
+
+// ------------------------------------------------------- segmented up pass
+static __device__ __forceinline__ int64_t seg_xcd_tile(int64_t n8) {
+  return (int64_t)(blockIdx.x % 8) * n8 + blockIdx.x / 8;
+}
+
+template <typename T, int UKCH>
+__global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
+                                                 T* qb, T* __restrict__ tout, LzChain L, int chain,
+                                                 const int* __restrict__ seg, int nseg, int64_t ntiles, int64_t n8) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut = reinterpret_cast<T*>(smem);
+  T* P = reinterpret_cast<T*>(smem + lds_head_bytes(sizeof(T)));
+  // tiles column-major (the segments of a column are consecutive) and each
+  // XCD takes a contiguous range of them
+  const int64_t t = seg_xcd_tile(n8);
+  if (t >= ntiles) return;
+  const T* __restrict__ q = qa;
+  if (chain) {
+    const LzScalars s = *L.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+  }
+  const int col = (int)(t / nseg), sg = (int)(t % nseg);
+  const int lo = seg[sg], hi = seg[sg + 1];
+  const int dimup = V.dimup;
+  const size_t base = (size_t)col * V.ld;
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+  for (int r = lo + threadIdx.x; r < hi; r += blockDim.x) P[r - lo] = q[base + r];
+  __syncthreads();
+  const uint32_t sd = (uint32_t)V.map_dw[col];
+  const double dgd = V.dg_dw[col];
+  for (int row = lo + threadIdx.x; row < hi; row += blockDim.x) {
+    const int nh = V.cnt_up[row];
+    T acc = (T)(V.dg_up[row] + dgd + inter2(I, (uint32_t)V.map_up[row], sd)) * P[row - lo];
+    for (int k0 = 0; k0 < nh; k0 += UKCH) {
+      uint32_t w[UKCH];
+#pragma unroll
+      for (int j = 0; j < UKCH; ++j) w[j] = (k0 + j < nh) ? V.pk_up[(size_t)(k0 + j) * dimup + row] : (255u << 24);
+      T x[UKCH];
+#pragma unroll
+      for (int j = 0; j < UKCH; ++j) {
+        const int nb = (int)(w[j] & 0xFFFFFFu);
+        const bool in = nb >= lo && nb < hi;
+        x[j] = (k0 + j < nh) ? (in ? P[in ? nb - lo : 0] : q[base + nb]) : (T)0;
+      }
+#pragma unroll
+      for (int j = 0; j < UKCH; ++j) acc += lut[w[j] >> 24] * x[j];
+    }
+    tout[base + row] = acc;
+  }
+}
+
+// Phase B of a chain over whole columns (block = columns b, b + grid, ...).
+template <typename T>
+__global__ __launch_bounds__(256) void k_phase_b(TwoPassView V, T* qa, T* qb, LzChain L) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (s.stopped || !s.pad) return;
+  const bool last = s.k + 1 >= L.nmax;
+  if (last && !L.x) return;
+  T* __restrict__ qn = ((s.k + 1) & 1) ? qb : qa;
+  const T* __restrict__ qk = (s.k & 1) ? qb : qa;
+  const T am = (T)(s.alpha_k / s.s_cur);
+  double* __restrict__ x = L.x;
+  const double cx = x ? L.ccoef[s.k] / s.s_cur : 0.0;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  double nrm = 0.0;
+  for (int col = blockIdx.x; col < dimdw; col += gridDim.x)
+    for (int row = threadIdx.x; row < dimup; row += blockDim.x) {
+      const size_t i = (size_t)col * ld + row;
+      const T b = qk[i];
+      if (!last) {
+        const T e = qn[i] - am * b;
+        qn[i] = e;
+        nrm += (double)e * (double)e;
+      }
+      if (x) x[(size_t)col * dimup + row] += cx * (double)b;
+    }
+  if (last) return;
+  const double v = block_sum(nrm, sh);
+  if (threadIdx.x == 0) L.part_up[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_lz_beta_n(LzChain L, int nparts) {
+  // same as k_lz_beta with an explicit partial count
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (s.stopped || !s.pad) return;
+  const bool more = (s.k + 1 < L.nmax);
+  const double s2 = more ? ordered_sum_blocks(L.part_up, nparts, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    const double b = sqrt(s2);
+    LzScalars n = s;
+    n.pad = 0;
+    if (more) L.beta[s.k + 1] = b;
+    const bool brk = !(b > L.brk_tol * (fabs(s.alpha_k) + s.beta_k));
+    if (!more || brk) {
+      n.stopped = 1;
+    } else {
+      n.s_old = s.s_cur;
+      n.s_cur = b;
+      n.beta_k = b;
+      n.k = s.k + 1;
+    }
+    *L.sc = n;
+  }
+}
+
+int seg_row_cap(size_t elem) {
+  const size_t head = lds_head_bytes(elem);
+  int cap = (int)(((size_t)lds_limit_bytes() - 256 - head) / elem);
+  if (const char* e = std::getenv("EDL_UP_SEGMENT_ROWS")) {
+    const int v = std::atoi(e);
+    EDL_CHECK(v >= 1, "EDL_UP_SEGMENT_ROWS must be a positive row count");
+    cap = std::min(cap, v);
+  }
+  return cap;
+}
+
+bool seg_path(const Sector& s, size_t elem) {
+  if (!(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1) return false;
+  return !two_pass_ok(s, elem) || seg_row_cap(elem) < s.sp[0].dim;
+}
+
+const std::vector<int>& seg_bounds(Sector& s, size_t elem) {
+  const int cap = seg_row_cap(elem);
+  if (s.d_seg && s.seg_cap == cap) return s.seg_h;
+  const SpinTable& t = s.sp[0];
+  std::vector<int32_t> map(t.dim);
+  EDL_HIP(hipMemcpy(map.data(), t.map, sizeof(int32_t) * t.dim, hipMemcpyDeviceToHost));
+  int bits = 1;
+  while (bits < 31 && (map.back() >> bits) != 0) ++bits;
+  // the coarsest split by top-site occupation whose runs all fit
+  std::vector<int> runs;
+  for (int sh = bits; sh >= 0; --sh) {
+    runs.assign(1, 0);
+    for (int i = 1; i < t.dim; ++i)
+      if ((map[i] >> sh) != (map[i - 1] >> sh)) runs.push_back(i);
+    runs.push_back(t.dim);
+    int longest = 0;
+    for (size_t k = 0; k + 1 < runs.size(); ++k) longest = std::max(longest, runs[k + 1] - runs[k]);
+    if (longest <= cap) break;
+  }
+  // adjacent runs merged while they fit (fewer segments, more local hops)
+  std::vector<int> b(1, 0);
+  for (size_t k = 1; k < runs.size(); ++k) {
+    if (runs[k] - b.back() > cap) b.push_back(runs[k - 1]);
+  }
+  b.push_back(t.dim);
+  if (s.d_seg) (void)hipFree(s.d_seg);
+  s.d_seg = nullptr;
+  EDL_HIP(hipMalloc(&s.d_seg, sizeof(int) * b.size()));
+  EDL_HIP(hipMemcpy(s.d_seg, b.data(), sizeof(int) * b.size(), hipMemcpyHostToDevice));
+  s.seg_h = b;
+  s.seg_cap = cap;
+  return s.seg_h;
+}
+
+TwoPassCfg seg_dw_cfg(int dimdw) {
+  TwoPassCfg p;
+  p.R = 64;
+  p.threads_dw = 256;
+  p.nchunk = std::max(1, std::min(256, dimdw));
+  p.ncol = 2;
+  p.kch = 8;
+  return p;
+}
+
+template <typename T>
+void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                   const LzChain* L, const Sector& s, hipStream_t st) {
+  EDL_CHECK(s.d_seg && s.seg_h.size() >= 2, "launch_up_seg: segment bounds not built");
+  const int nseg = (int)s.seg_h.size() - 1;
+  int rows = 0;
+  for (int k = 0; k < nseg; ++k) rows = std::max(rows, s.seg_h[k + 1] - s.seg_h[k]);
+  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)rows * sizeof(T);
+  EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_up_seg: segment exceeds LDS");
+  static bool attr = false;
+  if (!attr) {
+    EDL_HIP(hipFuncSetAttribute((const void*)k_up_seg<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                lds_limit_bytes() - 256));
+    attr = true;
+  }
+  const int64_t ntiles = (int64_t)V.dimdw * nseg;
+  const int64_t n8 = (ntiles + 7) / 8;
+  EDL_CHECK(8 * n8 < ((int64_t)1 << 31), "launch_up_seg: too many tiles");
+  hipLaunchKernelGGL((k_up_seg<T, 8>), dim3((unsigned)(8 * n8)), dim3(1024), lds, st, V, I, lut, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0, s.d_seg, nseg, ntiles, n8);
+  EDL_HIP(hipGetLastError());
+}
+
+int phase_b_grid(const TwoPassView& V) { return std::min(V.dimdw, 4096); }
+
+template <typename T>
+void launch_phase_b(const TwoPassView& V, T* qa, T* qb, const LzChain& L, hipStream_t st) {
+  hipLaunchKernelGGL(k_phase_b<T>, dim3(phase_b_grid(V)), dim3(256), 0, st, V, qa, qb, L);
+  EDL_HIP(hipGetLastError());
+}
+
+void launch_lz_beta_n(const LzChain& L, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_beta_n, dim3(1), dim3(256), 0, st, L, nparts);
+  EDL_HIP(hipGetLastError());
+}
+
+template void launch_up_seg<double>(const TwoPassView&, const InterParams&, const double*, double*, double*, double*,
+                                    const LzChain*, const Sector&, hipStream_t);
+template void launch_up_seg<float>(const TwoPassView&, const InterParams&, const float*, float*, float*, float*,
+                                   const LzChain*, const Sector&, hipStream_t);
+template void launch_phase_b<double>(const TwoPassView&, double*, double*, const LzChain&, hipStream_t);
+template void launch_phase_b<float>(const TwoPassView&, float*, float*, const LzChain&, hipStream_t);
 
 template void launch_pass_up<double>(const TwoPassView&, const InterParams&, const double*, double*, double*,
                                      double*, const LzChain*, const TwoPassCfg&, hipStream_t);

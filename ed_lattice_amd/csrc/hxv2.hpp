@@ -79,4 +79,39 @@ void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* t
 void launch_lz_beta(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
 void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
 
+// ---- segmented up pass: sectors whose up columns exceed LDS.
+// A column is cut into row segments that share the occupation of the top
+// up sites (states are sorted, so each is one contiguous index range); hops
+// that leave the top sites untouched stay inside a segment.  One block takes
+// one (column, segment) tile: the segment is staged in LDS, neighbours
+// inside it are gathered there and the others from global memory (L2: the
+// tiles of a column run next to each other on one XCD).
+//
+// Rows per segment at most: the LDS capacity for elements of `elem` bytes,
+// lowered by EDL_UP_SEGMENT_ROWS when set (tests force several segments on
+// small sectors with it).
+int seg_row_cap(size_t elem);
+// Whether the operator and the Lanczos chains of `s` use the segmented up
+// pass with the down pass: packed hop tables, and either the columns exceed
+// LDS or EDL_UP_SEGMENT_ROWS is set below DimUp.
+bool seg_path(const Sector& s, size_t elem);
+// Segment bounds of `s` for rows of `elem` bytes (built on first use).
+const std::vector<int>& seg_bounds(Sector& s, size_t elem);
+// Fixed down-pass schedule of segmented sectors (no tuning: one launch of
+// such a sector moves gigabytes).
+TwoPassCfg seg_dw_cfg(int dimdw);
+// Operator mode (L = null): tout = [D + H_up] qa.  Chain mode: T = [D +
+// H_up] q_k unscaled for the current k (phase B already done).
+template <typename T>
+void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                   const LzChain* L, const Sector& s, hipStream_t st);
+// Phase B of a chain as a streaming pass (r = t - alpha_k q_k / s_k over t,
+// block partials of |r|^2 into L.part_up, replay term x += c_k q_k / s_k),
+// for the segmented path; `grid` blocks = the partial count for
+// launch_lz_beta_n.
+int phase_b_grid(const TwoPassView& V);
+template <typename T>
+void launch_phase_b(const TwoPassView& V, T* qa, T* qb, const LzChain& L, hipStream_t st);
+void launch_lz_beta_n(const LzChain& L, int nparts, hipStream_t st);
+
 }  // namespace edl

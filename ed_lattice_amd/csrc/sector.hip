@@ -277,6 +277,10 @@ void build_sector_tables(Ctx& c, Sector& s) {
 }
 
 void free_sector(Sector& s) {
+  if (s.d_seg) (void)hipFree(s.d_seg);
+  s.d_seg = nullptr;
+  s.seg_h.clear();
+  s.seg_cap = 0;
   for (auto& t : s.sp) {
     if (t.map) (void)hipFree(t.map);
     if (t.nbr) (void)hipFree(t.nbr);

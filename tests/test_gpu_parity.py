@@ -93,7 +93,8 @@ CASES = [
                         [2.5], {}), (2, 4)),
     ("two_orbital", lambda: (_two_orb(), [3.0, 2.0], dict(Nsites=[2, 2], Ust=1.5, Jh=0.4, xmu=0.1)), (2, 2)),
     # DimUp = C(17,8) = 24310: an fp64 column (194 KB) exceeds LDS, so fp64
-    # takes the single-pass gather kernel and fp32 the two-pass kernels
+    # takes the segmented up pass (two row segments split by site 17) and
+    # fp32 the two-pass kernels
     ("ring17_wide_columns", lambda: (lattice.ring(17), [3.0], {}), (8, 1)),
 ]
 
@@ -138,6 +139,48 @@ def test_hxv_device_buffers_natural_layout(ed, orc, prec, dt, tol):
     got = ed.copy_to_host(hv, np.empty(dim, dt)).astype(np.float64)
     ref = orc.matvec(m, 7, 6, vh.astype(np.complex128), nthreads=8).real
     assert _relerr(got, ref) < tol
+
+
+@pytest.mark.parametrize("cfg,sec,rows", [("ring8", (4, 4), 16), ("rect3x4", (7, 6), 100), ("rect3x4", (6, 6), 300)])
+def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, monkeypatch):
+    """The segmented up pass (columns cut into row ranges that share their
+    top sites' occupation; cross-segment neighbours from global memory),
+    forced on small sectors by capping the rows per segment."""
+    monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", str(rows))
+    c = lattice.CONFIGS[cfg]
+    m = _use(ed, orc, c["H"](), [c["U"]])
+    isec = ed.sector_index(*sec)
+    dim, du, _ = ed.build_sector(isec)
+    print(f"segmented up pass {cfg} {sec} DimUp {du} rows/segment {rows}", flush=True)
+    up, dw, sched = ed.bench_passes(isec, "fp64", 1)
+    assert sched["two_pass"] == 2
+    for _ in range(2):
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+
+
+def test_lanczos_segmented_rect3x4_vs_oracle(ed, orc, monkeypatch):
+    """Ground state through the segmented step (phase B, beta, segmented up,
+    down, alpha): FP64 E0 against the oracle's Lanczos from the same start
+    vector, the mixed-precision E0 against FP64, and the densities of the
+    replayed eigenvector against the unsegmented run."""
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    ref_run = ed.lanczos_eigh(isec, "fp64", niter=200)
+    obs_ref = ed.observables()
+    monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "300")
+    r = ed.lanczos_eigh(isec, "fp64", niter=200)
+    obs = ed.observables()
+    Hc = orc.hamiltonian_csr(m, 6, 6)
+    e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
+    assert abs(r["e0"] - e0) < 1e-10 * abs(e0)
+    np.testing.assert_allclose(r["alpha"][:30], ref_run["alpha"][:30], rtol=1e-10, atol=1e-10)
+    for k in ("dens_up", "dens_dw", "docc"):
+        np.testing.assert_allclose(obs[k], obs_ref[k], atol=1e-10)
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=200, refine_tol=1e-13)
+    assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
 
 
 def test_hxv_rect3x4_gs_sector(ed, orc):
