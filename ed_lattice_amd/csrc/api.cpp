@@ -260,6 +260,69 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
 // otherwise the single-pass gather kernel.
 // v and out have column stride ld (DimUp: natural layout; padded_ld: the
 // engine's layout, two-pass sectors only).
+// Schedule of a segmented sector: the up-pass panel width C (segments are
+// re-cut for each C: wider panels amortise the hop words over more columns,
+// narrower ones keep more hops inside a segment) and the down-pass shape,
+// each timed on two launches of the sector itself -- a few dozen launches in
+// all, against the hundreds of the two-pass search, since one launch here
+// moves gigabytes.  Cached per sector shape, precision and row cap.
+template <typename T>
+static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
+  static std::map<std::tuple<int, int, int, int, int, int>, TwoPassCfg> cache;
+  const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T),
+                                   seg_row_cap(sizeof(T), 1));
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  hipStream_t st = c.stream;
+  const int ld = padded_ld(s.sp[0].dim);
+  const TwoPassView V = two_pass_view(s, ld);
+  const size_t bytes = sizeof(T) * (size_t)ld * s.sp[1].dim;
+  DevBuf v(bytes), t(bytes), o(bytes);
+  launch_zero_bytes(v.p, bytes, st);
+  TwoPassCfg best = seg_dw_cfg(s.sp[1].dim);
+  best.C = 1;
+  double bu = 1e300;
+  for (int C = 1; C <= 4; ++C) {
+    seg_bounds(s, sizeof(T), C);
+    double ms = 0.0;
+    for (int r = 0; r < 2; ++r) {
+      EventTimer tm(st);
+      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), t.as<T>(), nullptr, s, C, st);
+      ms = tm.stop();
+    }
+    if (verbose()) std::fprintf(stderr, "[edl] segmented up pass C %d: %zu segments, %.3f ms\n", C, s.seg_h.size() - 1, ms);
+    if (ms < bu) {
+      bu = ms;
+      best.C = C;
+    }
+  }
+  const int shapes[4][3] = {{64, 256, 256}, {64, 1024, 64}, {64, 512, 128}, {32, 256, 256}};
+  double bd = 1e300;
+  TwoPassCfg dw = best;
+  for (const auto& sh : shapes) {
+    TwoPassCfg w = best;
+    w.R = sh[0];
+    w.threads_dw = sh[1];
+    w.nchunk = std::max(1, std::min(sh[2], s.sp[1].dim));
+    double ms = 0.0;
+    for (int r = 0; r < 2; ++r) {
+      EventTimer tm(st);
+      launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+      ms = tm.stop();
+    }
+    if (verbose())
+      std::fprintf(stderr, "[edl] segmented sector down pass R %d x %d threads, %d chunks: %.3f ms\n", w.R,
+                   w.threads_dw, w.nchunk, ms);
+    if (ms < bd) {
+      bd = ms;
+      dw = w;
+    }
+  }
+  seg_bounds(s, sizeof(T), dw.C);
+  cache[key] = dw;
+  return dw;
+}
+
 // Column stride of the engine's own vectors on sector s: padded to whole
 // cache lines wherever the down pass runs (two-pass and segmented sectors).
 template <typename T>
@@ -271,12 +334,13 @@ template <typename T>
 static void apply_h(Ctx& c, Sector& s, const T* v, T* out, int ld, hipStream_t st) {
   if (seg_path(s, sizeof(T))) {
     // columns larger than LDS: segmented up pass, then the down pass
-    seg_bounds(s, sizeof(T));
+    const TwoPassCfg cfg = seg_cfg<T>(c, s);
+    seg_bounds(s, sizeof(T), cfg.C);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
     const TwoPassView V = two_pass_view(s, ld);
     T* vin = const_cast<T*>(v);  // operator mode only reads it
-    launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, st);
-    launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, out, nullptr, seg_dw_cfg(s.sp[1].dim), st);
+    launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, cfg.C, st);
+    launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, out, nullptr, cfg, st);
   } else if (two_pass_ok(s, sizeof(T))) {
     const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
@@ -467,7 +531,7 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
     T* q1 = static_cast<T*>(cs.h[0].q1);
     launch_phase_b<T>(tp->V, q0, q1, tp->L, st);
     launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
-    launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, st);
+    launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
     launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
     launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
   } else if (tp) {
@@ -588,11 +652,11 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   // Two-pass chains keep their vectors with the padded column stride.
   const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
+    tps.cfg = seg ? seg_cfg<T>(c, S) : two_pass_cfg<T>(c, S);
     if (seg) {
-      seg_bounds(S, sizeof(T));
+      seg_bounds(S, sizeof(T), tps.cfg.C);
       tps.seg = &S;
     }
-    tps.cfg = seg ? seg_dw_cfg(S.sp[1].dim) : two_pass_cfg<T>(c, S);
     tps.V = two_pass_view(S, ld);
     tps.lut_up = lut_dev<T>(S.sp[0]);
     tps.lut_dw = lut_dev<T>(S.sp[1]);
@@ -1495,13 +1559,13 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) EDL_HIP(hipEventCreate(&e));
   TwoPassCfg cf{};
   if (seg) {
-    cf = seg_dw_cfg(s.sp[1].dim);
-    cf.C = 0;
+    cf = seg_cfg<T>(c, s);
+    seg_bounds(s, sizeof(T), cf.C);
     const TwoPassView V = two_pass_view(s, ld);
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)Dp));
     for (int r = 0; r < reps; ++r) {
       EDL_HIP(hipEventRecord(ev[3 * r], st));
-      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, st);
+      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, cf.C, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
       launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 2], st));

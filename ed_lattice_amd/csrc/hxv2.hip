@@ -577,15 +577,16 @@ static __device__ __forceinline__ int64_t seg_xcd_tile(int64_t n8) {
   return (int64_t)(blockIdx.x % 8) * n8 + blockIdx.x / 8;
 }
 
-template <typename T, int UKCH>
+template <typename T, int C, int UKCH>
 __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                  T* qb, T* __restrict__ tout, LzChain L, int chain,
                                                  const int* __restrict__ seg, int nseg, int64_t ntiles, int64_t n8) {
+  typedef typename Vec<T, C>::type VT;
   extern __shared__ __align__(16) unsigned char smem[];
   T* lut = reinterpret_cast<T*>(smem);
-  T* P = reinterpret_cast<T*>(smem + lds_head_bytes(sizeof(T)));
-  // tiles column-major (the segments of a column are consecutive) and each
-  // XCD takes a contiguous range of them
+  VT* P = reinterpret_cast<VT*>(smem + lds_head_bytes(sizeof(T)));
+  // tiles (panel of C columns, segment), panel-major (the segments of a
+  // panel are consecutive), and each XCD takes a contiguous range of them
   const int64_t t = seg_xcd_tile(n8);
   if (t >= ntiles) return;
   const T* __restrict__ q = qa;
@@ -594,33 +595,58 @@ __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, c
     if (s.stopped) return;
     q = (s.k & 1) ? qb : qa;
   }
-  const int col = (int)(t / nseg), sg = (int)(t % nseg);
+  const int pan = (int)(t / nseg), sg = (int)(t % nseg);
   const int lo = seg[sg], hi = seg[sg + 1];
-  const int dimup = V.dimup;
-  const size_t base = (size_t)col * V.ld;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const int c0 = pan * C;
+  const size_t ld = (size_t)V.ld;
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
-  for (int r = lo + threadIdx.x; r < hi; r += blockDim.x) P[r - lo] = q[base + r];
+  for (int r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+    VT x;
+#pragma unroll
+    for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, (c0 + j < dimdw) ? q[(size_t)(c0 + j) * ld + r] : (T)0);
+    P[r - lo] = x;
+  }
   __syncthreads();
-  const uint32_t sd = (uint32_t)V.map_dw[col];
-  const double dgd = V.dg_dw[col];
+  uint32_t sd[C];
+  double dgd[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    const int cc = (c0 + j < dimdw) ? c0 + j : c0;
+    sd[j] = (uint32_t)V.map_dw[cc];
+    dgd[j] = V.dg_dw[cc];
+  }
   for (int row = lo + threadIdx.x; row < hi; row += blockDim.x) {
     const int nh = V.cnt_up[row];
-    T acc = (T)(V.dg_up[row] + dgd + inter2(I, (uint32_t)V.map_up[row], sd)) * P[row - lo];
+    const uint32_t su = (uint32_t)V.map_up[row];
+    const double dgu = V.dg_up[row];
+    const VT self = P[row - lo];
+    T acc[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
     for (int k0 = 0; k0 < nh; k0 += UKCH) {
       uint32_t w[UKCH];
 #pragma unroll
       for (int j = 0; j < UKCH; ++j) w[j] = (k0 + j < nh) ? V.pk_up[(size_t)(k0 + j) * dimup + row] : (255u << 24);
-      T x[UKCH];
 #pragma unroll
       for (int j = 0; j < UKCH; ++j) {
+        if (k0 + j >= nh) continue;
         const int nb = (int)(w[j] & 0xFFFFFFu);
-        const bool in = nb >= lo && nb < hi;
-        x[j] = (k0 + j < nh) ? (in ? P[in ? nb - lo : 0] : q[base + nb]) : (T)0;
-      }
+        const T cf = lut[w[j] >> 24];
+        VT x;
+        if (nb >= lo && nb < hi) {
+          x = P[nb - lo];
+        } else {
 #pragma unroll
-      for (int j = 0; j < UKCH; ++j) acc += lut[w[j] >> 24] * x[j];
+          for (int c = 0; c < C; ++c) Vec<T, C>::set(x, c, (c0 + c < dimdw) ? q[(size_t)(c0 + c) * ld + nb] : (T)0);
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] += cf * Vec<T, C>::get(x, c);
+      }
     }
-    tout[base + row] = acc;
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+      if (c0 + j < dimdw) tout[(size_t)(c0 + j) * ld + row] = acc[j];
   }
 }
 
@@ -681,9 +707,9 @@ __global__ __launch_bounds__(256) void k_lz_beta_n(LzChain L, int nparts) {
   }
 }
 
-int seg_row_cap(size_t elem) {
+int seg_row_cap(size_t elem, int C) {
   const size_t head = lds_head_bytes(elem);
-  int cap = (int)(((size_t)lds_limit_bytes() - 256 - head) / elem);
+  int cap = (int)(((size_t)lds_limit_bytes() - 256 - head) / (elem * (size_t)C));
   if (const char* e = std::getenv("EDL_UP_SEGMENT_ROWS")) {
     const int v = std::atoi(e);
     EDL_CHECK(v >= 1, "EDL_UP_SEGMENT_ROWS must be a positive row count");
@@ -694,11 +720,11 @@ int seg_row_cap(size_t elem) {
 
 bool seg_path(const Sector& s, size_t elem) {
   if (!(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1) return false;
-  return !two_pass_ok(s, elem) || seg_row_cap(elem) < s.sp[0].dim;
+  return !two_pass_ok(s, elem) || seg_row_cap(elem, 1) < s.sp[0].dim;
 }
 
-const std::vector<int>& seg_bounds(Sector& s, size_t elem) {
-  const int cap = seg_row_cap(elem);
+const std::vector<int>& seg_bounds(Sector& s, size_t elem, int C) {
+  const int cap = seg_row_cap(elem, C);
   if (s.d_seg && s.seg_cap == cap) return s.seg_h;
   const SpinTable& t = s.sp[0];
   std::vector<int32_t> map(t.dim);
@@ -741,27 +767,40 @@ TwoPassCfg seg_dw_cfg(int dimdw) {
   return p;
 }
 
-template <typename T>
-void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                   const LzChain* L, const Sector& s, hipStream_t st) {
-  EDL_CHECK(s.d_seg && s.seg_h.size() >= 2, "launch_up_seg: segment bounds not built");
+template <typename T, int C>
+static void launch_up_seg_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                            const LzChain* L, const Sector& s, hipStream_t st) {
   const int nseg = (int)s.seg_h.size() - 1;
   int rows = 0;
   for (int k = 0; k < nseg; ++k) rows = std::max(rows, s.seg_h[k + 1] - s.seg_h[k]);
-  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)rows * sizeof(T);
+  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)rows * C * sizeof(T);
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_up_seg: segment exceeds LDS");
   static bool attr = false;
   if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_up_seg<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    EDL_HIP(hipFuncSetAttribute((const void*)k_up_seg<T, C, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 lds_limit_bytes() - 256));
     attr = true;
   }
-  const int64_t ntiles = (int64_t)V.dimdw * nseg;
+  const int64_t ntiles = (int64_t)((V.dimdw + C - 1) / C) * nseg;
   const int64_t n8 = (ntiles + 7) / 8;
   EDL_CHECK(8 * n8 < ((int64_t)1 << 31), "launch_up_seg: too many tiles");
-  hipLaunchKernelGGL((k_up_seg<T, 8>), dim3((unsigned)(8 * n8)), dim3(1024), lds, st, V, I, lut, qa, qb, tout,
+  hipLaunchKernelGGL((k_up_seg<T, C, 8>), dim3((unsigned)(8 * n8)), dim3(1024), lds, st, V, I, lut, qa, qb, tout,
                      L ? *L : LzChain{}, L ? 1 : 0, s.d_seg, nseg, ntiles, n8);
   EDL_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                   const LzChain* L, const Sector& s, int C, hipStream_t st) {
+  EDL_CHECK(s.d_seg && s.seg_h.size() >= 2 && s.seg_cap == seg_row_cap(sizeof(T), C),
+            "launch_up_seg: segment bounds not built for this panel width");
+  switch (C) {
+    case 1: return launch_up_seg_c<T, 1>(V, I, lut, qa, qb, tout, L, s, st);
+    case 2: return launch_up_seg_c<T, 2>(V, I, lut, qa, qb, tout, L, s, st);
+    case 3: return launch_up_seg_c<T, 3>(V, I, lut, qa, qb, tout, L, s, st);
+    case 4: return launch_up_seg_c<T, 4>(V, I, lut, qa, qb, tout, L, s, st);
+    default: throw Error("launch_up_seg: unsupported panel width");
+  }
 }
 
 int phase_b_grid(const TwoPassView& V) { return std::min(V.dimdw, 4096); }
@@ -778,9 +817,9 @@ void launch_lz_beta_n(const LzChain& L, int nparts, hipStream_t st) {
 }
 
 template void launch_up_seg<double>(const TwoPassView&, const InterParams&, const double*, double*, double*, double*,
-                                    const LzChain*, const Sector&, hipStream_t);
+                                    const LzChain*, const Sector&, int, hipStream_t);
 template void launch_up_seg<float>(const TwoPassView&, const InterParams&, const float*, float*, float*, float*,
-                                   const LzChain*, const Sector&, hipStream_t);
+                                   const LzChain*, const Sector&, int, hipStream_t);
 template void launch_phase_b<double>(const TwoPassView&, double*, double*, const LzChain&, hipStream_t);
 template void launch_phase_b<float>(const TwoPassView&, float*, float*, const LzChain&, hipStream_t);
 

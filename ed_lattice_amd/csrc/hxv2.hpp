@@ -87,24 +87,27 @@ void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& c
 // inside it are gathered there and the others from global memory (L2: the
 // tiles of a column run next to each other on one XCD).
 //
-// Rows per segment at most: the LDS capacity for elements of `elem` bytes,
-// lowered by EDL_UP_SEGMENT_ROWS when set (tests force several segments on
-// small sectors with it).
-int seg_row_cap(size_t elem);
+// Rows per segment at most: the LDS capacity for C columns of elements of
+// `elem` bytes, lowered by EDL_UP_SEGMENT_ROWS when set (tests force several
+// segments on small sectors with it).
+int seg_row_cap(size_t elem, int C);
 // Whether the operator and the Lanczos chains of `s` use the segmented up
 // pass with the down pass: packed hop tables, and either the columns exceed
 // LDS or EDL_UP_SEGMENT_ROWS is set below DimUp.
 bool seg_path(const Sector& s, size_t elem);
-// Segment bounds of `s` for rows of `elem` bytes (built on first use).
-const std::vector<int>& seg_bounds(Sector& s, size_t elem);
+// Segment bounds of `s` for panels of C columns of `elem` bytes (rebuilt
+// when the row cap changes).
+const std::vector<int>& seg_bounds(Sector& s, size_t elem, int C);
 // Fixed down-pass schedule of segmented sectors (no tuning: one launch of
 // such a sector moves gigabytes).
 TwoPassCfg seg_dw_cfg(int dimdw);
 // Operator mode (L = null): tout = [D + H_up] qa.  Chain mode: T = [D +
-// H_up] q_k unscaled for the current k (phase B already done).
+// H_up] q_k unscaled for the current k (phase B already done).  Tiles are
+// (panel of C columns, segment); seg_bounds(s, sizeof(T), C) must be the
+// sector's current plan.
 template <typename T>
 void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                   const LzChain* L, const Sector& s, hipStream_t st);
+                   const LzChain* L, const Sector& s, int C, hipStream_t st);
 // Phase B of a chain as a streaming pass (r = t - alpha_k q_k / s_k over t,
 // block partials of |r|^2 into L.part_up, replay term x += c_k q_k / s_k),
 // for the segmented path; `grid` blocks = the partial count for
