@@ -577,12 +577,20 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
   return done;
 }
 
-// EDL_BATCHED_KERNELS=single keeps batched chains on the single-pass kernels
-// (tests run both paths on the same seeds).
+// EDL_BATCHED_KERNELS=single keeps batched chains on the single-pass kernels,
+// =chain runs them one at a time through the two-pass (or segmented) step
+// wherever the sectors allow (tests run every path on the same seeds).
 static bool batched_single_pass_forced() {
   const char* e = std::getenv("EDL_BATCHED_KERNELS");
   return e && std::strcmp(e, "single") == 0;
 }
+static bool batched_chain_forced() {
+  const char* e = std::getenv("EDL_BATCHED_KERNELS");
+  return e && std::strcmp(e, "chain") == 0;
+}
+// Sectors from this many states on fill the GPU with one chain: a group of
+// them runs its chains one after another through the two-pass step.
+static constexpr int64_t kChainAloneDim = 50000000;
 
 // `steps` Lanczos steps of every chain: up pass (with the previous step's
 // phase B), down pass + phase A; captured once as a graph and replayed.
@@ -1287,15 +1295,27 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     max_dw = std::max(max_dw, J.sp[1].dim);
     packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok;
   }
-  // batched two-pass kernels when every target sector fits them
-  const bool bt = packed && bt_shape(max_up, max_dw, sizeof(T)).RS > 0 && !batched_single_pass_forced();
+  // batched two-pass kernels when every target sector fits them; chains on
+  // sectors large enough to fill the GPU alone: one at a time through the
+  // two-pass or segmented step; otherwise the single-pass batch
+  bool bt = packed && bt_shape(max_up, max_dw, sizeof(T)).RS > 0 && !batched_single_pass_forced();
+  bool alone = false;
+  if (packed && !batched_single_pass_forced() && (batched_chain_forced() || !bt)) {
+    alone = true;
+    for (size_t k = 0; k < idx.size(); ++k) {
+      Sector& J = get_sector(c, targets[idx[k]]);
+      const bool fits = two_pass_ok(J, sizeof(T)) || seg_path(J, sizeof(T));
+      alone = alone && fits && (batched_chain_forced() || J.dim >= kChainAloneDim);
+    }
+    if (alone) bt = false;
+  }
   // every buffer of the group in one allocation
   Arena ar;
   {
     size_t bytes = 0;
     for (size_t k = 0; k < idx.size(); ++k) {
       const Sector& J = get_sector(c, targets[idx[k]]);
-      const int ld = bt ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
+      const int ld = (bt || alone) ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
       const int64_t D = (int64_t)ld * J.sp[1].dim;
       bytes += ChainAlloc<T>::arena_bytes(D, (int)std::min<int64_t>(nlanc, J.dim));
       if (bt)
@@ -1323,7 +1343,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     device_dot_to(tmp.as<double>(), tmp.as<double>(), J.dim, part.as<double>(), d_n2 + k, st);
     const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
     ChainDev cd{};
-    const int ld = bt ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
+    const int ld = (bt || alone) ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
     allocs[k].ar = &ar;
     init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), 0.0, nmax, st, ld, d_ss + k, part.as<double>());
     if (bt) {
@@ -1373,6 +1393,37 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   EventTimer run_tm(st);
   if (bt) {
     run_steps_bt<T>(c, bs, steps, st);
+  } else if (alone) {
+    for (size_t k = 0; k < chains.size(); ++k) {
+      Sector& J = get_sector(c, targets[idx[k]]);
+      const bool seg = seg_path(J, sizeof(T));
+      TwoPassStep<T> tps{};
+      tps.cfg = seg ? seg_cfg<T>(c, J) : two_pass_cfg<T>(c, J);
+      if (seg) {
+        seg_bounds(J, sizeof(T), tps.cfg.C);
+        tps.seg = &J;
+      }
+      tps.V = two_pass_view(J, chains[k].ld);
+      tps.lut_up = lut_dev<T>(J.sp[0]);
+      tps.lut_dw = lut_dev<T>(J.sp[1]);
+      tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)chains[k].ld * J.sp[1].dim));
+      DevBuf pu(sizeof(double) * (seg ? phase_b_grid(tps.V) : pass_up_grid(tps.V, tps.cfg)));
+      DevBuf pd(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
+      tps.L.part_up = pu.as<double>();
+      tps.L.part_dw = pd.as<double>();
+      tps.L.sc = chains[k].sc;
+      tps.L.alpha = chains[k].alpha;
+      tps.L.beta = chains[k].beta;
+      tps.L.nmax = nmaxs[k];
+      tps.L.brk_tol = chains[k].brk_tol;
+      ChainSet<T> one;
+      one.h.push_back(chains[k]);
+      Schedule sch;
+      sch.block_threads = 256;
+      sch.grid = c.num_cu * 8;
+      chainset_finalize(c, one, sch);
+      run_steps(c, one, nmaxs[k], 0, 0.0, st, &tps);
+    }
   } else {
     Schedule sch = autotune<T>(c, chains, st);
     chainset_finalize(c, cs, sch);
@@ -1381,8 +1432,8 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   if (verbose()) {
     const double ms = run_tm.stop();
     std::fprintf(stderr, "[edl] batched group: %zu chains, %s kernels (C %d, RS %d), %d steps in %.3f ms (%.1f us/step)\n",
-                 chains.size(), bt ? "two-pass" : "single-pass", bt ? shp.C : 0, bt ? shp.RS : 0, steps, ms,
-                 1e3 * ms / steps);
+                 chains.size(), bt ? "two-pass" : (alone ? "one-chain two-pass" : "single-pass"), bt ? shp.C : 0,
+                 bt ? shp.RS : 0, steps, ms, 1e3 * ms / steps);
   }
   int64_t hxv = 0;
   for (size_t k = 0; k < chains.size(); ++k) {

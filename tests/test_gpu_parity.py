@@ -295,12 +295,18 @@ def _assert_converged_poles_match(gpu, ora, tol, window=1e-6, conv_tol=1e-10):
             assert abs((p[sg] * w[sg]).sum() / wg - (po[so] * wo[so]).sum() / wo_) < tol * max(1.0, 1e-6 / wg)
 
 
-@pytest.fixture(params=["two_pass", "single"])
+@pytest.fixture(params=["two_pass", "single", "chain", "chain_segmented"])
 def batched_kernels(request, monkeypatch):
-    """Kernels of the batched chains: the two-pass LDS kernels (default
-    where the sectors fit) or the single-pass gather kernels."""
+    """Kernels of the batched chains: the batched two-pass LDS kernels
+    (default where the sectors fit), the single-pass gather kernels, or one
+    chain at a time through the two-pass step (the route of sectors large
+    enough to fill the GPU alone), also with the segmented up pass."""
     if request.param == "single":
         monkeypatch.setenv("EDL_BATCHED_KERNELS", "single")
+    elif request.param.startswith("chain"):
+        monkeypatch.setenv("EDL_BATCHED_KERNELS", "chain")
+        if request.param == "chain_segmented":
+            monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "24")
     else:
         monkeypatch.delenv("EDL_BATCHED_KERNELS", raising=False)
     print(f"batched kernels: {request.param}", flush=True)
