@@ -60,7 +60,13 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
     writes H v (3*D*elem)."""
     t = "float" if elem == 4 else "double"
     out = {}
-    if sched["two_pass"]:
+    if sched["two_pass"] == 2:  # segmented up pass (columns larger than LDS)
+        for name, ms, nbytes, k in (
+                ("pass_up", up_ms, 2 * dim * elem, f"k_up_seg<{t}, {sched['up_cols']}, 8>"),
+                ("pass_dw", dw_ms, 3 * dim * elem,
+                 f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>")):
+            out[name] = {"kernel": k, "ms": ms, "algorithmic_bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+    elif sched["two_pass"]:
         for name, ms, nbytes, k in (
                 ("pass_up", up_ms, 2 * dim * elem,
                  f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}>"),
@@ -121,6 +127,26 @@ def seeds_benchmark(ed, nseq=24):
     return out
 
 
+def large_sector_benchmark(ed, reps=3):
+    """3x6 open cluster, U = 4, seeded disorder (bonds U[0.8,1.2], onsite
+    U[-0.5,0.5]), sector (9,9), dim 2 363 904 400 (an fp32 vector is
+    9.5 GB): the operator on device-resident vectors in fp32 and fp64 (its
+    up-spin columns exceed LDS, so the up pass runs on row segments)."""
+    from ed_lattice_amd import lattice
+
+    ed.set_hamiltonian(lattice.disordered_rect(3, 6), [4.0])
+    isec = ed.sector_index(9, 9)
+    dim, _, _ = ed.build_sector(isec)
+    out = {"workload": "3x6 OBC Hubbard U=4, seeded disorder, sector (9,9), dim 2363904400"}
+    for prec, elem in (("fp32", 4), ("fp64", 8)):
+        ms = ed.bench_hxv(isec, prec, reps)
+        up, dw, sched = ed.bench_passes(isec, prec, reps)
+        out[prec] = {"ms_per_hxv": ms, "hxv_per_s": 1e3 / ms,
+                     "hxv_algorithmic_gbs": 2 * dim * elem / (ms * 1e-3) / 1e9,
+                     "kernels": kernel_table(dim, elem, up, dw, sched), "schedule": sched}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-seeds", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -199,6 +226,8 @@ def main():
         }
     if not args.no_seeds:
         out["gf_seeds"] = seeds_benchmark(ed)
+    if not args.no_large:
+        out["obc3x6"] = large_sector_benchmark(ed)
     ed.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(H, U, nup, ndw, dim)
