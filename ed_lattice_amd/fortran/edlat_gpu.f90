@@ -27,6 +27,10 @@ program edlat_gpu
   real(8) :: Uloc = 2d0, Ust = 0d0, Jh = 0d0, Jx = 0d0, Jp = 0d0, temp = 0.001d0, xmu = 0d0, eps = 0.01d0
   real(8) :: wini = -5d0, wfin = 5d0, gs_threshold = 1d-9, lanc_tolerance = 1d-18
   logical :: hfmode = .true., gf_flag = .false., offdiag_gf_flag = .false., chispin_flag = .false.
+  ! ED_SPARSE_H (INPUT_VARS.f90:151, default T): the stored-H path's operator
+  ! keeps the -xmu*N and hfmode -U/2*N diagonal terms at any ED_FILLING
+  ! (stored/H_local.f90:61-90); both paths run on the on-the-fly kernels
+  logical :: ed_sparse_h = .true.
   character(len=256) :: hij_file = "hij.in", ed_precision = "fp64", input_file = "inputED.conf"
   integer(c_int64_t) :: lanc_seed = int(z'5EED', c_int64_t)
 
@@ -105,6 +109,7 @@ contains
        case ("TEMP"); read(val, *) temp
        case ("XMU"); read(val, *) xmu
        case ("HFMODE"); read(val, *) hfmode
+       case ("ED_SPARSE_H"); read(val, *) ed_sparse_h
        case ("EPS"); read(val, *) eps
        case ("WINI"); read(val, *) wini
        case ("WFIN"); read(val, *) wfin
@@ -125,7 +130,7 @@ contains
        case ("ED_PRECISION"); ed_precision = trim(val)
        case ("LANC_SEED"); read(val, *) lanc_seed
        case default
-          ! other reference variables (ED_SPARSE_H, LANC_METHOD, ...) are accepted and ignored
+          ! other reference variables (ED_METHOD, LANC_METHOD, ...) are accepted and ignored
        end select
     enddo
     close(unit)
@@ -167,7 +172,7 @@ contains
     nsites(1) = Ns
     U(1) = Uloc
     call edl_check(ctx, edl_set_hamiltonian(ctx, Ns, Nspin, 1, nsites, Hij, c_null_ptr, U, Ust, Jh, Jx, Jp, xmu, &
-         merge(1, 0, hfmode), ed_filling), "edl_set_hamiltonian")
+         merge(1, 0, hfmode), merge(0, ed_filling, ed_sparse_h)), "edl_set_hamiltonian")
   end subroutine set_model
 
   subroutine select_sectors()

@@ -101,7 +101,14 @@ class EDLattice:
 
     # ------------------------------------------------------------ model
     def set_hamiltonian(self, Hij, Uloc, Nsites=None, Ust=0.0, Jh=0.0, Jx=0.0, Jp=0.0, xmu=0.0, hfmode=True,
-                        ed_filling=0):
+                        ed_filling=0, sparse_h=False):
+        """sparse_h: the operator of the reference's stored-H path
+        (ED_SPARSE_H=T, its default).  That path conjugates Hij on the hops
+        (stored/H_up.f90:19; the same for the real Hij accepted here) and
+        applies the -xmu*N and hfmode -U/2*N diagonal terms whatever
+        ED_FILLING is (stored/H_local.f90:61-90, where the direct path has
+        them only for ED_FILLING=0, direct/HxV_local.f90:63-92), so it is the
+        direct operator with ED_FILLING=0 and runs on the same kernels."""
         H = np.asarray(Hij)
         if H.ndim == 2:
             H = H[None]
@@ -113,7 +120,7 @@ class EDLattice:
         im = np.asfortranarray(np.imag(H)).ravel(order="F").copy() if np.iscomplexobj(H) else None
         self._ck(self.lib.edl_set_hamiltonian(self.h, Ns, nspin, Norb, _ip(Nsites), _dp(re),
                                               _dp(im) if im is not None else None, _dp(Uloc), Ust, Jh, Jx, Jp,
-                                              xmu, int(bool(hfmode)), ed_filling))
+                                              xmu, int(bool(hfmode)), 0 if sparse_h else ed_filling))
         self.Ns = Ns
 
     # ----------------------------------------------------------- sectors

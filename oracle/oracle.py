@@ -71,7 +71,14 @@ class Model:
     """The module globals directMatVec reads: Hij (Nspin,Ns,Ns) and input variables."""
 
     def __init__(self, Hij, Uloc, Nsites=None, Ust=0.0, Jh=0.0, Jx=0.0, Jp=0.0, xmu=0.0, hfmode=True,
-                 ed_filling=0):
+                 ed_filling=0, sparse_h=False):
+        """sparse_h: the stored-H path's operator (ED_SPARSE_H=T).  Its
+        diagonal applies the -xmu*N and hfmode -U/2*N terms unconditionally
+        (stored/H_local.f90:61-90: the ED_FILLING test of the direct path,
+        direct/HxV_local.f90:63, is commented out there), and its hops use
+        conjg(Hij) (stored/H_up.f90:19, stored/H_dw.f90), equal to Hij for the
+        real matrices restated here: the direct restatement with the filling
+        test off."""
         H = np.asarray(Hij)
         if H.ndim == 2:
             H = H[None]
@@ -82,7 +89,8 @@ class Model:
         self.Nsites = np.asarray(Nsites if Nsites is not None else [self.Ns // self.Norb] * self.Norb,
                                  dtype=np.int32)
         self.Ust, self.Jh, self.Jx, self.Jp, self.xmu = Ust, Jh, Jx, Jp, xmu
-        self.hfmode, self.ed_filling = int(bool(hfmode)), ed_filling
+        self.hfmode = int(bool(hfmode))
+        self.ed_filling = 0 if sparse_h else ed_filling
         self._re = np.asfortranarray(np.real(H)).ravel(order="F").copy()
         self._im = np.asfortranarray(np.imag(H)).ravel(order="F").copy()
 

@@ -19,18 +19,18 @@ INPUT = """NORB=1
 NSITES=8          ! sites
 NSPIN=1
 ULOC=4.0
-XMU=0.0
+XMU={xmu}
 HFMODE=T
-ED_FILLING=0
-ED_SPARSE_H=F
+ED_FILLING={filling}
+ED_SPARSE_H={sparse}
 ED_METHOD=lanczos
 LANC_METHOD=lanczos
 LANC_NSTATES_SECTOR=1
 LANC_NITER=200
 LANC_NGFITER=200
-GF_FLAG=T
+GF_FLAG={gf}
 OFFDIAG_GF_FLAG={offdiag}
-CHISPIN_FLAG=T
+CHISPIN_FLAG={gf}
 LMATS=400
 LREAL=400
 HIJ_FILE=hij.in
@@ -40,8 +40,9 @@ ED_PRECISION={prec}
 """
 
 
-def _prepare(tmp_path, offdiag="F", prec="fp64"):
-    (tmp_path / "inputED.conf").write_text(INPUT.format(offdiag=offdiag, prec=prec))
+def _prepare(tmp_path, offdiag="F", prec="fp64", xmu=0.0, filling=0, sparse="F", gf="T"):
+    (tmp_path / "inputED.conf").write_text(INPUT.format(offdiag=offdiag, prec=prec, xmu=xmu, filling=filling,
+                                                        sparse=sparse, gf=gf))
     lattice.write_hij(str(tmp_path / "hij.in"), lattice.ring(8))
 
 
@@ -99,3 +100,19 @@ def test_driver_ring8_vs_oracle(tmp_path, orc, prec):
     np.testing.assert_allclose(gf, ref, atol=1e-6)
     assert (tmp_path / "gfmatrix.restart").exists()
     assert (tmp_path / "spinChi_i0001_l1_tau.ed").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sparse", ["T", "F"])
+def test_driver_ed_sparse_h_route(tmp_path, orc, sparse):
+    """ED_SPARSE_H=T (the reference default) runs the stored path's operator
+    on the on-the-fly kernels: at ED_FILLING = 8 its E0 keeps the -xmu*N -
+    U/2*N terms that the direct path (ED_SPARSE_H=F) drops."""
+    _prepare(tmp_path, xmu=0.3, filling=8, sparse=sparse, gf="F")
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    m = orc.Model(lattice.ring(8), [4.0], xmu=0.3, ed_filling=8, sparse_h=(sparse == "T"))
+    w0 = np.linalg.eigvalsh(orc.hamiltonian_csr(m, 4, 4).toarray().real)[0]
+    egs = float((tmp_path / "egs.ed").read_text().split()[0])
+    assert abs(egs - w0) < 1e-10 * abs(w0)

@@ -245,6 +245,24 @@ def test_mixed_precision_e0(ed, orc, cfg):
     assert abs(dmx - d64) < 1e-7
 
 
+def test_stored_path_route_e0(ed, orc):
+    """ED_SPARSE_H=T routed to the on-the-fly kernels: with ED_FILLING = N
+    the stored path keeps the -xmu*N - U/2*N diagonal terms the direct path
+    drops, so E0 moves by exactly that constant; both runs agree with dense
+    eigh of the oracle's matching operator."""
+    H, U, xmu, sec = lattice.ring(8), 4.0, 0.3, (4, 4)
+    N = sum(sec)
+    e = {}
+    for sparse in (False, True):
+        ed.set_hamiltonian(H, [U], xmu=xmu, ed_filling=N, sparse_h=sparse)
+        isec = ed.sector_index(*sec)
+        e[sparse] = ed.lanczos_eigh(isec, "fp64", niter=200)["e0"]
+        m = orc.Model(H, [U], xmu=xmu, ed_filling=N, sparse_h=sparse)
+        w0 = np.linalg.eigvalsh(orc.hamiltonian_csr(m, *sec).toarray().real)[0]
+        assert abs(e[sparse] - w0) < 1e-10 * abs(w0)
+    assert abs((e[True] - e[False]) - (-(xmu + U / 2) * N)) < 1e-9
+
+
 def test_rect3x4_e0_vs_oracle(ed, orc):
     m = _use(ed, orc, lattice.square(3, 4), [6.0])
     isec = ed.sector_index(6, 6)

@@ -113,3 +113,21 @@ def test_dim_overflow_refused(orc):
     m = orc.Model(lattice.disordered_rect(3, 6), [4.0])
     with pytest.raises(RuntimeError):
         orc.hxv_rows(m, 9, 9, np.array([0]), np.zeros(1))
+
+
+@pytest.mark.parametrize("hfmode", [True, False])
+def test_stored_path_is_direct_plus_sector_constant(orc, hfmode):
+    """The stored-H operator (ED_SPARSE_H=T) applies -xmu*N and, with hfmode,
+    -U/2*N on the diagonal whatever ED_FILLING is (stored/H_local.f90:61-90);
+    the direct operator only at ED_FILLING=0 (direct/HxV_local.f90:63-92).
+    In a sector of N = Nup + Ndw electrons on a one-orbital lattice the two
+    therefore differ by the constant -(xmu + hfmode*U/2)*N times the
+    identity, with every hop and U term equal."""
+    H, U, xmu, sec = lattice.ring(6), 3.0, 0.45, (3, 2)
+    N = sum(sec)
+    direct = orc.Model(H, [U], xmu=xmu, hfmode=hfmode, ed_filling=N)
+    stored = orc.Model(H, [U], xmu=xmu, hfmode=hfmode, ed_filling=N, sparse_h=True)
+    Hd = orc.hamiltonian_csr(direct, *sec).toarray()
+    Hs = orc.hamiltonian_csr(stored, *sec).toarray()
+    shift = -(xmu + (U / 2 if hfmode else 0.0)) * N
+    np.testing.assert_allclose(Hs - Hd, shift * np.eye(Hd.shape[0]), atol=1e-12)
