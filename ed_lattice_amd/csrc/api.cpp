@@ -814,8 +814,6 @@ extern "C" int edl_set_hamiltonian(edl_ctx* ctx, int Ns, int Nspin, int Norb, co
   int sum = 0;
   for (int k = 0; k < Norb; ++k) sum += Nsites[k];
   EDL_CHECK(sum == Ns, "edl_set_hamiltonian: Ns != sum(Nsites)");
-  EDL_CHECK(!(Norb > 1 && (Jx != 0.0 || Jp != 0.0)),
-            "edl_set_hamiltonian: spin-exchange / pair-hopping (Jhflag) is not supported");
   const size_t n = (size_t)Nspin * Ns * Ns;
   for (size_t k = 0; k < n; ++k)
     EDL_CHECK(!Him || Him[k] == 0.0, "edl_set_hamiltonian: complex Hij is not supported (real kernels)");
@@ -874,6 +872,9 @@ extern "C" int edl_set_hamiltonian(edl_ctx* ctx, int Ns, int Nspin, int Norb, co
       }
   I.npairs = np;
   I.Ust = Ust;
+  // Jhflag = Norb > 1 and (Jx or Jp non-zero) (SETUP.f90:35-36)
+  I.Jx = Norb > 1 ? Jx : 0.0;
+  I.Jp = Norb > 1 ? Jp : 0.0;
   M.inter = I;
   M.set = true;
   return 0;
@@ -1293,7 +1294,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     maxdim = std::max<int64_t>(maxdim, J.dim);
     max_up = std::max(max_up, J.sp[0].dim);
     max_dw = std::max(max_dw, J.sp[1].dim);
-    packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok;
+    packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok && !J.nonlocal;
   }
   // batched two-pass kernels when every target sector fits them; chains on
   // sectors large enough to fill the GPU alone: one at a time through the

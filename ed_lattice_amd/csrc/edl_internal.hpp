@@ -43,6 +43,10 @@ struct InterParams {
   int npairs;                // same-site, different-orbital pairs (io, jo)
   uint8_t pi[kMaxPairs], pj[kMaxPairs];  // 0-based positions
   double Ust;
+  // Non-local Kanamori block (HxV_non_local.f90), on when Norb > 1 and Jx or
+  // Jp is non-zero (Jhflag, SETUP.f90:35-36): spin exchange and pair
+  // hopping between the orbitals of a site.
+  double Jx, Jp;
 };
 
 // One spin species of one sector: sorted strings, hopping table, diagonal.
@@ -81,6 +85,9 @@ struct Sector {
   std::vector<int> seg_h;
   int* d_seg = nullptr;
   int seg_cap = 0;
+  // The model has the non-local Kanamori block: every operator and chain of
+  // this sector runs on the single-pass kernels, which carry that term.
+  bool nonlocal = false;
 };
 
 struct Model {

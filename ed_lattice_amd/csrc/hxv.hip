@@ -52,6 +52,49 @@ static __device__ __forceinline__ double inter_term(const InterParams& I, uint32
   return e;
 }
 
+// Fermionic sign of c_pos / cdg_pos on string s (AUX_FUNX.f90:76-102): the
+// parity of the occupied sites below pos.
+static __device__ __forceinline__ double fsign(uint32_t s, uint32_t pos) {
+  return (__popc(s & ((1u << pos) - 1u)) & 1) ? -1.0 : 1.0;
+}
+
+// Row (su, sd) of the non-local Kanamori block (HxV_non_local.f90:14-88) in
+// gather form.  The reference scatters, from a source string pair, to the
+// pair reached by c+_io c_jo (up) c+_jo c_io (down) with Jx (spin exchange)
+// or c+_io c_jo (up) c+_io c_jo (down) with Jp (pair hopping), for every
+// ordered same-site orbital pair; the block is symmetric, so row i gathers
+// with the coefficient of the move that starts at i.
+template <typename T>
+static __device__ __forceinline__ T nonlocal_row(const InterParams& I, uint32_t su, uint32_t sd,
+                                                 const T* __restrict__ q, size_t ld) {
+  double acc = 0.0;
+  for (int p = 0; p < I.npairs; ++p)
+    for (int o = 0; o < 2; ++o) {
+      const uint32_t io = o ? I.pj[p] : I.pi[p], jo = o ? I.pi[p] : I.pj[p];
+      const uint32_t bio = 1u << io, bjo = 1u << jo;
+      const bool up_j = su & bjo, up_i = su & bio, dw_j = sd & bjo, dw_i = sd & bio;
+      if (I.Jx != 0.0 && up_j && !up_i && dw_i && !dw_j) {
+        const uint32_t k1 = sd ^ bio;                    // c_io (down)
+        const double s12 = fsign(sd, io) * fsign(k1, jo);  // then cdg_jo
+        const uint32_t k2 = k1 | bjo;
+        const uint32_t k3 = su ^ bjo;                    // c_jo (up)
+        const double s34 = fsign(su, jo) * fsign(k3, io);  // then cdg_io
+        const uint32_t k4 = k3 | bio;
+        acc += I.Jx * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
+      }
+      if (I.Jp != 0.0 && up_j && dw_j && !up_i && !dw_i) {
+        const uint32_t k1 = sd ^ bjo;                    // c_jo (down)
+        const double s12 = fsign(sd, jo) * fsign(k1, io);  // then cdg_io
+        const uint32_t k2 = k1 | bio;
+        const uint32_t k3 = su ^ bjo;                    // c_jo (up)
+        const double s34 = fsign(su, jo) * fsign(k3, io);  // then cdg_io
+        const uint32_t k4 = k3 | bio;
+        acc += I.Jp * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
+      }
+    }
+  return (T)acc;
+}
+
 // Tile walk of one block: XCD-synchronous strided order (see header).
 struct TileWalk {
   int64_t t, end, step;
@@ -121,6 +164,7 @@ static __device__ __forceinline__ void column_hq(const ChainDev& C, const InterP
 #pragma unroll
       for (int j = 0; j < KCH; ++j) acc += c[j] * x[j];
     }
+    if (I.Jx != 0.0 || I.Jp != 0.0) acc += nonlocal_row<T>(I, su, sd, q, C.ld);
     emit(col + iup, acc);
   }
 }

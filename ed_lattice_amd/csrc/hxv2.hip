@@ -438,7 +438,7 @@ static size_t up_lds_bytes(int dimup, int C, size_t sz) {
 }
 
 bool two_pass_ok(const Sector& s, size_t elem) {
-  return s.sp[0].packed_ok && s.sp[1].packed_ok &&
+  return !s.nonlocal && s.sp[0].packed_ok && s.sp[1].packed_ok &&
          up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes() - 256;
 }
 
@@ -719,7 +719,8 @@ int seg_row_cap(size_t elem, int C) {
 }
 
 bool seg_path(const Sector& s, size_t elem) {
-  if (!(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1) return false;
+  if (s.nonlocal || !(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1)
+    return false;
   return !two_pass_ok(s, elem) || seg_row_cap(elem, 1) < s.sp[0].dim;
 }
 

@@ -274,6 +274,7 @@ void build_sector_tables(Ctx& c, Sector& s) {
   build_spin(c, c.model, s.sp[0], 0);
   build_spin(c, c.model, s.sp[1], 1);
   s.dim = (int64_t)s.sp[0].dim * (int64_t)s.sp[1].dim;
+  s.nonlocal = c.model.inter.Jx != 0.0 || c.model.inter.Jp != 0.0;
 }
 
 void free_sector(Sector& s) {

@@ -45,9 +45,10 @@ int edl_ctx_destroy(edl_ctx* ctx);
 const char* edl_last_error(const edl_ctx* ctx);
 
 /* Hamiltonian parameters: Hij as returned by Hij_get (GRAPH_MATRIX.f90:334-345)
- * plus the input variables read by HxV_local / HxV_up / HxV_dw
- * (INPUT_VARS.f90:106-166).  Jx, Jp != 0 with Norb > 1 (Jhflag,
- * SETUP.f90:35-36) is refused: the non-local Kanamori block is not built.
+ * plus the input variables read by HxV_local / HxV_up / HxV_dw /
+ * HxV_non_local (INPUT_VARS.f90:106-166).  With Norb > 1 and Jx or Jp
+ * non-zero (Jhflag, SETUP.f90:35-36) the spin-exchange and pair-hopping
+ * block is applied too; such models run on the single-pass kernels.
  * Hij must be Hermitian to 1e-6 (GRAPH_MATRIX.f90:324-326). */
 int edl_set_hamiltonian(edl_ctx* ctx, int Ns, int Nspin, int Norb, const int* Nsites, const double* Hij_re,
                         const double* Hij_im, const double* Uloc, double Ust, double Jh, double Jx, double Jp,

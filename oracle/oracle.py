@@ -129,9 +129,13 @@ def cdg_op(pos, s):
 
 
 def matvec(model, nup, ndw, v, frac=1.0, nthreads=0):
-    """directMatVec on a complex(8) vector (faithful scatter restatement)."""
+    """directMatVec on a complex(8) vector (faithful scatter restatement).
+    The threaded split is race-free only without the non-local block, whose
+    moves change both strings: models with it run serially."""
     vin = np.ascontiguousarray(v, dtype=np.complex128)
     out = np.zeros_like(vin)
+    if model.Norb > 1 and (model.Jx != 0.0 or model.Jp != 0.0):
+        nthreads = 0
     if nthreads:
         rc = lib().or_direct_matvec_omp(*model.args(), nup, ndw, _dp(vin.view(np.float64)),
                                         _dp(out.view(np.float64)), frac, nthreads)

@@ -92,11 +92,30 @@ CASES = [
     ("nspin2", lambda: (np.stack([lattice.disordered_rect(2, 3, seed=5)[0], lattice.disordered_rect(2, 3, seed=6)[0]]),
                         [2.5], {}), (2, 4)),
     ("two_orbital", lambda: (_two_orb(), [3.0, 2.0], dict(Nsites=[2, 2], Ust=1.5, Jh=0.4, xmu=0.1)), (2, 2)),
+    # spin exchange and pair hopping (HxV_non_local.f90), sectors with and
+    # without the moves available to every state
+    ("two_orbital_kanamori", lambda: (_two_orb(), [3.0, 2.0], dict(Nsites=[2, 2], Ust=1.5, Jh=0.4, Jx=0.4, Jp=0.35)),
+     (2, 2)),
+    ("three_orbital_kanamori", lambda: (_three_orb(), [4.0, 3.0, 2.5],
+                                        dict(Nsites=[2, 2, 2], Ust=2.0, Jh=0.6, Jx=0.6, Jp=0.5, xmu=0.2)), (3, 2)),
     # DimUp = C(17,8) = 24310: an fp64 column (194 KB) exceeds LDS, so fp64
     # takes the segmented up pass (two row segments split by site 17) and
     # fp32 the two-pass kernels
     ("ring17_wide_columns", lambda: (lattice.ring(17), [3.0], {}), (8, 1)),
 ]
+
+
+def _three_orb():
+    # 3 orbitals x 2 sites: intra-orbital hops between the sites, weak
+    # inter-orbital hybridisation on a site, crystal field
+    Ns = 6
+    H = np.zeros((1, Ns, Ns))
+    pairs = [(0, 1, -1.0), (2, 3, -0.8), (4, 5, -0.6), (0, 2, -0.2), (1, 3, -0.2), (2, 4, -0.1), (3, 5, -0.1)]
+    for a, b, t in pairs:
+        H[0, a, b] = H[0, b, a] = t
+    for k, e in enumerate([0.3, 0.3, 0.0, 0.0, -0.2, -0.2]):
+        H[0, k, k] = e
+    return H
 
 
 def _two_orb():
@@ -261,6 +280,23 @@ def test_stored_path_route_e0(ed, orc):
         w0 = np.linalg.eigvalsh(orc.hamiltonian_csr(m, *sec).toarray().real)[0]
         assert abs(e[sparse] - w0) < 1e-10 * abs(w0)
     assert abs((e[True] - e[False]) - (-(xmu + U / 2) * N)) < 1e-9
+
+
+def test_kanamori_ground_state_vs_dense(ed, orc):
+    """Ground state of a three-orbital model with spin exchange and pair
+    hopping (the non-local block on the single-pass kernels): E0 and the
+    double occupancies against dense eigh of the oracle's Hamiltonian,
+    FP64 and mixed precision."""
+    kw = dict(Nsites=[2, 2, 2], Ust=2.0, Jh=0.6, Jx=0.6, Jp=0.5, xmu=0.2)
+    m = _use(ed, orc, _three_orb(), [4.0, 3.0, 2.5], **kw)
+    isec = ed.sector_index(3, 3)
+    w, V = np.linalg.eigh(orc.hamiltonian_csr(m, 3, 3).toarray().real)
+    r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    assert abs(r["e0"] - w[0]) < 1e-10 * abs(w[0])
+    ref = orc.observables(6, 3, 3, V[:, 0])
+    np.testing.assert_allclose(ed.observables()["docc"], ref["docc"], atol=1e-9)
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=300, refine_tol=1e-13)
+    assert abs(rmx["e0"] - w[0]) < 1e-10 * abs(w[0])
 
 
 def test_rect3x4_e0_vs_oracle(ed, orc):
