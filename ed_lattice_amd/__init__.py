@@ -6,4 +6,4 @@ builds the hopping matrices of the benchmark lattices.
 """
 from . import lattice  # noqa: F401
 from ._lib import LIB_PATH, load  # noqa: F401
-from .solver import EDError, EDLattice, eval_gf, lanczos_poles, sector_index  # noqa: F401
+from .solver import EDError, EDLattice, eval_gf, lanczos_poles, optical_conductivity, sector_index  # noqa: F401

@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libedlat_hip.so")
 
 EDL_FP64, EDL_FP32, EDL_MIXED = 0, 1, 2
-EDL_OP_C, EDL_OP_CDG, EDL_OP_SZ = 1, 2, 3
+EDL_OP_C, EDL_OP_CDG, EDL_OP_SZ, EDL_OP_J = 1, 2, 3, 4
 
 
 class SeedDesc(ctypes.Structure):

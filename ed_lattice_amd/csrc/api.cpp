@@ -1157,10 +1157,41 @@ extern "C" int edl_get_state(edl_ctx* ctx, double* vec) {
 }
 
 // target sector of a seed (getCsector / getCDGsector, SETUP.f90:265-310)
+// Terms of the current of orbital iorb (1-based), OC_ELECTRONS.f90:139-197:
+// for each site i of the orbital and its ring successor i+1 (the last site
+// wraps to the first), both spins, H(s,i,i+1) c+_i c_i+1 where that element
+// is non-zero and -conjg(H(s,i+1,i)) c+_i+1 c_i where that one is; the spin-2
+// terms read Hij(Nspin,:,:).
+static CurrentSeed current_terms(const Model& M, int iorb) {
+  auto pack = [&](int isite, int k) { return (k == 0) ? isite : isite + k * M.Nsites[k - 1]; };  // AUX_FUNX.f90:42-53
+  auto h = [&](int spin, int io, int jo) {  // Hij(spin, io, jo), 1-based
+    return M.Hij[(size_t)(spin - 1) + (size_t)M.Nspin * ((size_t)(io - 1) + (size_t)M.Ns * (size_t)(jo - 1))];
+  };
+  CurrentSeed J{};
+  const int k = iorb - 1, ns = M.Nsites[k];
+  for (int isite = 1; isite <= ns; ++isite) {
+    const int jsite = (isite == ns) ? 1 : isite + 1;
+    const int a = pack(isite, k), b = pack(jsite, k);
+    for (int spin = 1; spin <= 2; ++spin) {
+      const int hs = (spin == 1) ? 1 : M.Nspin;
+      const double fwd = h(hs, a, b), bwd = h(hs, b, a);
+      if (fwd != 0.0) {
+        EDL_CHECK(J.n < kMaxCurrentTerms, "current seed: too many terms");
+        J.t[J.n++] = CurrentTerm{(uint8_t)(a - 1), (uint8_t)(b - 1), (uint8_t)spin, fwd};
+      }
+      if (bwd != 0.0) {
+        EDL_CHECK(J.n < kMaxCurrentTerms, "current seed: too many terms");
+        J.t[J.n++] = CurrentTerm{(uint8_t)(b - 1), (uint8_t)(a - 1), (uint8_t)spin, -bwd};
+      }
+    }
+  }
+  return J;
+}
+
 static int target_sector(const Ctx& c, int isector, const SeedSpec& s) {
   const int Ns = c.model.Ns;
   const int ndw = (isector - 1) % (Ns + 1), nup = ((isector - 1) / (Ns + 1)) % (Ns + 1);
-  if (s.op == EDL_OP_SZ) return isector;
+  if (s.op == EDL_OP_SZ || s.op == EDL_OP_J) return isector;
   const int d = (s.op == EDL_OP_CDG) ? +1 : -1;
   int u = nup, w = ndw;
   if (s.ispin == 1) u += d; else w += d;
@@ -1339,7 +1370,10 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       EDL_HIP(hipMemcpyAsync(tmp.p, host_seed, sizeof(double) * J.dim, hipMemcpyHostToDevice, st));
     } else {
       const Sector& I = get_sector(c, c.gs_sector);
-      launch_seed(seeds[s], c.d_gs, I.sp[0].dim, J, tmp.as<double>(), st);
+      if (seeds[s].op == EDL_OP_J)
+        launch_current_seed(current_terms(c.model, seeds[s].io), c.d_gs, J, tmp.as<double>(), st);
+      else
+        launch_seed(seeds[s], c.d_gs, I.sp[0].dim, J, tmp.as<double>(), st);
     }
     device_dot_to(tmp.as<double>(), tmp.as<double>(), J.dim, part.as<double>(), d_n2 + k, st);
     const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
@@ -1488,10 +1522,15 @@ extern "C" int edl_tridiag_batched(edl_ctx* ctx, int nseeds, const edl_seed_desc
   std::vector<int> targets(nseeds, 0), active;
   for (int s = 0; s < nseeds; ++s) {
     sp[s] = SeedSpec{seeds[s].op, seeds[s].ispin, seeds[s].io, seeds[s].jo};
-    EDL_CHECK(sp[s].op >= EDL_OP_C && sp[s].op <= EDL_OP_SZ, "edl_tridiag_batched: bad op");
-    EDL_CHECK(sp[s].op == EDL_OP_SZ || sp[s].ispin == 1 || sp[s].ispin == 2, "edl_tridiag_batched: bad ispin");
-    EDL_CHECK(sp[s].io >= 1 && sp[s].io <= Ns && sp[s].jo >= 0 && sp[s].jo <= Ns,
-              "edl_tridiag_batched: site index out of range");
+    EDL_CHECK(sp[s].op >= EDL_OP_C && sp[s].op <= EDL_OP_J, "edl_tridiag_batched: bad op");
+    if (sp[s].op == EDL_OP_J) {
+      EDL_CHECK(sp[s].io >= 1 && sp[s].io <= c.model.Norb && sp[s].jo == 0,
+                "edl_tridiag_batched: current seed needs an orbital index 1..Norb and jo = 0");
+    } else {
+      EDL_CHECK(sp[s].op == EDL_OP_SZ || sp[s].ispin == 1 || sp[s].ispin == 2, "edl_tridiag_batched: bad ispin");
+      EDL_CHECK(sp[s].io >= 1 && sp[s].io <= Ns && sp[s].jo >= 0 && sp[s].jo <= Ns,
+                "edl_tridiag_batched: site index out of range");
+    }
     targets[s] = target_sector(c, c.gs_sector, sp[s]);
     if (isector_out) isector_out[s] = targets[s];
     for (int j = 0; j < nlanc; ++j) alpha[(size_t)s * nlanc + j] = beta[(size_t)s * nlanc + j] = 0.0;

@@ -58,6 +58,40 @@ __global__ void k_seed(const double* __restrict__ gs, int gs_dimup, const int32_
   }
 }
 
+// Current seed in gather form: term coef * c+_a c_b reaches target j from
+// the source with a (of that spin) moved back to b; the reference's signs
+// are those of c_b on the source and then c+_a (AUX_FUNX.f90:76-102).
+__global__ void k_seed_current(const double* __restrict__ gs, const int32_t* __restrict__ map_up,
+                               const int32_t* __restrict__ map_dw, int dimup, int64_t dim, CurrentSeed J,
+                               double* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += (int64_t)gridDim.x * blockDim.x) {
+    const int jup = (int)(j % dimup), jdw = (int)(j / dimup);
+    const uint32_t tu = (uint32_t)map_up[jup], td = (uint32_t)map_dw[jdw];
+    double v = 0.0;
+    for (int k = 0; k < J.n; ++k) {
+      const CurrentTerm T = J.t[k];
+      const uint32_t t = (T.spin == 1) ? tu : td;
+      const uint32_t ba = 1u << T.a, bb = 1u << T.b;
+      if (T.a == T.b || !(t & ba) || (t & bb)) continue;
+      const uint32_t src = (t & ~ba) | bb;
+      const uint32_t k1 = src & ~bb;
+      const double sg = ((__popc(src & (bb - 1u)) + __popc(k1 & (ba - 1u))) & 1) ? -1.0 : 1.0;
+      const int32_t r = rank_of(src);
+      const size_t o = (T.spin == 1) ? (size_t)jdw * dimup + (size_t)r : (size_t)r * dimup + (size_t)jup;
+      v += T.coef * sg * gs[o];
+    }
+    out[j] = v;
+  }
+}
+
+void launch_current_seed(const CurrentSeed& J, const double* gs, const Sector& s, double* out, hipStream_t st) {
+  int64_t g = (s.dim + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  hipLaunchKernelGGL(k_seed_current, dim3((int)g), dim3(kBlock), 0, st, gs, s.sp[0].map, s.sp[1].map, s.sp[0].dim,
+                     s.dim, J, out);
+  EDL_HIP(hipGetLastError());
+}
+
 // Per-site double occupancy sum_i |gs(i)|^2 n_up(p) n_dw(p), and the row /
 // column sums of |gs|^2 from which the densities follow.
 template <int NS>

@@ -10,6 +10,23 @@ struct SeedSpec {
 };
 
 void launch_seed(const SeedSpec& s, const double* gs, int gs_dimup, const Sector& target, double* out, hipStream_t st);
+
+// Current-operator seed of the optical conductivity of one orbital
+// (lanc_ed_build_oc, OC_ELECTRONS.f90:103-213): terms coef * c+_a c_b of one
+// spin along the ring order of the orbital's sites, in the ground-state
+// sector.  The reference multiplies the seed by i (OC_ELECTRONS.f90:205);
+// with a real H that factor leaves norm2, alpha and beta unchanged, so the
+// seed is built real.
+struct CurrentTerm {
+  uint8_t a, b, spin;  // 0-based sites, spin 1 up / 2 down
+  double coef;
+};
+constexpr int kMaxCurrentTerms = 128;
+struct CurrentSeed {
+  int n;
+  CurrentTerm t[kMaxCurrentTerms];
+};
+void launch_current_seed(const CurrentSeed& J, const double* gs, const Sector& s, double* out, hipStream_t st);
 void device_observables(Ctx& c, const Sector& s, const double* gs, double* dens_up, double* dens_dw, double* docc);
 
 }  // namespace edl

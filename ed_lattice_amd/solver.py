@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EDL_FP32, EDL_FP64, EDL_MIXED, EDL_OP_C, EDL_OP_CDG, EDL_OP_SZ, SeedDesc
+from ._lib import EDL_FP32, EDL_FP64, EDL_MIXED, EDL_OP_C, EDL_OP_CDG, EDL_OP_J, EDL_OP_SZ, SeedDesc
 
 PREC = {"fp64": EDL_FP64, "fp32": EDL_FP32, "mixed": EDL_MIXED}
 
@@ -57,6 +57,22 @@ def lanczos_poles(alpha, beta, norm2, Ei, isign):
     if lib.edl_lanczos_poles(n, _dp(a), _dp(b), float(norm2), float(Ei), int(isign), _dp(p), _dp(w)):
         raise EDError(lib.edl_last_error(None).decode())
     return p, w
+
+
+def optical_conductivity(poles, weights, vr, eps, cutoff=1e-9, pesoBZ=1.0):
+    """lanc_ed_eval_oc (OC_ELECTRONS.f90:285-315) for one state's current
+    channel: Drude weight sum peso/dE and OptCond_w(vr) = sum peso/dE *
+    eps / ((vr - dE)^2 + eps^2), peso = pesoBZ * weight, skipping poles with
+    |dE| < cutoff or |peso/dE| < cutoff (and non-finite ratios)."""
+    p = np.asarray(poles, float)
+    w = pesoBZ * np.asarray(weights, float)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = w / p
+    keep = np.isfinite(r) & (np.abs(p) >= cutoff) & (np.abs(r) >= cutoff)
+    r, p = r[keep], p[keep]
+    vr = np.asarray(vr, float)
+    sigma = (r[None, :] * eps / ((vr[:, None] - p[None, :]) ** 2 + eps ** 2)).sum(1)
+    return float(r.sum()), sigma
 
 
 def eval_gf(channels, wm, wr, eps, pesoBZ=1.0):
@@ -265,8 +281,10 @@ class EDLattice:
         return a, b, n2.value
 
     def tridiag_batched(self, seeds, nlanc, prec="fp64", group=0):
-        """seeds: list of (op, ispin, io, jo) with op in {'c','cdg','sz'}."""
-        opmap = {"c": EDL_OP_C, "cdg": EDL_OP_CDG, "sz": EDL_OP_SZ}
+        """seeds: list of (op, ispin, io, jo) with op in {'c','cdg','sz','j'};
+        'j' is the optical-conductivity current of orbital io (ispin and jo
+        unused, give 0)."""
+        opmap = {"c": EDL_OP_C, "cdg": EDL_OP_CDG, "sz": EDL_OP_SZ, "j": EDL_OP_J}
         n = len(seeds)
         arr = (SeedDesc * max(n, 1))()
         for k, (op, ispin, io, jo) in enumerate(seeds):

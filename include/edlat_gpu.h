@@ -36,7 +36,7 @@ typedef struct edl_ctx edl_ctx;
 /* arithmetic of vectors and H x v */
 enum { EDL_FP64 = 0, EDL_FP32 = 1, EDL_MIXED = 2 };
 /* seed operators */
-enum { EDL_OP_C = 1, EDL_OP_CDG = 2, EDL_OP_SZ = 3 };
+enum { EDL_OP_C = 1, EDL_OP_CDG = 2, EDL_OP_SZ = 3, EDL_OP_J = 4 };
 
 /* Context on HIP device `device`.  Replaces the module-global state of
  * ED_HAMILTONIAN_COMMON (HAMILTONIAN_COMMON.f90:13-40). */
@@ -119,7 +119,10 @@ int edl_tridiag(edl_ctx* ctx, int isector, int prec, const double* seed, int nla
  * {op, ispin, io, jo}: op EDL_OP_C / EDL_OP_CDG acting on spin ispin in the
  * N-1 / N+1 sector (getCsector / getCDGsector, SETUP.f90:265-310), or
  * EDL_OP_SZ in the same sector; jo > 0 adds the jo term (the mixed seeds of
- * GF_ELECTRONS.f90:303-405, CHI_SPIN_ELECTRONS.f90:244-293).  Each chain
+ * GF_ELECTRONS.f90:303-405, CHI_SPIN_ELECTRONS.f90:244-293).  EDL_OP_J is
+ * the optical-conductivity current of orbital io (1-based; ispin and jo
+ * unused), lanc_ed_build_oc (OC_ELECTRONS.f90:103-213), in the same sector;
+ * its poles are E_j - Ei (edl_lanczos_poles with isign = +1).  Each chain
  * runs Nlanc = min(Dim_target, nlanc) steps (SECTOR.f90:112).  Replaces the
  * sequential per-seed loops of GF_ELECTRONS.f90:202-405 and
  * CHI_SPIN_ELECTRONS.f90:189-293.  Outputs: alpha, beta [nseeds*nlanc]

@@ -369,3 +369,84 @@ def dense_hamiltonian(model, nup, ndw):
     dws = [s for s in range(2 ** Ns) if bin(s).count("1") == ndw]
     idx = np.array([u + (2 ** Ns) * d for d in dws for u in ups])
     return H[np.ix_(idx, idx)]
+
+
+# ------------------------------------------------------ optical conductivity
+def seed_current(model, nup, ndw, iorb, state):
+    """lanc_ed_build_oc (OC_ELECTRONS.f90:139-197) in its scatter form on the
+    (nup, ndw) sector, before the global factor i of OC_ELECTRONS.f90:205:
+    for each basis state, each site isite of orbital iorb and its ring
+    successor jsite (the last wraps to the first), both spins: the hop
+    jsite -> isite with +Hij(s,io,jo) and isite -> jsite with
+    -conjg(Hij(s,jsite,isite)), each where that Hij element is non-zero, with
+    the c / cdg signs and binary_search of AUX_FUNX.f90.  Python loops:
+    small sectors only."""
+    Ns, Nsp = model.Ns, model.Nspin
+    mu, md = build_map(Ns, nup), build_map(Ns, ndw)
+    du = len(mu)
+    st = np.asarray(state, dtype=np.complex128)
+    out = np.zeros(du * len(md), np.complex128)
+    ns = int(model.Nsites[iorb - 1])
+
+    def pack(isite):  # pack_indices, AUX_FUNX.f90:42-53 (1-based)
+        return isite if iorb == 1 else isite + (iorb - 1) * int(model.Nsites[iorb - 2])
+
+    for i in range(len(out)):
+        if st[i] == 0:
+            continue
+        iup, idw = i % du, i // du
+        mup, mdw = int(mu[iup]), int(md[idw])
+        for isite in range(1, ns + 1):
+            jsite = isite + 1 if isite < ns else 1
+            for io, jo, sgn in ((pack(isite), pack(jsite), 1.0), (pack(jsite), pack(isite), -1.0)):
+                for spin, m, hs in ((1, mup, 0), (2, mdw, Nsp - 1)):
+                    h = model.H[hs, io - 1, jo - 1]
+                    if h == 0 or not (m >> (jo - 1)) & 1 or (m >> (io - 1)) & 1:
+                        continue
+                    k1, s1 = c_op(jo, m)
+                    k2, s2 = cdg_op(io, k1)
+                    coef = h if sgn > 0 else -np.conj(h)
+                    if spin == 1:
+                        j = (binary_search(mu, k2) - 1) + idw * du
+                    else:
+                        j = iup + (binary_search(md, k2) - 1) * du
+                    out[j] += coef * s1 * s2 * st[i]
+    return out
+
+
+def dense_current(model, nup, ndw, iorb):
+    """The same current operator built independently from Jordan-Wigner
+    matrices (sum over the orbital's ring bonds and both spins of
+    H(s,a,b) c+_a c_b - H(s,b,a) c+_b c_a), projected on the sector in the
+    reference order.  Full-Fock dense matrices (4^Ns squared): Ns <= 6."""
+    assert model.Ns <= 6, "dense_current builds 4^Ns x 4^Ns matrices"
+    Ns, Nsp = model.Ns, model.Nspin
+    ca = _jw_annihilators(Ns)
+    cu = [np.kron(np.eye(2 ** Ns), c) for c in ca]
+    cd = [np.kron(c, np.eye(2 ** Ns)) for c in ca]
+    ns = int(model.Nsites[iorb - 1])
+
+    def pack(isite):
+        return isite if iorb == 1 else isite + (iorb - 1) * int(model.Nsites[iorb - 2])
+
+    K = np.zeros((4 ** Ns, 4 ** Ns))
+    for isite in range(1, ns + 1):
+        jsite = isite + 1 if isite < ns else 1
+        a, b = pack(isite) - 1, pack(jsite) - 1
+        if a == b:
+            continue
+        for ops, hs in ((cu, 0), (cd, Nsp - 1)):
+            K += model.H[hs, a, b].real * (ops[a].T @ ops[b]) - model.H[hs, b, a].real * (ops[b].T @ ops[a])
+    ups = [s for s in range(2 ** Ns) if bin(s).count("1") == nup]
+    dws = [s for s in range(2 ** Ns) if bin(s).count("1") == ndw]
+    idx = np.array([u + (2 ** Ns) * d for d in dws for u in ups])
+    return K[np.ix_(idx, idx)]
+
+
+def oc_lehmann(Hdense, seed, E0):
+    """Exact current spectrum of a state: poles E_n - E0 and weights
+    |<n|seed>|^2 over the eigenpairs of Hdense, seed = K|gs> (the function
+    the continued fraction of add_to_lanczos_oc approximates)."""
+    w, V = np.linalg.eigh(Hdense)
+    amp = V.T @ np.asarray(seed, float)
+    return w - E0, amp ** 2

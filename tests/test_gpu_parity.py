@@ -487,3 +487,38 @@ def test_chi_batched_vs_oracle(ed, orc, batched_kernels):
         _assert_leading(raw["alpha"][k], raw["beta"][k], ao, bo, 10, 1e-9)
         _assert_converged_poles_match(_ritz(raw["alpha"][k], raw["beta"][k], raw["norm2"][k], r["e0"], 1),
                                       _ritz(ao, bo, s @ s, r["e0"], 1), 1e-8)
+
+
+def test_optical_conductivity_ring8_vs_oracle(ed, orc, batched_kernels):
+    """Current seed of lanc_ed_build_oc on the ring8 ground state: norm2 and
+    the first alpha/beta against the oracle's scatter seed and Lanczos, and
+    the broadened optical conductivity and Drude weight of lanc_ed_eval_oc
+    against the exact Lehmann sum of dense diagonalisation."""
+    from ed_lattice_amd import optical_conductivity
+
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    r = ed.lanczos_eigh(isec, "fp64", niter=200, want_vector=True)
+    gs, e0 = r["gs"], r["e0"]
+    res = ed.tridiag_batched([("j", 0, 1, 0)], 200, "fp64")
+    assert res["isector"][0] == isec
+    a, b, n2 = res["alpha"][0], res["beta"][0], res["norm2"][0]
+    seed = orc.seed_current(m, 4, 4, 1, gs).real
+    assert abs(n2 - seed @ seed) < 1e-10 * (seed @ seed)
+    # the seed lies in a small symmetry subspace: its Ritz values converge
+    # within a few steps, after which two runs without reorthogonalisation
+    # drift apart in rounding; the recurrence is compared over its first
+    # steps and the spectrum through sigma(w) and the Drude weight below
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    ao, bo, _ = orc.lanczos_tridiag(Hc, seed, 10)
+    np.testing.assert_allclose(a[:10], ao, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(b[1:10], bo[1:], rtol=1e-9, atol=1e-9)
+    p, w = ed.lanczos_poles(a[:res["nsteps"][0]], b[:res["nsteps"][0]], n2, e0, +1)
+    Hd = Hc.toarray().real
+    pl, wl = orc.oc_lehmann(Hd, seed, e0)
+    vr = np.linspace(0.0, 12.0, 400)
+    dg, sg = optical_conductivity(p, w, vr, 0.2)
+    dl, sl = optical_conductivity(pl, wl, vr, 0.2)
+    print(f"OC ring8: norm2 {n2:.10f} Drude {dg:.10f} exact {dl:.10f}", flush=True)
+    assert abs(dg - dl) < 1e-7 * abs(dl)
+    np.testing.assert_allclose(sg, sl, atol=1e-7 * np.max(np.abs(sl)))

@@ -169,3 +169,28 @@ def test_gf_sum_rules_and_chi(orc):
     assert abs(sz @ sz - expect) < 1e-10
     # half filling + particle-hole symmetry of the ring: n = 1/2 per spin
     np.testing.assert_allclose(obs["dens_up"], 0.5, atol=1e-8)
+
+
+@pytest.mark.parametrize("case", ["ring6", "ring5_open", "two_orbital"])
+def test_current_seed_equals_jordan_wigner(orc, case):
+    """The scatter restatement of lanc_ed_build_oc equals the Jordan-Wigner
+    current operator applied to random states (both spins, wrap-around bond,
+    missing bonds skipped, a second orbital)."""
+    rng = np.random.default_rng(7)
+    if case == "ring6":
+        m, sec, iorb = orc.Model(lattice.ring(6), [3.0]), (3, 2), 1
+    elif case == "ring5_open":
+        H = lattice.ring(5)
+        H[0, 0, 4] = H[0, 4, 0] = 0.0  # open chain: the wrap-around bond has no hopping
+        m, sec, iorb = orc.Model(H, [2.0]), (2, 3), 1
+    else:  # two orbitals of three sites each (a two-site ring carries no current)
+        H = np.zeros((1, 6, 6))
+        for a, b, t in [(0, 1, -1.0), (1, 2, -0.9), (2, 0, -0.8), (3, 4, -0.6), (4, 5, -0.7), (5, 3, -0.5),
+                        (0, 3, -0.3), (1, 4, -0.3)]:
+            H[0, a, b] = H[0, b, a] = t
+        m, sec, iorb = orc.Model(H, [3.0, 2.0], Nsites=[3, 3]), (3, 2), 2
+    K = orc.dense_current(m, *sec, iorb)
+    assert np.allclose(K, -K.T) and np.abs(K).max() > 0.1
+    for _ in range(2):
+        v = rng.standard_normal(K.shape[0])
+        np.testing.assert_allclose(orc.seed_current(m, *sec, iorb, v).real, K @ v, atol=1e-12)
