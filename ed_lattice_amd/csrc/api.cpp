@@ -1596,6 +1596,17 @@ extern "C" int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, d
   API_END
 }
 
+extern "C" int edl_local_energy(edl_ctx* ctx, double* e) {
+  API_BEGIN
+  Ctx& c = ctx->c;
+  EDL_CHECK(c.d_gs, "edl_local_energy: no ground state installed");
+  EDL_CHECK(e, "edl_local_energy: null output");
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = get_sector(c, c.gs_sector);
+  device_energy(c, S, c.d_gs, e);
+  API_END
+}
+
 extern "C" int edl_last_timing(edl_ctx* ctx, double* ms, int64_t* hxv_count) {
   API_BEGIN
   if (ms) *ms = ctx->c.last_ms;

@@ -64,35 +64,43 @@ static __device__ __forceinline__ double fsign(uint32_t s, uint32_t pos) {
 // or c+_io c_jo (up) c+_io c_jo (down) with Jp (pair hopping), for every
 // ordered same-site orbital pair; the block is symmetric, so row i gathers
 // with the coefficient of the move that starts at i.
+// cx, cp: the coefficients of the exchange and pair-hopping moves (Jx, Jp
+// for the operator; 1, 0 and 0, 1 for the expectation values Dse, Dph).
 template <typename T>
-static __device__ __forceinline__ T nonlocal_row(const InterParams& I, uint32_t su, uint32_t sd,
-                                                 const T* __restrict__ q, size_t ld) {
+static __device__ __forceinline__ double nonlocal_gather(const InterParams& I, uint32_t su, uint32_t sd,
+                                                         const T* __restrict__ q, size_t ld, double cx, double cp) {
   double acc = 0.0;
   for (int p = 0; p < I.npairs; ++p)
     for (int o = 0; o < 2; ++o) {
       const uint32_t io = o ? I.pj[p] : I.pi[p], jo = o ? I.pi[p] : I.pj[p];
       const uint32_t bio = 1u << io, bjo = 1u << jo;
       const bool up_j = su & bjo, up_i = su & bio, dw_j = sd & bjo, dw_i = sd & bio;
-      if (I.Jx != 0.0 && up_j && !up_i && dw_i && !dw_j) {
+      if (cx != 0.0 && up_j && !up_i && dw_i && !dw_j) {
         const uint32_t k1 = sd ^ bio;                    // c_io (down)
         const double s12 = fsign(sd, io) * fsign(k1, jo);  // then cdg_jo
         const uint32_t k2 = k1 | bjo;
         const uint32_t k3 = su ^ bjo;                    // c_jo (up)
         const double s34 = fsign(su, jo) * fsign(k3, io);  // then cdg_io
         const uint32_t k4 = k3 | bio;
-        acc += I.Jx * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
+        acc += cx * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
       }
-      if (I.Jp != 0.0 && up_j && dw_j && !up_i && !dw_i) {
+      if (cp != 0.0 && up_j && dw_j && !up_i && !dw_i) {
         const uint32_t k1 = sd ^ bjo;                    // c_jo (down)
         const double s12 = fsign(sd, jo) * fsign(k1, io);  // then cdg_io
         const uint32_t k2 = k1 | bio;
         const uint32_t k3 = su ^ bjo;                    // c_jo (up)
         const double s34 = fsign(su, jo) * fsign(k3, io);  // then cdg_io
         const uint32_t k4 = k3 | bio;
-        acc += I.Jp * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
+        acc += cp * s12 * s34 * (double)q[(size_t)rank_of(k2) * ld + rank_of(k4)];
       }
     }
-  return (T)acc;
+  return acc;
+}
+
+template <typename T>
+static __device__ __forceinline__ T nonlocal_row(const InterParams& I, uint32_t su, uint32_t sd,
+                                                 const T* __restrict__ q, size_t ld) {
+  return (T)nonlocal_gather<T>(I, su, sd, q, ld, I.Jx, I.Jp);
 }
 
 // Tile walk of one block: XCD-synchronous strided order (see header).

@@ -200,4 +200,107 @@ void device_observables(Ctx& c, const Sector& s, const double* gs, double* dens_
   }
 }
 
+// ------------------------------------------------------------- energies
+struct EnergyParams {
+  InterParams I;                 // U masks, same-site pairs, Ust, Jx, Jp
+  double hd_up[kMaxNs], hd_dw[kMaxNs];  // Re diag Hij(1,:,:), Hij(Nspin,:,:)
+  double u_site[kMaxNs];         // Uloc of the orbital each site belongs to
+  double Jh;
+  int Ns, hfmode;
+};
+
+// Per-element terms of lanc_energy (ENERGY.f90:124-305), block partials
+// part[8 * block + k] of: <K>, U/Ust/Jh/Jx/Jp energy, <Eloc>, <Ehf>, Dust,
+// Dund, Dse, Dph.  The kinetic term gathers each row's hops (the hopping
+// block is symmetric), the exchange and pair-hopping terms the moves that
+// start at the element, as the reference's sums over i of
+// state(i) * conjg(state(j)).
+__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ gs, const int32_t* __restrict__ map_up,
+                                                const int32_t* __restrict__ map_dw, int dimup, int dimdw,
+                                                int64_t dim, const int32_t* __restrict__ nbr_up,
+                                                const double* __restrict__ cu, const int32_t* __restrict__ cnt_up,
+                                                const int32_t* __restrict__ nbr_dw, const double* __restrict__ cd,
+                                                const int32_t* __restrict__ cnt_dw, EnergyParams P,
+                                                double* __restrict__ part) {
+  __shared__ double sh[16];
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += (int64_t)gridDim.x * blockDim.x) {
+    const int iup = (int)(j % dimup), idw = (int)(j / dimup);
+    const uint32_t su = (uint32_t)map_up[iup], sd = (uint32_t)map_dw[idw];
+    const double v = gs[j], w = v * v;
+    double hk = 0.0;
+    for (int k = 0; k < cnt_up[iup]; ++k)
+      hk += cu[(size_t)k * dimup + iup] * gs[(size_t)idw * dimup + nbr_up[(size_t)k * dimup + iup]];
+    for (int k = 0; k < cnt_dw[idw]; ++k)
+      hk += cd[(size_t)k * dimdw + idw] * gs[(size_t)nbr_dw[(size_t)k * dimdw + idw] * dimup + iup];
+    a[0] += v * hk;
+    double ek = 0.0, eh = 0.0;
+    for (int io = 0; io < P.Ns; ++io) {
+      const double nu = (double)((su >> io) & 1u), nd = (double)((sd >> io) & 1u);
+      ek += P.hd_up[io] * nu + P.hd_dw[io] * nd;
+      eh -= 0.5 * P.u_site[io] * (nu + nd);
+    }
+    double ep = inter_term(P.I, su, sd), dust = 0.0, dund = 0.0;
+    for (int p = 0; p < P.I.npairs; ++p) {
+      const uint32_t x = P.I.pi[p], y = P.I.pj[p];
+      const double ux = (su >> x) & 1u, uy = (su >> y) & 1u, dx = (sd >> x) & 1u, dy = (sd >> y) & 1u;
+      dust += ux * dy + uy * dx;
+      const double same = ux * uy + dx * dy;
+      dund += same;
+      ep += (P.I.Ust - P.Jh) * same;
+      eh -= (0.5 * P.I.Ust + 0.5 * (P.I.Ust - P.Jh)) * (ux + dx + uy + dy);
+    }
+    a[1] += w * ep;
+    a[2] += w * ek;
+    if (P.hfmode) a[3] += w * eh;
+    a[4] += w * dust;
+    a[5] += w * dund;
+    if (P.I.npairs > 0) {
+      a[6] += v * nonlocal_gather<double>(P.I, su, sd, gs, (size_t)dimup, 1.0, 0.0);
+      a[7] += v * nonlocal_gather<double>(P.I, su, sd, gs, (size_t)dimup, 0.0, 1.0);
+    }
+  }
+  for (int k = 0; k < 8; ++k) {
+    const double s = block_sum(a[k], sh);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8 + k] = s;
+    __syncthreads();
+  }
+}
+
+void device_energy(Ctx& c, const Sector& s, const double* gs, double* e11) {
+  const Model& M = c.model;
+  EnergyParams P{};
+  P.I = M.inter;
+  P.Ns = M.Ns;
+  P.Jh = M.Jh;
+  P.hfmode = M.hfmode;
+  for (int io = 0; io < M.Ns; ++io) {
+    P.hd_up[io] = M.Hij[(size_t)M.Nspin * (io + (size_t)M.Ns * io)];
+    P.hd_dw[io] = M.Hij[(size_t)(M.Nspin - 1) + (size_t)M.Nspin * (io + (size_t)M.Ns * io)];
+  }
+  for (int k = 0; k < M.Norb; ++k)
+    for (int isite = 1; isite <= M.Nsites[k]; ++isite) {
+      const int io = (k == 0) ? isite : isite + k * M.Nsites[k - 1];  // pack_indices
+      if (io >= 1 && io <= M.Ns) P.u_site[io - 1] = M.Uloc[k];
+    }
+  const int nb = (int)std::min<int64_t>(1024, (s.dim + 255) / 256);
+  double* part = nullptr;
+  EDL_HIP(hipMalloc(&part, sizeof(double) * 8 * (size_t)nb));
+  hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, c.stream, gs, s.sp[0].map, s.sp[1].map, s.sp[0].dim,
+                     s.sp[1].dim, s.dim, s.sp[0].nbr, s.sp[0].coef64, s.sp[0].cnt, s.sp[1].nbr, s.sp[1].coef64,
+                     s.sp[1].cnt, P, part);
+  EDL_HIP(hipGetLastError());
+  std::vector<double> h((size_t)nb * 8);
+  EDL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+  EDL_HIP(hipStreamSynchronize(c.stream));
+  EDL_HIP(hipFree(part));
+  double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 8; ++k) t[k] += h[(size_t)b * 8 + k];
+  const double ekin = t[0], eknot = t[2], ehf = t[3], dse = t[6], dph = t[7];
+  const double epot = t[1] + M.inter.Jx * dse + M.inter.Jp * dph + ehf;  // ed_Epot + ed_Ehartree
+  const double out[11] = {ekin, epot, epot - ehf, eknot, ehf, t[4], t[5], dse, dph, 0.0, 0.0};
+  for (int k = 0; k < 11; ++k) e11[k] = out[k];
+}
+
 }  // namespace edl

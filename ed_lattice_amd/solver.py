@@ -269,6 +269,16 @@ class EDLattice:
         self._ck(self.lib.edl_observables(self.h, _dp(u), _dp(d), _dp(o)))
         return dict(dens_up=u, dens_dw=d, dens=u + d, docc=o)
 
+    ENERGY_KEYS = ("Ekin", "Epot", "Eint", "Eknot", "Ehartree", "Dust", "Dund", "Dse", "Dph", "Dkxy", "Dkz")
+
+    def local_energy(self):
+        """lanc_energy (ENERGY.f90:70-353) of the installed ground state, in
+        the column order of energy_last.ed: <K>, <Hi>, <V> = <Hi> - <Ehf>,
+        <Eloc>, <Ehf>, Dust, Dund, Dse, Dph, Dkxy, Dkz."""
+        e = np.zeros(11)
+        self._ck(self.lib.edl_local_energy(self.h, _dp(e)))
+        return dict(zip(self.ENERGY_KEYS, e))
+
     # ------------------------------------------------------------ seeds
     def tridiag_Hv_sector(self, isector, vvinit, nlanc, prec="fp64"):
         v = np.ascontiguousarray(vvinit, dtype=np.float64)

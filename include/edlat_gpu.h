@@ -148,6 +148,12 @@ int edl_lanczos_poles(int nlanc, const double* alpha, const double* beta, double
  * dens_up, dens_dw, docc per site (weight 1). */
 int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, double* docc);
 
+/* lanc_energy (ENERGY.f90:70-353) of the installed ground state (weight 1):
+ * e[11] in the column order of energy_last.ed (ENERGY.f90:654-657): <K>,
+ * <Hi>, <V> = <Hi> - <Ehf>, <Eloc>, <Ehf>, Dust, Dund, Dse, Dph, Dkxy, Dkz
+ * (Dkxy, Dkz: 0, as the reference's Lanczos path leaves them). */
+int edl_local_energy(edl_ctx* ctx, double* e);
+
 /* Device time (HIP events on the context stream) of the last edl_hxv,
  * edl_lanczos_gs, edl_tridiag or edl_tridiag_batched call, and the number
  * of H x v applications it performed. */

@@ -522,3 +522,59 @@ def test_optical_conductivity_ring8_vs_oracle(ed, orc, batched_kernels):
     print(f"OC ring8: norm2 {n2:.10f} Drude {dg:.10f} exact {dl:.10f}", flush=True)
     assert abs(dg - dl) < 1e-7 * abs(dl)
     np.testing.assert_allclose(sg, sl, atol=1e-7 * np.max(np.abs(sl)))
+
+
+@pytest.mark.parametrize("cfg", ["ring8_mu", "rect3x4", "three_orbital", "three_orbital_filling"])
+def test_local_energy_vs_oracle(ed, orc, cfg):
+    """lanc_energy of the installed ground state: E0 = <K> + <Eloc> + <Hi> -
+    xmu*N at ED_FILLING = 0 (<Hi> holds the Hartree terms that the diagonal
+    carries then), <K> + <Eloc> + <Hi> - <Ehf> otherwise; <K>, Dse and Dph as
+    expectation values of the oracle's operators built from only that term;
+    <Eloc> from the basis; U*docc for one orbital."""
+    if cfg == "ring8_mu":
+        H, U, kw, sec = lattice.ring(8), [4.0], dict(xmu=0.3), (4, 4)
+    elif cfg == "rect3x4":
+        H, U, kw, sec = lattice.square(3, 4), [6.0], {}, (6, 6)
+    else:
+        H, U, sec = _three_orb(), [4.0, 3.0, 2.5], (3, 3)
+        kw = dict(Nsites=[2, 2, 2], Ust=2.0, Jh=0.6, Jx=0.6, Jp=0.5, xmu=0.2)
+        if cfg == "three_orbital_filling":
+            kw["ed_filling"] = 6
+    m = _use(ed, orc, H, U, **kw)
+    isec = ed.sector_index(*sec)
+    r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+    gs, e0 = r["gs"], r["e0"]
+    e = ed.local_energy()
+    N = sum(sec)
+    if kw.get("ed_filling", 0) == 0:
+        total = e["Ekin"] + e["Eknot"] + e["Epot"] - kw.get("xmu", 0.0) * N
+    else:
+        total = e["Ekin"] + e["Eknot"] + e["Epot"] - e["Ehartree"]
+    assert abs(total - e0) < 1e-10 * abs(e0)
+    Hs = np.array(H if np.ndim(H) == 3 else H[None], float).copy()
+    Ns = Hs.shape[1]
+    hop = Hs.copy()
+    for i in range(Ns):
+        hop[:, i, i] = 0.0
+    nsites = kw.get("Nsites")
+    zeros = [0.0] * len(U)
+    mk = orc.Model(hop, zeros, Nsites=nsites, hfmode=False)
+    assert abs(e["Ekin"] - gs @ orc.matvec(mk, *sec, gs).real) < 1e-10 * max(1.0, abs(e["Ekin"]))
+    mu, md = orc.build_map(Ns, sec[0]), orc.build_map(Ns, sec[1])
+    occ_u = (mu[:, None] >> np.arange(Ns)) & 1
+    occ_d = (md[:, None] >> np.arange(Ns)) & 1
+    w = (gs ** 2).reshape(len(md), len(mu))
+    nu_site = (w.sum(0)[:, None] * occ_u).sum(0)
+    nd_site = (w.sum(1)[:, None] * occ_d).sum(0)
+    eknot = np.diag(Hs[0]) @ nu_site + np.diag(Hs[-1]) @ nd_site
+    assert abs(e["Eknot"] - eknot) < 1e-10 * max(1.0, abs(eknot))
+    if len(U) == 1:
+        docc = ed.observables()["docc"].sum()
+        assert abs(e["Eint"] - U[0] * docc) < 1e-10 * max(1.0, abs(e["Eint"]))
+        assert e["Dse"] == 0.0 and e["Dph"] == 0.0
+    else:
+        zero_h = np.zeros_like(Hs)
+        for key, jx, jp in (("Dse", 1.0, 0.0), ("Dph", 0.0, 1.0)):
+            mj = orc.Model(zero_h, zeros, Nsites=nsites, Jx=jx, Jp=jp, hfmode=False)
+            ref = gs @ orc.matvec(mj, *sec, gs).real
+            assert abs(e[key] - ref) < 1e-10 * max(1.0, abs(ref)), key
