@@ -8,7 +8,7 @@ module ED_GPU
   private
 
   integer(c_int), parameter, public :: EDL_FP64 = 0, EDL_FP32 = 1, EDL_MIXED = 2
-  integer(c_int), parameter, public :: EDL_OP_C = 1, EDL_OP_CDG = 2, EDL_OP_SZ = 3
+  integer(c_int), parameter, public :: EDL_OP_C = 1, EDL_OP_CDG = 2, EDL_OP_SZ = 3, EDL_OP_J = 4
 
   type, bind(C), public :: edl_seed_desc
      integer(c_int) :: op, ispin, io, jo

@@ -25,12 +25,14 @@ program edlat_gpu
   integer :: Norb = 1, Ns = 0, Nspin = 1, ed_filling = 0, Lmats = 4096, Lreal = 5000, Ltau = 1024
   integer :: lanc_niter = 512, lanc_ngfiter = 200, sector_nup = -1, sector_ndw = -1, ed_verbose = 3
   real(8) :: Uloc = 2d0, Ust = 0d0, Jh = 0d0, Jx = 0d0, Jp = 0d0, temp = 0.001d0, xmu = 0d0, eps = 0.01d0
-  real(8) :: wini = -5d0, wfin = 5d0, gs_threshold = 1d-9, lanc_tolerance = 1d-18
+  real(8) :: wini = -5d0, wfin = 5d0, gs_threshold = 1d-9, lanc_tolerance = 1d-18, cutoff = 1d-9
   logical :: hfmode = .true., gf_flag = .false., offdiag_gf_flag = .false., chispin_flag = .false.
   ! ED_SPARSE_H (INPUT_VARS.f90:151, default T): the stored-H path's operator
   ! keeps the -xmu*N and hfmode -U/2*N diagonal terms at any ED_FILLING
   ! (stored/H_local.f90:61-90); both paths run on the on-the-fly kernels
   logical :: ed_sparse_h = .true.
+  ! OC_FLAG (INPUT_VARS.f90:138): optical conductivity and Drude weight
+  logical :: oc_flag = .false.
   character(len=256) :: hij_file = "hij.in", ed_precision = "fp64", input_file = "inputED.conf"
   integer(c_int64_t) :: lanc_seed = int(z'5EED', c_int64_t)
 
@@ -50,6 +52,7 @@ program edlat_gpu
   complex(8), allocatable :: Gmats(:, :, :, :), Greal(:, :, :, :)
   real(8), allocatable :: chitau(:, :)
   real(8) :: zeta
+  real(8) :: energy(11)   ! lanc_energy columns of energy_last.ed
 
   call get_command_argument(1, input_file)
   if (len_trim(input_file) == 0) input_file = "inputED.conf"
@@ -71,6 +74,7 @@ program edlat_gpu
   call observables()
   if (gf_flag) call greens_functions()
   if (chispin_flag) call spin_chi()
+  if (oc_flag) call optical_conductivity()
   call edl_check(ctx, edl_ctx_destroy(ctx), "edl_ctx_destroy")
 
 contains
@@ -119,6 +123,8 @@ contains
        case ("GF_FLAG"); read(val, *) gf_flag
        case ("OFFDIAG_GF_FLAG"); read(val, *) offdiag_gf_flag
        case ("CHISPIN_FLAG"); read(val, *) chispin_flag
+       case ("OC_FLAG"); read(val, *) oc_flag
+       case ("CUTOFF"); read(val, *) cutoff
        case ("LANC_NITER"); read(val, *) lanc_niter
        case ("LANC_NGFITER"); read(val, *) lanc_ngfiter
        case ("LANC_TOLERANCE"); read(val, *) lanc_tolerance
@@ -263,15 +269,26 @@ contains
   ! --------------------------------------------------------- OBSERVABLES
   subroutine observables()
     integer :: unit
+    real(8) :: e(11)
     allocate(dens_up(Ns), dens_dw(Ns), docc(Ns), du(Ns), dd(Ns), dc(Ns))
-    dens_up = 0d0; dens_dw = 0d0; docc = 0d0
+    dens_up = 0d0; dens_dw = 0d0; docc = 0d0; energy = 0d0
     do ist = 1, nstates
        call install(ist)
        call edl_check(ctx, edl_observables(ctx, du, dd, dc), "lanc_observables")
        dens_up = dens_up + du / zeta
        dens_dw = dens_dw + dd / zeta
        docc = docc + dc / zeta
+       call edl_check(ctx, edl_local_energy(ctx, e), "lanc_energy")
+       energy = energy + e / zeta
     enddo
+    ! write_energy (ENERGY.f90:635-659)
+    open(newunit=unit, file="energy_info.ed", action="write")
+    write(unit, "(A1,90(A14,1X))") "#", "1<K>", "2<Hi>", "3<V>=<Hi-Ehf>", "4<Eloc>", "5<Ehf>", "6<Dst>", &
+         "7<Dnd>", "8<Dse>", "9<Dph>", "10<Dkxy>", "11<Dkz>"
+    close(unit)
+    open(newunit=unit, file="energy_last.ed", action="write", position="append")
+    write(unit, "(90F15.9)") energy
+    close(unit)
     open(newunit=unit, file="dens.ed", action="write")
     write(unit, "(90(F20.12,1X))") (dens_up(k) + dens_dw(k), k = 1, Ns)
     close(unit)
@@ -417,6 +434,49 @@ contains
     enddo
     close(unit)
   end subroutine greens_functions
+
+  ! ----------------------------------------------------------------- OC
+  !> build_oc_electrons / eval_oc_electrons (OC_ELECTRONS.f90:103-319) for
+  !> the one orbital of this host, then eval_oc_lattice
+  !> (OC_FUNCTIONS.f90:45-89): Drude weight -pi/Nsites*(<K> + 2 D) with D the
+  !> current-correlation sum of peso/dE, resistivity -1/Drude.
+  subroutine optical_conductivity()
+    integer :: j, nl, unit
+    type(edl_seed_desc) :: seeds(1)
+    real(8), allocatable :: alpha(:), beta(:), poles(:), weights(:)
+    real(8) :: norm2(1), d_jj, drude, res, peso
+    integer(c_int) :: nsteps(1), jsec(1)
+    allocate(alpha(lanc_ngfiter), beta(lanc_ngfiter), poles(lanc_ngfiter), weights(lanc_ngfiter))
+    d_jj = 0d0
+    do ist = 1, nstates
+       call install(ist)
+       seeds(1) = edl_seed_desc(EDL_OP_J, 0, 1, 0)
+       call edl_check(ctx, edl_tridiag_batched(ctx, 1, seeds, lanc_ngfiter, merge(EDL_FP32, EDL_FP64, &
+            prec == EDL_FP32), 0, alpha, beta, norm2, nsteps, jsec), "tridiag_Hv_sector")
+       nl = nsteps(1)
+       if (nl <= 0 .or. norm2(1) == 0d0) cycle
+       call edl_check(c_null_ptr, edl_lanczos_poles(nl, alpha, beta, norm2(1), st_energy(ist), 1, poles, weights), &
+            "add_to_lanczos_oc")
+       do j = 1, nl
+          peso = weights(j) / zeta
+          if (abs(poles(j)) < cutoff) cycle
+          if (abs(peso / poles(j)) < cutoff) cycle
+          d_jj = d_jj + peso / poles(j)
+       enddo
+    enddo
+    drude = -acos(-1d0) / dble(Ns) * (energy(1) + 2d0 * d_jj)
+    res = -1d0 / drude
+    open(newunit=unit, file="drude.ed", action="write", position="append")
+    write(unit, *) drude
+    close(unit)
+    open(newunit=unit, file="resistivity.ed", action="write", position="append")
+    write(unit, *) res
+    close(unit)
+    open(newunit=unit, file="jjcorr.ed", action="write", position="append")
+    write(unit, *) d_jj
+    close(unit)
+    write(*, "(A,3F18.10)") "Drude, resistivity, D_jj =", drude, res, d_jj
+  end subroutine optical_conductivity
 
   ! ---------------------------------------------------------------- CHI
   subroutine spin_chi()

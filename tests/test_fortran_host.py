@@ -31,6 +31,7 @@ LANC_NGFITER=200
 GF_FLAG={gf}
 OFFDIAG_GF_FLAG={offdiag}
 CHISPIN_FLAG={gf}
+OC_FLAG={oc}
 LMATS=400
 LREAL=400
 HIJ_FILE=hij.in
@@ -40,9 +41,9 @@ ED_PRECISION={prec}
 """
 
 
-def _prepare(tmp_path, offdiag="F", prec="fp64", xmu=0.0, filling=0, sparse="F", gf="T"):
+def _prepare(tmp_path, offdiag="F", prec="fp64", xmu=0.0, filling=0, sparse="F", gf="T", oc="F"):
     (tmp_path / "inputED.conf").write_text(INPUT.format(offdiag=offdiag, prec=prec, xmu=xmu, filling=filling,
-                                                        sparse=sparse, gf=gf))
+                                                        sparse=sparse, gf=gf, oc=oc))
     lattice.write_hij(str(tmp_path / "hij.in"), lattice.ring(8))
 
 
@@ -116,3 +117,34 @@ def test_driver_ed_sparse_h_route(tmp_path, orc, sparse):
     w0 = np.linalg.eigvalsh(orc.hamiltonian_csr(m, 4, 4).toarray().real)[0]
     egs = float((tmp_path / "egs.ed").read_text().split()[0])
     assert abs(egs - w0) < 1e-10 * abs(w0)
+
+
+@pytest.mark.gpu
+def test_driver_energy_and_optical_conductivity(tmp_path, orc):
+    """energy_last.ed (lanc_energy) and drude.ed / jjcorr.ed (eval_oc_lattice)
+    of the ring8 ground state: E0 = <K> + <Eloc> + <Hi> at ED_FILLING = 0 and
+    xmu = 0; <K> against the oracle's hopping-only operator on the dense
+    ground state; the current correlation D against the exact Lehmann sum of
+    the oracle's current seed, and Drude = -pi/Ns (<K> + 2 D)."""
+    _prepare(tmp_path, gf="F", oc="T")
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    H = lattice.ring(8)
+    m = orc.Model(H, [4.0])
+    Hc = orc.hamiltonian_csr(m, 4, 4)
+    w, V = np.linalg.eigh(Hc.toarray().real)
+    gs, e0 = V[:, 0], w[0]
+    e = np.array((tmp_path / "energy_last.ed").read_text().split(), float)
+    assert e.size == 11
+    assert abs(e[0] + e[3] + e[1] - e0) < 1e-9 * abs(e0)
+    ekin = gs @ orc.matvec(orc.Model(H, [0.0], hfmode=False), 4, 4, gs).real
+    assert abs(e[0] - ekin) < 1e-9 * abs(ekin)
+    seed = orc.seed_current(m, 4, 4, 1, gs).real
+    p, wt = orc.oc_lehmann(Hc.toarray().real, seed, e0)
+    keep = (np.abs(p) >= 1e-9) & (np.abs(wt / np.where(p == 0, 1, p)) >= 1e-9)
+    d = (wt[keep] / p[keep]).sum()
+    djj = float((tmp_path / "jjcorr.ed").read_text().split()[-1])
+    drude = float((tmp_path / "drude.ed").read_text().split()[-1])
+    assert abs(djj - d) < 1e-7 * abs(d)
+    assert abs(drude - (-np.pi / 8 * (ekin + 2 * d))) < 1e-7 * abs(drude)
