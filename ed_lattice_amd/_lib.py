@@ -47,6 +47,7 @@ SIGNATURES = {
     "edl_copy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_int64]),
     "edl_lanczos_gs": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_uint64, c_double, _dp, _ip, _dp, _dp,
                                _dp]),
+    "edl_lanczos_states": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_uint64, _dp, _dp]),
     "edl_lanczos_stats": (c_int, [c_void_p, _ip, _ip, _dp]),
     "edl_set_state": (c_int, [c_void_p, c_int, c_double, _dp]),
     "edl_get_state": (c_int, [c_void_p, _dp]),

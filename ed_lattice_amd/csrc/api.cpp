@@ -517,6 +517,11 @@ struct TwoPassStep {
   T* tmp;
   LzChain L;
   const Sector* seg = nullptr;  // set: segmented up pass, phase B as its own launch
+  // Deflation against nx fixed vectors X (phase B as its own launch too)
+  const double* X = nullptr;
+  int nx = 0;
+  double* defl_part = nullptr;
+  double* defl_coef = nullptr;
 };
 
 template <typename T>
@@ -525,13 +530,18 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  if (tp && tp->seg) {
-    // phase B of the previous step, beta, segmented up, down (+ phase A), alpha
+  if (tp && (tp->seg || tp->nx > 0)) {
+    // phase B of the previous step (+ deflation), beta, up (segmented, or the
+    // column-panel pass with no phase B pending), down (+ phase A), alpha
     T* q0 = static_cast<T*>(cs.h[0].q0);
     T* q1 = static_cast<T*>(cs.h[0].q1);
     launch_phase_b<T>(tp->V, q0, q1, tp->L, st);
+    if (tp->nx > 0) launch_deflate<T>(tp->V, q0, q1, tp->L, tp->X, tp->nx, tp->defl_part, tp->defl_coef, st);
     launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
-    launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
+    if (tp->seg)
+      launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
+    else
+      launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
     launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
     launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
   } else if (tp) {
@@ -646,7 +656,8 @@ struct GsRun {
 // the lowest eigenvalue is rebuilt into d_x (fp64) by replaying the same
 // deterministic recurrence with the tridiagonal eigenvector as weights.
 template <typename T>
-static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double tol, int check, double* d_x) {
+static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double tol, int check, double* d_x,
+                    const double* d_X = nullptr, int nX = 0) {
   hipStream_t st = c.stream;
   const int nmax = (int)std::min<int64_t>(niter, S.dim);
   GsRun r;
@@ -669,12 +680,22 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     tps.lut_up = lut_dev<T>(S.sp[0]);
     tps.lut_dw = lut_dev<T>(S.sp[1]);
     tps.tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * S.sp[1].dim));
-    tp_up = DevBuf(sizeof(double) * (seg ? phase_b_grid(tps.V) : pass_up_grid(tps.V, tps.cfg)));
+    tp_up = DevBuf(sizeof(double) * std::max(phase_b_grid(tps.V), pass_up_grid(tps.V, tps.cfg)));
     tp_dw = DevBuf(sizeof(double) * pass_dw_grid(tps.V, tps.cfg));
     tps.L.part_up = tp_up.as<double>();
     tps.L.part_dw = tp_dw.as<double>();
     sch.block_threads = 256;
     sch.grid = c.num_cu * 8;
+  }
+  DevBuf defl_part, defl_coef;
+  if (nX > 0) {
+    EDL_CHECK(two, "deflated Lanczos needs packed hop tables (two-pass or segmented sectors)");
+    defl_part = DevBuf(sizeof(double) * (size_t)nX * phase_b_grid(tps.V));
+    defl_coef = DevBuf(sizeof(double) * (size_t)nX);
+    tps.X = d_X;
+    tps.nx = nX;
+    tps.defl_part = defl_part.as<double>();
+    tps.defl_coef = defl_coef.as<double>();
   }
   auto bind = [&](const ChainDev& cd, int n, double* x, const double* ccoef) {
     tps.L.sc = cd.sc;
@@ -1122,6 +1143,67 @@ extern "C" int edl_lanczos_gs(edl_ctx* ctx, int isector, int prec, int niter, do
     EDL_HIP(hipMemcpyAsync(gs_host, c.d_gs, sizeof(double) * D, hipMemcpyDeviceToHost, st));
     EDL_HIP(hipStreamSynchronize(st));
   }
+  API_END
+}
+
+// Lowest nstates eigenpairs by deflated Lanczos (the role of sp_eigh at
+// DIAG.f90:182-203 when lanc_nstates_sector > 1).  State m: the start vector
+// and every Lanczos step projected against states 0..m-1, lowest Ritz pair
+// rebuilt by the replay, then orthonormalised against the earlier states.
+// Each state starts from its own SplitMix64 vector (seed + m): Lanczos sees
+// a degenerate level only along the start vector's projection on it, which
+// for a shared start vector is the partner already found and deflated.
+template <typename T>
+static void lanczos_states_impl(Ctx& c, Sector& S, int nstates, int niter, double tol, uint64_t seed, double* X,
+                                double* energies, int64_t* hxv) {
+  hipStream_t st = c.stream;
+  const int64_t D = S.dim;
+  DevBuf start(sizeof(double) * D);
+  for (int m = 0; m < nstates; ++m) {
+    launch_start_vector(start.as<double>(), D, seed + (uint64_t)m, st);
+    double* xm = X + (size_t)m * D;
+    for (int j = 0; j < m; ++j) {
+      const double* xj = X + (size_t)j * D;
+      launch_axpy(start.as<double>(), xj, -device_dot(c, xj, start.as<double>(), D, st), D, st);
+    }
+    GsRun r = gs_run<T>(c, S, start.as<double>(), niter, tol, 5, xm, X, m);
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j < m; ++j) {
+        const double* xj = X + (size_t)j * D;
+        launch_axpy(xm, xj, -device_dot(c, xj, xm, D, st), D, st);
+      }
+    const double n2 = device_dot(c, xm, xm, D, st);
+    launch_scale(xm, D, 1.0 / std::sqrt(n2), st);
+    energies[m] = r.e0;
+    *hxv += r.hxv;
+  }
+  EDL_HIP(hipStreamSynchronize(st));
+}
+
+extern "C" int edl_lanczos_states(edl_ctx* ctx, int isector, int prec, int nstates, int niter, double tol,
+                                  uint64_t seed, double* energies, double* vecs) {
+  API_BEGIN
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_lanczos_states: prec must be EDL_FP64 or EDL_FP32");
+  EDL_CHECK(niter >= 1 && nstates >= 1 && energies, "edl_lanczos_states: bad arguments");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = get_sector(c, isector);
+  const int64_t D = S.dim;
+  EDL_CHECK(nstates <= D, "edl_lanczos_states: more states than the sector dimension");
+  DevBuf X(sizeof(double) * (size_t)D * nstates);
+  EventTimer tm(c.stream);
+  int64_t hxv = 0;
+  if (prec == EDL_FP64)
+    lanczos_states_impl<double>(c, S, nstates, niter, tol, seed, X.as<double>(), energies, &hxv);
+  else
+    lanczos_states_impl<float>(c, S, nstates, niter, tol, seed, X.as<double>(), energies, &hxv);
+  c.last_ms = tm.stop();
+  c.last_hxv = hxv;
+  install_gs(c, isector, D);
+  c.gs_energy = energies[0];
+  EDL_HIP(hipMemcpyAsync(c.d_gs, X.p, sizeof(double) * D, hipMemcpyDeviceToDevice, c.stream));
+  if (vecs) EDL_HIP(hipMemcpyAsync(vecs, X.p, sizeof(double) * (size_t)D * nstates, hipMemcpyDeviceToHost, c.stream));
+  EDL_HIP(hipStreamSynchronize(c.stream));
   API_END
 }
 

@@ -105,6 +105,8 @@ double device_dot(Ctx& c, const double* a, const double* b, int64_t n, hipStream
 template <typename T>
 double device_dot_t(const T* a, const T* b, int64_t n, hipStream_t st);
 void launch_scale(double* x, int64_t n, double s, hipStream_t st);
+// y += a * x
+void launch_axpy(double* y, const double* x, double a, int64_t n, hipStream_t st);
 // Device-side reductions into device memory (no host round trip); `part`
 // holds partials_size(n) doubles.
 int partials_size(int64_t n);

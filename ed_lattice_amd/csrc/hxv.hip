@@ -336,6 +336,11 @@ __global__ void k_to64(const T* __restrict__ s, double* __restrict__ d, int64_t 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     d[i] = (double)s[i];
 }
+__global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, double a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
 __global__ void k_scale(double* x, int64_t n, double s) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] *= s;
@@ -374,6 +379,11 @@ __global__ void k_sumsq_partial(const T* __restrict__ a, int64_t n, double* part
 static int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   return (int)(g > 4096 ? 4096 : (g < 1 ? 1 : g));
+}
+
+void launch_axpy(double* y, const double* x, double a, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, a, n);
+  EDL_HIP(hipGetLastError());
 }
 
 void launch_start_vector(double* d_x, int64_t n, uint64_t seed, hipStream_t st) {

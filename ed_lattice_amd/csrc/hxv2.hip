@@ -682,6 +682,72 @@ __global__ __launch_bounds__(256) void k_phase_b(TwoPassView V, T* qa, T* qb, Lz
   if (threadIdx.x == 0) L.part_up[blockIdx.x] = v;
 }
 
+// Deflation of a chain against nx fixed vectors X (fp64, natural layout,
+// normalised, mutually orthogonal): after phase B, r -= sum_m (x_m . r) x_m,
+// and the block partials of |r|^2 are taken after the projection.  Three
+// launches on the phase-B grid: dots, one-block coefficients, update.
+// They run only where phase B ran (same scalar tests as k_phase_b).
+static __device__ __forceinline__ bool deflate_active(const LzScalars& s, const LzChain& L) {
+  return !s.stopped && s.pad && s.k + 1 < L.nmax;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_defl_dot(TwoPassView V, T* qa, T* qb, LzChain L,
+                                                  const double* __restrict__ X, int nx, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (!deflate_active(s, L)) return;
+  const T* __restrict__ r = ((s.k + 1) & 1) ? qb : qa;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld, D = (size_t)dimup * dimdw;
+  for (int m = 0; m < nx; ++m) {
+    const double* __restrict__ x = X + (size_t)m * D;
+    double d = 0.0;
+    for (int col = blockIdx.x; col < dimdw; col += gridDim.x)
+      for (int row = threadIdx.x; row < dimup; row += blockDim.x)
+        d += x[(size_t)col * dimup + row] * (double)r[(size_t)col * ld + row];
+    d = block_sum(d, sh);
+    if (threadIdx.x == 0) part[(size_t)m * gridDim.x + blockIdx.x] = d;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_defl_coef(LzChain L, const double* __restrict__ part, int nparts, int nx,
+                                                   double* __restrict__ coef) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (!deflate_active(s, L)) return;
+  for (int m = 0; m < nx; ++m) {
+    const double c = ordered_sum_blocks(part + (size_t)m * nparts, nparts, sh);
+    if (threadIdx.x == 0) coef[m] = c;
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_defl_update(TwoPassView V, T* qa, T* qb, LzChain L,
+                                                     const double* __restrict__ X, int nx,
+                                                     const double* __restrict__ coef) {
+  __shared__ double sh[16];
+  const LzScalars s = *L.sc;
+  if (!deflate_active(s, L)) return;
+  T* __restrict__ r = ((s.k + 1) & 1) ? qb : qa;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld, D = (size_t)dimup * dimdw;
+  double nrm = 0.0;
+  for (int col = blockIdx.x; col < dimdw; col += gridDim.x)
+    for (int row = threadIdx.x; row < dimup; row += blockDim.x) {
+      const size_t n = (size_t)col * dimup + row, i = (size_t)col * ld + row;
+      double e = (double)r[i];
+      for (int m = 0; m < nx; ++m) e -= coef[m] * X[(size_t)m * D + n];
+      const T et = (T)e;
+      r[i] = et;
+      nrm += (double)et * (double)et;
+    }
+  const double v = block_sum(nrm, sh);
+  if (threadIdx.x == 0) L.part_up[blockIdx.x] = v;
+}
+
 __global__ __launch_bounds__(256) void k_lz_beta_n(LzChain L, int nparts) {
   // same as k_lz_beta with an explicit partial count
   __shared__ double sh[16];
@@ -821,6 +887,22 @@ template void launch_up_seg<double>(const TwoPassView&, const InterParams&, cons
                                     const LzChain*, const Sector&, int, hipStream_t);
 template void launch_up_seg<float>(const TwoPassView&, const InterParams&, const float*, float*, float*, float*,
                                    const LzChain*, const Sector&, int, hipStream_t);
+template <typename T>
+void launch_deflate(const TwoPassView& V, T* qa, T* qb, const LzChain& L, const double* X, int nx, double* part,
+                    double* coef, hipStream_t st) {
+  const int g = phase_b_grid(V);
+  hipLaunchKernelGGL(k_defl_dot<T>, dim3(g), dim3(256), 0, st, V, qa, qb, L, X, nx, part);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_defl_coef, dim3(1), dim3(256), 0, st, L, part, g, nx, coef);
+  EDL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_defl_update<T>, dim3(g), dim3(256), 0, st, V, qa, qb, L, X, nx, coef);
+  EDL_HIP(hipGetLastError());
+}
+
+template void launch_deflate<double>(const TwoPassView&, double*, double*, const LzChain&, const double*, int,
+                                     double*, double*, hipStream_t);
+template void launch_deflate<float>(const TwoPassView&, float*, float*, const LzChain&, const double*, int,
+                                    double*, double*, hipStream_t);
 template void launch_phase_b<double>(const TwoPassView&, double*, double*, const LzChain&, hipStream_t);
 template void launch_phase_b<float>(const TwoPassView&, float*, float*, const LzChain&, hipStream_t);
 

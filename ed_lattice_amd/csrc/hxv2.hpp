@@ -116,5 +116,11 @@ int phase_b_grid(const TwoPassView& V);
 template <typename T>
 void launch_phase_b(const TwoPassView& V, T* qa, T* qb, const LzChain& L, hipStream_t st);
 void launch_lz_beta_n(const LzChain& L, int nparts, hipStream_t st);
+// After launch_phase_b: r -= sum_m (x_m . r) x_m against nx fp64 vectors X
+// (natural layout, orthonormal), with the |r|^2 partials of L.part_up
+// retaken after the projection (part: nx * phase_b_grid doubles, coef: nx).
+template <typename T>
+void launch_deflate(const TwoPassView& V, T* qa, T* qb, const LzChain& L, const double* X, int nx, double* part,
+                    double* coef, hipStream_t st);
 
 }  // namespace edl

@@ -21,6 +21,7 @@ module ED_GPU
   public :: edl_device_alloc, edl_copy_h2d, edl_copy_d2h, edl_vec_random, edl_vec_dot
   public :: edl_lanczos_gs, edl_lanczos_stats, edl_set_state, edl_get_state
   public :: edl_tridiag, edl_tridiag_batched, edl_lanczos_poles, edl_observables, edl_local_energy
+  public :: edl_lanczos_states
   public :: edl_last_timing, edl_bench_hxv, edl_bench_passes
   public :: edl_check, edl_error_message
 
@@ -216,6 +217,17 @@ module ED_GPU
        type(c_ptr), value :: ctx
        real(c_double) :: dens_up(*), dens_dw(*), docc(*)
      end function edl_observables
+
+     integer(c_int) function edl_lanczos_states(ctx, isector, prec, nstates, niter, tol, seed, energies, vecs) &
+          bind(C, name="edl_lanczos_states")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: isector, prec, nstates, niter
+       real(c_double), value :: tol
+       integer(c_int64_t), value :: seed
+       real(c_double) :: energies(*)
+       type(c_ptr), value :: vecs
+     end function edl_lanczos_states
 
      integer(c_int) function edl_local_energy(ctx, e) bind(C, name="edl_local_energy")
        import :: c_int, c_ptr, c_double

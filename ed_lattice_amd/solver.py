@@ -253,6 +253,17 @@ class EDLattice:
         return dict(e0=e0.value, nsteps=ns.value, alpha=a[:ns.value], beta=b[:ns.value], gs=gs,
                     steps_fp32=lo.value, steps_fp64=hi.value, e0_fp32=elo.value, ms=ms, hxv=nh)
 
+    def lanczos_states(self, isector, nstates, prec="fp64", niter=300, tol=1e-12, seed=0x5EED, want_vectors=True):
+        """Lowest nstates eigenpairs by deflated Lanczos (sp_eigh role for
+        lanc_nstates_sector > 1): energies and, if asked, vectors (rows)."""
+        dim, _, _ = self.build_sector(isector)
+        e = np.zeros(nstates)
+        v = np.zeros((nstates, dim)) if want_vectors else None
+        self._ck(self.lib.edl_lanczos_states(self.h, isector, PREC[prec], nstates, niter, tol, seed, _dp(e),
+                                             _dp(v) if v is not None else None))
+        ms, nh = self.last_timing()
+        return dict(energies=e, vecs=v, ms=ms, hxv=nh)
+
     def set_state(self, isector, energy, vec):
         v = np.ascontiguousarray(vec, dtype=np.float64)
         self._ck(self.lib.edl_set_state(self.h, isector, energy, _dp(v)))

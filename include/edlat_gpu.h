@@ -100,6 +100,17 @@ int edl_copy_d2h(edl_ctx* ctx, void* dst_host, const void* src_dev, int64_t byte
  * = 0) receive the tridiagonal of the last Lanczos run. */
 int edl_lanczos_gs(edl_ctx* ctx, int isector, int prec, int niter, double tol, uint64_t seed, double refine_tol,
                    double* e0, int* nsteps, double* alpha, double* beta, double* gs_host);
+/* Lowest nstates eigenpairs of a sector (the role of sp_eigh / ARPACK at
+ * DIAG.f90:182-203 when lanc_nstates_sector > 1): state m is the lowest
+ * Ritz pair of a Lanczos run whose start vector and every step are
+ * projected against states 0..m-1 (deflation; degenerate states come out
+ * one by one: state m starts from the SplitMix64 vector of seed + m).  Each run takes up to niter steps, fewer when |dE| < tol*|E|
+ * between checks.  energies[nstates] ascending as found; vecs (nstates*Dim
+ * doubles, row m = state m, normalised) when non-NULL.  State 0 is
+ * installed as by edl_lanczos_gs.  Two-pass or segmented sectors only (the
+ * packed hop tables). */
+int edl_lanczos_states(edl_ctx* ctx, int isector, int prec, int nstates, int niter, double tol, uint64_t seed,
+                       double* energies, double* vecs);
 /* Statistics of the last edl_lanczos_gs: FP32 steps, FP64 steps, and the
  * FP32 energy before refinement. */
 int edl_lanczos_stats(edl_ctx* ctx, int* steps_lo, int* steps_hi, double* e0_lo);

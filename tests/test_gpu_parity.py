@@ -578,3 +578,45 @@ def test_local_energy_vs_oracle(ed, orc, cfg):
             mj = orc.Model(zero_h, zeros, Nsites=nsites, Jx=jx, Jp=jp, hfmode=False)
             ref = gs @ orc.matvec(mj, *sec, gs).real
             assert abs(e[key] - ref) < 1e-10 * max(1.0, abs(ref)), key
+
+
+@pytest.mark.parametrize("route", ["two_pass", "segmented", "fp32"])
+def test_lanczos_states_ring8_vs_dense(ed, orc, route, monkeypatch):
+    """Lowest eigenpairs by deflated Lanczos against dense eigh of the
+    oracle's ring8 (4,4) Hamiltonian, degenerate levels included: energies,
+    residuals |Hx - Ex| and orthonormality of the vectors."""
+    if route == "segmented":
+        monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "24")
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    Hd = orc.hamiltonian_csr(m, 4, 4).toarray().real
+    w = np.linalg.eigvalsh(Hd)
+    k = 6
+    prec = "fp32" if route == "fp32" else "fp64"
+    r = ed.lanczos_states(isec, k, prec=prec, niter=400, tol=1e-13)
+    e, X = r["energies"], r["vecs"]
+    print(f"states {route}: {e} dense {w[:k]}", flush=True)
+    tol = 2e-5 if prec == "fp32" else 1e-9
+    np.testing.assert_allclose(np.sort(e), w[:k], rtol=tol, atol=tol)
+    np.testing.assert_allclose(X @ X.T, np.eye(k), atol=1e-9)
+    for j in range(k):
+        res = np.linalg.norm(Hd @ X[j] - e[j] * X[j])
+        assert res < (2e-3 if prec == "fp32" else 1e-6) * abs(e[j]), (j, res)
+
+
+def test_lanczos_states_rect3x4(ed, orc):
+    """3x4 (6,6), dim 853 776: the first state equals the ground state of
+    the single-state solver, the others lie above it, and every vector is an
+    eigenvector of the oracle's operator."""
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    e0 = ed.lanczos_eigh(isec, "fp64", niter=300, tol=1e-13)["e0"]
+    r = ed.lanczos_states(isec, 3, prec="fp64", niter=400, tol=1e-13)
+    e, X = r["energies"], r["vecs"]
+    print(f"rect3x4 states {e}", flush=True)
+    assert abs(e[0] - e0) < 1e-10 * abs(e0)
+    assert np.all(e[1:] >= e[0] - 1e-9)
+    np.testing.assert_allclose(X @ X.T, np.eye(3), atol=1e-9)
+    for j in range(3):
+        hx = orc.matvec(m, 6, 6, X[j], nthreads=8).real
+        assert np.linalg.norm(hx - e[j] * X[j]) < 1e-5 * abs(e[j]), j
