@@ -33,6 +33,10 @@ program edlat_gpu
   logical :: ed_sparse_h = .true.
   ! OC_FLAG (INPUT_VARS.f90:138): optical conductivity and Drude weight
   logical :: oc_flag = .false.
+  ! ED_FINITE_TEMP with LANC_NSTATES_SECTOR / LANC_NSTATES_TOTAL
+  ! (SETUP.f90:87-103): several states per sector, Boltzmann weights
+  logical :: ed_finite_temp = .false.
+  integer :: lanc_nstates_sector = 2, lanc_nstates_total = 10
   character(len=256) :: hij_file = "hij.in", ed_precision = "fp64", input_file = "inputED.conf"
   integer(c_int64_t) :: lanc_seed = int(z'5EED', c_int64_t)
 
@@ -40,10 +44,11 @@ program edlat_gpu
   real(8), allocatable :: Hij(:, :, :), wm(:), wr(:), tau(:)
   integer :: prec, nsel, k, isel, ist, nstates
   integer, allocatable :: sel(:)
-  ! state list (T = 0)
-  integer, parameter :: maxstates = 64
+  ! state list (EIGENSPACE.f90): sector, energy, Boltzmann factor
+  ! exp(-(E - Emin)/T) at finite temperature (1 at T = 0), vector
+  integer, parameter :: maxstates = 1024
   integer :: st_sector(maxstates)
-  real(8) :: st_energy(maxstates), emin
+  real(8) :: st_energy(maxstates), st_w(maxstates), emin
   type vecbox
      real(8), allocatable :: v(:)
   end type vecbox
@@ -58,6 +63,11 @@ program edlat_gpu
   if (len_trim(input_file) == 0) input_file = "inputED.conf"
   call read_input(trim(input_file))
   call read_hij(trim(hij_file))
+  if (ed_finite_temp) then   ! SETUP.f90:91-103
+     if (mod(lanc_nstates_sector, 2) /= 0) lanc_nstates_sector = lanc_nstates_sector + 1
+     lanc_nstates_total = lanc_nstates_sector * (Ns + 1)**2 + 10
+     if (mod(lanc_nstates_total, 2) /= 0) lanc_nstates_total = lanc_nstates_total + 1
+  endif
 
   select case (trim(ed_precision))
   case ("fp64"); prec = EDL_FP64
@@ -70,7 +80,9 @@ program edlat_gpu
   call set_model()
   call select_sectors()
   call diagonalize()
-  zeta = dble(nstates)
+  ! get_Z_function (DIAG.f90:56-76): the number of kept states at T = 0,
+  ! sum of the Boltzmann factors of the trimmed list at finite temperature
+  zeta = sum(st_w(1:nstates))
   call observables()
   if (gf_flag) call greens_functions()
   if (chispin_flag) call spin_chi()
@@ -125,6 +137,9 @@ contains
        case ("CHISPIN_FLAG"); read(val, *) chispin_flag
        case ("OC_FLAG"); read(val, *) oc_flag
        case ("CUTOFF"); read(val, *) cutoff
+       case ("ED_FINITE_TEMP"); read(val, *) ed_finite_temp
+       case ("LANC_NSTATES_SECTOR"); read(val, *) lanc_nstates_sector
+       case ("LANC_NSTATES_TOTAL"); read(val, *) lanc_nstates_total
        case ("LANC_NITER"); read(val, *) lanc_niter
        case ("LANC_NGFITER"); read(val, *) lanc_ngfiter
        case ("LANC_TOLERANCE"); read(val, *) lanc_tolerance
@@ -215,6 +230,10 @@ contains
        ierr = edl_sector_qn(Ns, isector, nup, ndw)
        call edl_check(ctx, edl_sector_build(ctx, isector, dim, dimup, dimdw), "build_sector")
        niter = int(min(int(lanc_niter, c_int64_t), dim))
+       if (ed_finite_temp) then
+          call sector_states(isector, nup, ndw, dim, niter, unit)
+          cycle
+       endif
        allocate(alpha(niter), beta(niter), vec(dim))
        call edl_check(ctx, edl_lanczos_gs(ctx, isector, prec, niter, 0d0, lanc_seed, 1d-13, e0, nsteps, alpha, &
             beta, c_loc(vec)), "sp_lanc_eigh")
@@ -234,6 +253,8 @@ contains
     enddo
     close(unit)
     emin = minval(st_energy(1:nstates))
+    st_w(1:nstates) = 1d0
+    if (ed_finite_temp) call trim_states()
     open(newunit=unit, file="state_list.ed", action="write")
     do ist = 1, nstates
        ierr = edl_sector_qn(Ns, st_sector(ist), nup, ndw)
@@ -242,6 +263,64 @@ contains
     enddo
     close(unit)
   end subroutine diagonalize
+
+  !> ed_diag_d at finite temperature (DIAG.f90:142-151, 262-265): the lowest
+  !> Neigen = min(lanc_nstates_sector, Dim) states of the sector by deflated
+  !> Lanczos, added to the state list.
+  subroutine sector_states(isector, nup, ndw, dim, niter, unit)
+    integer, intent(in) :: isector, nup, ndw, niter, unit
+    integer(c_int64_t), intent(in) :: dim
+    integer :: neigen, i
+    real(8), allocatable :: e(:)
+    real(8), allocatable, target :: vecs(:)
+    neigen = int(min(int(lanc_nstates_sector, c_int64_t), dim))
+    allocate(e(neigen), vecs(dim * neigen))
+    call edl_check(ctx, edl_lanczos_states(ctx, isector, merge(EDL_FP32, EDL_FP64, prec == EDL_FP32), neigen, &
+         niter, 1d-13, lanc_seed, e, c_loc(vecs)), "sp_eigh")
+    do i = 1, neigen
+       write(unit, "(I6,2I4,F24.15)") isector, nup, ndw, e(i)
+       call add_state(isector, e(i), vecs((i - 1) * dim + 1:i * dim))
+    enddo
+    if (ed_verbose > 2) write(*, "(A,I6,A,I3,A,I3,A,I12,A,20F16.10)") "sector", isector, " nup", nup, " ndw", &
+         ndw, " dim", dim, " E", e
+  end subroutine sector_states
+
+  !> es_add_state with size = lanc_nstates_total keeps the lowest states of
+  !> all sectors; es_trim_size (EIGENSPACE.f90:440-475) then drops the
+  !> states from the first one with exp(-(E - Emin)/T) <= CUTOFF on.
+  subroutine trim_states()
+    integer :: i, j, keep
+    integer :: isec_t
+    real(8) :: e_t
+    type(vecbox) :: v_t
+    do i = 2, nstates   ! insertion sort by energy (stable)
+       j = i
+       do while (j > 1)
+          if (st_energy(j - 1) <= st_energy(j)) exit
+          isec_t = st_sector(j); st_sector(j) = st_sector(j - 1); st_sector(j - 1) = isec_t
+          e_t = st_energy(j); st_energy(j) = st_energy(j - 1); st_energy(j - 1) = e_t
+          call move_alloc(st_vec(j)%v, v_t%v)
+          call move_alloc(st_vec(j - 1)%v, st_vec(j)%v)
+          call move_alloc(v_t%v, st_vec(j - 1)%v)
+          j = j - 1
+       enddo
+    enddo
+    keep = min(nstates, lanc_nstates_total)
+    do i = 1, keep
+       if (exp(-(st_energy(i) - emin) / temp) <= cutoff) then
+          keep = i - 1
+          exit
+       endif
+    enddo
+    if (keep == 0) stop "es_return_size: list does not contain any node"
+    do i = keep + 1, nstates
+       if (allocated(st_vec(i)%v)) deallocate(st_vec(i)%v)
+    enddo
+    nstates = keep
+    do i = 1, nstates
+       st_w(i) = exp(-(st_energy(i) - emin) / temp)
+    enddo
+  end subroutine trim_states
 
   subroutine free_states()
     do ist = 1, nstates
@@ -275,11 +354,11 @@ contains
     do ist = 1, nstates
        call install(ist)
        call edl_check(ctx, edl_observables(ctx, du, dd, dc), "lanc_observables")
-       dens_up = dens_up + du / zeta
-       dens_dw = dens_dw + dd / zeta
-       docc = docc + dc / zeta
+       dens_up = dens_up + du * st_w(ist) / zeta
+       dens_dw = dens_dw + dd * st_w(ist) / zeta
+       docc = docc + dc * st_w(ist) / zeta
        call edl_check(ctx, edl_local_energy(ctx, e), "lanc_energy")
-       energy = energy + e / zeta
+       energy = energy + e * st_w(ist) / zeta
     enddo
     ! write_energy (ENERGY.f90:635-659)
     open(newunit=unit, file="energy_info.ed", action="write")
@@ -374,8 +453,10 @@ contains
              poles = 0d0; weights = 0d0
           endif
           do j = 1, nl
-             Gmats(ispin, io, jo, :) = Gmats(ispin, io, jo, :) + weights(j) / zeta / (dcmplx(0d0, wm) - poles(j))
-             Greal(ispin, io, jo, :) = Greal(ispin, io, jo, :) + weights(j) / zeta / (dcmplx(wr, eps) - poles(j))
+             Gmats(ispin, io, jo, :) = Gmats(ispin, io, jo, :) + weights(j) * st_w(ist) / zeta / &
+                  (dcmplx(0d0, wm) - poles(j))
+             Greal(ispin, io, jo, :) = Greal(ispin, io, jo, :) + weights(j) * st_w(ist) / zeta / &
+                  (dcmplx(wr, eps) - poles(j))
           enddo
           call move_alloc(poles, store(ich, ist, ispin, io, jo)%p)
           call move_alloc(weights, store(ich, ist, ispin, io, jo)%w)
@@ -458,7 +539,7 @@ contains
        call edl_check(c_null_ptr, edl_lanczos_poles(nl, alpha, beta, norm2(1), st_energy(ist), 1, poles, weights), &
             "add_to_lanczos_oc")
        do j = 1, nl
-          peso = weights(j) / zeta
+          peso = weights(j) * st_w(ist) / zeta
           if (abs(poles(j)) < cutoff) cycle
           if (abs(peso / poles(j)) < cutoff) cycle
           d_jj = d_jj + peso / poles(j)
@@ -509,7 +590,7 @@ contains
                beta((s - 1) * lanc_ngfiter + 1), norm2(s), st_energy(ist), 1, poles, weights), "add_to_lanczos_spinChi")
           ! chi(tau) with the Ltau/2 symmetrisation (CHI_SPIN_ELECTRONS.f90:435-441)
           do j = 1, nl
-             peso = weights(j) / zeta
+             peso = weights(j) * st_w(ist) / zeta
              de = poles(j)
              do i = 0, Ltau / 2 - 1
                 chitau(s, i) = chitau(s, i) + peso * exp(-tau(i) * de)

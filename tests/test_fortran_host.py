@@ -148,3 +148,70 @@ def test_driver_energy_and_optical_conductivity(tmp_path, orc):
     drude = float((tmp_path / "drude.ed").read_text().split()[-1])
     assert abs(djj - d) < 1e-7 * abs(d)
     assert abs(drude - (-np.pi / 8 * (ekin + 2 * d))) < 1e-7 * abs(drude)
+
+
+FT_INPUT = """NORB=1
+NSITES=6
+NSPIN=1
+ULOC=3.0
+XMU=0.0
+HFMODE=T
+ED_FILLING=0
+ED_SPARSE_H=F
+ED_FINITE_TEMP=T
+TEMP=0.5
+LANC_NSTATES_SECTOR=2
+LANC_NITER=200
+CUTOFF=1e-9
+GF_FLAG=F
+CHISPIN_FLAG=F
+HIJ_FILE=hij.in
+"""
+
+
+@pytest.mark.gpu
+def test_driver_finite_temperature_ring6(tmp_path, orc):
+    """ED_FINITE_TEMP: every sector of the 6-site ring contributes its two
+    lowest states (deflated Lanczos), the list is cut to LANC_NSTATES_TOTAL
+    and trimmed at exp(-(E - Emin)/T) <= CUTOFF, and observables and
+    energies carry the Boltzmann weights.  The oracle repeats the selection
+    with dense eigenpairs per sector; compared are the kept energies and the
+    translation-invariant thermal averages (total density, total double
+    occupancy, energies), which do not depend on the combination kept when a
+    degenerate level is cut."""
+    (tmp_path / "inputED.conf").write_text(FT_INPUT)
+    lattice.write_hij(str(tmp_path / "hij.in"), lattice.ring(6))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    H, U, T = lattice.ring(6), 3.0, 0.5
+    m = orc.Model(H, [U])
+    kept = []
+    for nup in range(7):
+        for ndw in range(7):
+            Hd = orc.hamiltonian_csr(m, nup, ndw).toarray().real
+            w, V = np.linalg.eigh(Hd)
+            for j in range(min(2, len(w))):
+                kept.append((w[j], nup, ndw, V[:, j]))
+    kept.sort(key=lambda t: t[0])
+    kept = kept[:2 * 49 + 10]
+    emin = kept[0][0]
+    kept = [k for k in kept if np.exp(-(k[0] - emin) / T) > 1e-9]
+    bw = np.array([np.exp(-(k[0] - emin) / T) for k in kept])
+    Z = bw.sum()
+    ntot = docc = 0.0
+    for (e, nup, ndw, v), b in zip(kept, bw):
+        o = orc.observables(6, nup, ndw, v)
+        ntot += b / Z * (o["dens_up"] + o["dens_dw"]).sum()
+        docc += b / Z * o["docc"].sum()
+    egs = float((tmp_path / "egs.ed").read_text().split()[0])
+    assert abs(egs - emin) < 1e-10 * abs(emin)
+    lst = np.loadtxt(tmp_path / "state_list.ed", ndmin=2)
+    np.testing.assert_allclose(lst[:, 1], [k[0] for k in kept], rtol=0, atol=1e-9)
+    dens = np.array((tmp_path / "dens.ed").read_text().split(), float)
+    dc = np.array((tmp_path / "docc.ed").read_text().split(), float)
+    assert abs(dens.sum() - ntot) < 1e-8
+    assert abs(dc.sum() - docc) < 1e-8
+    e = np.array((tmp_path / "energy_last.ed").read_text().split(), float)
+    assert abs(e[2] - U * docc) < 1e-8   # <V> = <Hi> - <Ehf> = U sum docc
+    assert abs(e[4] + U / 2 * ntot) < 1e-8  # <Ehf> = -U/2 <N>
