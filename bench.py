@@ -53,6 +53,13 @@ def cpu_baseline(H, U, nup, ndw, dim, target_s=15.0):
                        f"{full_s:.1f} s per full H x v")
 
 
+def dw_kernel_name(t, sched):
+    """Name of the down-pass kernel a schedule runs (as rocprofv3 prints it)."""
+    if sched.get("dw_vector_gather"):
+        return f"k_pass_dwv<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>"
+    return f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>"
+
+
 def kernel_table(dim, elem, up_ms, dw_ms, sched):
     """Algorithmic HBM bytes per launch and achieved GB/s of the two kernels
     of the operator: the up pass reads v and writes the intermediate T
@@ -64,14 +71,14 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
         for name, ms, nbytes, k in (
                 ("pass_up", up_ms, 2 * dim * elem, f"k_up_seg<{t}, {sched['up_cols']}, 8>"),
                 ("pass_dw", dw_ms, 3 * dim * elem,
-                 f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>")):
+                 dw_kernel_name(t, sched))):
             out[name] = {"kernel": k, "ms": ms, "algorithmic_bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
     elif sched["two_pass"]:
         for name, ms, nbytes, k in (
                 ("pass_up", up_ms, 2 * dim * elem,
                  f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}>"),
                 ("pass_dw", dw_ms, 3 * dim * elem,
-                 f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>")):
+                 dw_kernel_name(t, sched))):
             out[name] = {"kernel": k, "ms": ms, "algorithmic_bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
     else:
         nbytes = 2 * dim * elem

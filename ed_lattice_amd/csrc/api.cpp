@@ -149,37 +149,45 @@ static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg>& tp_cache() {
 template <typename T>
 static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T));
-  auto it = tp_cache().find(key);
-  if (it != tp_cache().end()) return it->second;
   // timed on the engine's own (padded) layout
   const int ld = padded_ld(s.sp[0].dim);
   const TwoPassView V = two_pass_view(s, ld);
   // EDL_TWO_PASS_CFG_F32 / _F64 =
-  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch]]"
+  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec]]]"
   // pins the schedule of that precision (reproducible profiles of one
-  // configuration); no tuning then.
+  // configuration), ahead of any tuned schedule; no tuning then.
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
-                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch);
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
+                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec);
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
-                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch]]");
+                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
               "EDL_TWO_PASS_CFG_F32/_F64: (up_rpt, up_kch) must be (1, 8|16) or (2, 4|8)");
-    EDL_CHECK((p.ncol == 1 && (p.kch == 8 || p.kch == 16 || p.kch == 32)) ||
-                  (p.ncol == 2 && (p.kch == 8 || p.kch == 16)),
-              "EDL_TWO_PASS_CFG_F32/_F64: (ncol, kch) must be (1, 8|16|32) or (2, 8|16)");
-    EDL_CHECK(p.C >= 1 && p.C <= 4 && (p.R == 32 || p.R == 64),
-              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1 to 4 and R 32 or 64");
+    EDL_CHECK(pass_dw_lds_bytes(V, p) <= (size_t)(160 * 1024 - 4096),
+              "EDL_TWO_PASS_CFG_F32/_F64: down-pass column table exceeds LDS");
+    EDL_CHECK(p.dw_vec == 0 || p.dw_vec == 1, "EDL_TWO_PASS_CFG_F32/_F64: dw_vec must be 0 or 1");
+    if (p.dw_vec)
+      EDL_CHECK(((p.ncol == 1 && (p.kch == 2 || p.kch == 4)) || (p.ncol == 2 && (p.kch == 1 || p.kch == 2 || p.kch == 4))) &&
+                    (p.R == 16 || p.R == 32 || p.R == 64 || p.R == 128 || (p.R == 256 && sizeof(T) == 4)),
+                "EDL_TWO_PASS_CFG_F32/_F64: the vector-gather down pass takes R 16|32|64|128 (256 in fp32), "
+                "(ncol, kch) in (1, 2|4) or (2, 1|2|4)");
+    else
+      EDL_CHECK((p.ncol == 1 && (p.kch == 8 || p.kch == 16 || p.kch == 32)) ||
+                    (p.ncol == 2 && (p.kch == 8 || p.kch == 16)),
+                "EDL_TWO_PASS_CFG_F32/_F64: (ncol, kch) must be (1, 8|16|32) or (2, 8|16)");
+    EDL_CHECK(p.C >= 1 && p.C <= 4 && (p.R == 32 || p.R == 64 || p.dw_vec),
+              "EDL_TWO_PASS_CFG_F32/_F64: C must be 1 to 4 and R 32 or 64 (or a vector-gather height)");
     EDL_CHECK(p.threads_up % 64 == 0 && p.threads_up >= 64 && p.threads_up <= 1024 && p.threads_dw % 64 == 0 &&
                   p.threads_dw >= 64 && p.threads_dw <= 1024 && p.grid_up >= 1 && p.nchunk >= 1 &&
                   p.nchunk <= V.dimdw,
               "EDL_TWO_PASS_CFG_F32/_F64: bad block size, grid or chunk count");
     EDL_CHECK(256 * sizeof(T) + 16 + (size_t)V.dimup * p.C * sizeof(T) <= (size_t)(160 * 1024),
               "EDL_TWO_PASS_CFG_F32/_F64: up-pass columns exceed LDS");
-    tp_cache()[key] = p;
     return p;
   }
+  auto it = tp_cache().find(key);
+  if (it != tp_cache().end()) return it->second;
   hipStream_t st = c.stream;
   const int64_t D = s.dim, Dp = (int64_t)ld * s.sp[1].dim;
   DevBuf d64(sizeof(double) * D), q(sizeof(T) * Dp), t(sizeof(T) * Dp), o(sizeof(T) * Dp);
@@ -221,28 +229,32 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best.grid_up = w.grid_up;
         }
       }
-  for (int R : {32, 64})
-    for (int nk : {108, 116, 132, 208, 216})
+  // 4-byte (8-byte) gathers, then 16-byte vector gathers
+  for (int vec : {0, 1})
+  for (int R : {16, 32, 64, 128, 256})
+    for (int nk : {108, 116, 132, 208, 216, 102, 104, 201, 202, 204})
       for (int th : {256, 512, 1024})
         for (int nch : {32, 64, 128, 256}) {
         if (nch > V.dimdw) continue;
         const int nc = nk / 100, kc = nk % 100;
+        const bool vk = kc <= 4;
+        if (vk != (vec == 1) || (!vec && (R == 16 || R > 64)) || (R == 256 && sizeof(T) != 4)) continue;
         TwoPassCfg w = best;
         w.R = R;
         w.ncol = nc;
         w.kch = kc;
         w.threads_dw = th;
         w.nchunk = nch;
-        const size_t tbl = sizeof(uint32_t) * (size_t)((V.dimdw + nch - 1) / nch + 1) * (V.kdw + 1);
-        if (tbl > (size_t)lds - 4096) continue;
+        w.dw_vec = vec;
+        if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
         launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
         EventTimer tm(st);
         for (int r = 0; r < 3; ++r)
           launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
         const double ms = tm.stop();
         if (verbose())
-          std::fprintf(stderr, "[edl] tune dw  fp%d R %d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
-                       8 * (int)sizeof(T), R, nc, kc, th, nch, ms / 3);
+          std::fprintf(stderr, "[edl] tune dw  fp%d vec %d R %3d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
+                       8 * (int)sizeof(T), vec, R, nc, kc, th, nch, ms / 3);
         if (ms < best_dw) {
           best_dw = ms;
           best.R = R;
@@ -250,6 +262,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best.kch = kc;
           best.threads_dw = th;
           best.nchunk = nch;
+          best.dw_vec = vec;
         }
       }
   tp_cache()[key] = best;
@@ -1765,6 +1778,8 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
       launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
     }
+    const void* bufs[3] = {v.p, tmp, h.p};
+    if (!pass_dw_vec_ok(V, cf, sizeof(T), bufs, 3)) cf.dw_vec = 0;  // the form that ran
   } else {
     for (int r = 0; r < reps; ++r) {
       EDL_HIP(hipEventRecord(ev[3 * r], st));
@@ -1785,9 +1800,10 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
   ms_pass[1] = (two || seg) ? dw / reps : 0.0;
-  const int32_t out[12] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
-                           cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch};
-  for (int k = 0; k < 12; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
+  const int32_t out[13] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+                           cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
+                           cf.dw_vec};
+  for (int k = 0; k < 13; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

@@ -375,6 +375,150 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   }
 }
 
+// Vector-gather down pass (R-row panels, R = 16, 32 or 64; padded layout:
+// ld % 32 == 0).  A lane loads VW = 16 / sizeof(T) consecutive rows with
+// one 16-byte load, so an R-row neighbour segment takes LPS = R / VW lanes
+// and one wave instruction fetches GPW = 64 / LPS segments: lane group g takes the hops
+// k = g, g + GPW, g + 2 GPW, ... of the wave's column, and the groups' sums
+// meet by lane shuffles.  Against one 4-byte element per lane this issues
+// GPW times fewer gather instructions for the same bytes (the address unit,
+// not HBM, bounds the 4-byte form).  Same block mapping, modes and LDS
+// column table as k_pass_dw.  A quad that straddles DimUp loads rows of the
+// column's padding (inside the column) and stores only rows < DimUp.
+template <typename T, int R, int MODE, int NCOL, int KCH>
+__global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
+                                                   const T* __restrict__ tin, T* outp, LzChain A, int nchunk) {
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int LPS = R / VW;
+  constexpr int GPW = 64 / LPS;
+  typedef T VT __attribute__((ext_vector_type(VW)));
+  __shared__ T lut[256];
+  __shared__ double sh[16];
+  LzScalars s{};
+  const T* __restrict__ q = qa;
+  T* __restrict__ out = outp;
+  if (MODE == 1) {
+    s = *A.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+    out = (s.k & 1) ? qa : qb;
+  }
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_dw[k];
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  const int b = blockIdx.x;
+  const int xcd = b % 8, ib = b / 8;
+  const int jb = ib % nchunk;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / LPS, l = lane % LPS;
+  const int nwaves = blockDim.x >> 6;
+  const int npan = (dimup + R - 1) / R;
+  const int p = 8 * (ib / nchunk) + xcd;
+  const int cb = (int)((int64_t)jb * dimdw / nchunk), ce = (int)((int64_t)(jb + 1) * dimdw / nchunk);
+  extern __shared__ __align__(16) unsigned char dsm[];
+  uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);
+  const int ncol = ce - cb;
+  const int w0 = V.off_dw[cb], nw = V.off_dw[ce] - w0;
+  int* toff = reinterpret_cast<int*>(tbl + nw);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = __builtin_nontemporal_load(&V.csr_dw[w0 + i]);
+  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = __builtin_nontemporal_load(&V.off_dw[cb + c]) - w0;
+  __syncthreads();
+  const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
+  const T ts = (T)inv_s;
+  const T bo = (MODE == 1) ? (T)(s.beta_k / s.s_old) : (T)0;
+  double part = 0.0;
+  if (p < npan) {
+    const int iup0 = p * R + l * VW;
+    const bool ok = iup0 < dimup;
+    const int iuc = ok ? iup0 : 0;
+    const int nval = ok ? min(VW, dimup - iup0) : 0;  // rows of the quad below DimUp
+    const bool lead = ok && g == 0;
+    for (int c0 = cb + wave; c0 < ce; c0 += NCOL * nwaves) {
+      int col[NCOL], nh[NCOL];
+      bool okc[NCOL];
+      VT t0[NCOL], o0[NCOL], q0v[NCOL], acc[NCOL];
+      int nmax = 0;
+#pragma unroll
+      for (int h = 0; h < NCOL; ++h) {
+        col[h] = c0 + h * nwaves;
+        okc[h] = col[h] < ce;
+        if (!okc[h]) col[h] = c0;
+        nh[h] = okc[h] ? toff[col[h] - cb + 1] - toff[col[h] - cb] : 0;
+        nmax = nh[h] > nmax ? nh[h] : nmax;
+        const size_t i = (size_t)col[h] * ld + iuc;
+        const bool live = lead && okc[h];
+        t0[h] = live ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(&tin[i])) : (VT)0;
+        o0[h] = (MODE == 1 && live) ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(&out[i])) : (VT)0;
+        q0v[h] = (MODE == 1 && live) ? *reinterpret_cast<const VT*>(&q[i]) : (VT)0;
+        acc[h] = (VT)0;
+      }
+      for (int k0 = 0; k0 < nmax; k0 += GPW * KCH) {
+        uint32_t w[NCOL][KCH];
+#pragma unroll
+        for (int h = 0; h < NCOL; ++h) {
+          const uint32_t* tw = tbl + toff[col[h] - cb];
+#pragma unroll
+          for (int j = 0; j < KCH; ++j) {
+            const int k = k0 + g + GPW * j;
+            w[h][j] = (k < nh[h]) ? tw[k] : ((uint32_t)col[h] | (255u << 24));
+          }
+        }
+        VT x[NCOL][KCH];
+#pragma unroll
+        for (int h = 0; h < NCOL; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j) {
+            const uint32_t nbr = w[h][j] & 0xFFFFFFu;
+            x[h][j] = (k0 + g + GPW * j < nh[h]) ? *reinterpret_cast<const VT*>(&q[(size_t)nbr * ld + iuc]) : (VT)0;
+          }
+#pragma unroll
+        for (int h = 0; h < NCOL; ++h)
+#pragma unroll
+          for (int j = 0; j < KCH; ++j) acc[h] += lut[w[h][j] >> 24] * x[h][j];
+      }
+#pragma unroll
+      for (int h = 0; h < NCOL; ++h)
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+          T v = acc[h][e];
+#pragma unroll
+          for (int m = LPS; m < 64; m *= 2) v += __shfl_xor(v, m);
+          acc[h][e] = v;
+        }
+      if (lead) {
+#pragma unroll
+        for (int h = 0; h < NCOL; ++h) {
+          if (!okc[h]) continue;
+          const size_t i = (size_t)col[h] * ld + iup0;
+          VT tv;
+          if (MODE == 0) {
+            tv = t0[h] + acc[h];
+          } else {
+            tv = (t0[h] + acc[h]) * ts - bo * o0[h];
+          }
+          if (nval == VW) {
+            __builtin_nontemporal_store(tv, reinterpret_cast<VT*>(&out[i]));
+            if (MODE == 1)
+#pragma unroll
+              for (int e = 0; e < VW; ++e) part += (double)tv[e] * (double)q0v[h][e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+              if (e < nval) {
+                out[i + e] = tv[e];
+                if (MODE == 1) part += (double)tv[e] * (double)q0v[h][e];
+              }
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 1) {
+    const double v = block_sum(part * inv_s, sh);
+    if (threadIdx.x == 0) A.part_dw[b] = v;
+  }
+}
+
 // ------------------------------------------------------ chain reductions
 // One block each, between the passes: the kernel boundary makes the block
 // partials visible to it (a fence inside the passes would write back and
@@ -546,14 +690,77 @@ static void launch_dw_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
   EDL_HIP(hipGetLastError());
 }
 
+// Dynamic LDS of a down-pass block: the chunk's hop words and their
+// offsets, bounded by ncmax * (kdw + 1) words, ncmax = ceil(dimdw / nchunk) + 1.
+size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg) {
+  const size_t ncmax = (size_t)((V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1);
+  return sizeof(uint32_t) * ncmax * (size_t)(V.kdw + 1);
+}
+
+template <typename T, int R, int MODE, int NCOL, int KCH>
+static void launch_dwv_k(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
+                         const TwoPassCfg& cfg, size_t lds, int grid, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_dwv<T, R, MODE, NCOL, KCH>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_pass_dwv<T, R, MODE, NCOL, KCH>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
+                     tin, out, A, cfg.nchunk);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int MODE, int NCOL, int KCH>
+static void launch_dwv_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
+                         const TwoPassCfg& cfg, size_t lds, int grid, hipStream_t st) {
+  switch (cfg.R) {
+    case 16: return launch_dwv_k<T, 16, MODE, NCOL, KCH>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 32: return launch_dwv_k<T, 32, MODE, NCOL, KCH>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 64: return launch_dwv_k<T, 64, MODE, NCOL, KCH>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 128: return launch_dwv_k<T, 128, MODE, NCOL, KCH>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+    case 256:
+      if constexpr (sizeof(T) == 4)
+        return launch_dwv_k<T, 256, MODE, NCOL, KCH>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      [[fallthrough]];
+    default: throw Error("launch_pass_dw: unsupported panel height of the vector-gather form");
+  }
+}
+
+bool pass_dw_vec_ok(const TwoPassView& V, const TwoPassCfg& cfg, size_t elem, const void* const* bufs, int nbufs) {
+  const bool rok = cfg.R == 16 || cfg.R == 32 || cfg.R == 64 || cfg.R == 128 || (cfg.R == 256 && elem == 4);
+  if (!cfg.dw_vec || !rok || V.ld % 32 != 0) return false;
+  for (int i = 0; i < nbufs; ++i)
+    if (bufs[i] && reinterpret_cast<uintptr_t>(bufs[i]) % 16 != 0) return false;
+  return true;
+}
+
 template <typename T, int MODE>
 static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
-                        const TwoPassCfg& cfg, hipStream_t st) {
+                        const TwoPassCfg& cfg_in, hipStream_t st) {
+  TwoPassCfg cfg = cfg_in;
+  const void* bufs[4] = {qa, qb, tin, out};
+  if (cfg.dw_vec && !pass_dw_vec_ok(V, cfg, sizeof(T), bufs, 4)) {
+    // natural layout (ld = DimUp) or unaligned caller buffers: the 4-byte form
+    cfg.dw_vec = 0;
+    cfg.ncol = 2;
+    cfg.kch = 8;
+    if (cfg.R != 32 && cfg.R != 64) cfg.R = 64;
+  }
   EDL_CHECK(cfg.nchunk >= 1 && cfg.nchunk <= V.dimdw, "launch_pass_dw: bad column chunk count");
-  const int ncol = (V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1;
-  const size_t lds = sizeof(uint32_t) * (size_t)ncol * (V.kdw + 1);
+  const size_t lds = pass_dw_lds_bytes(V, cfg);
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 4096, "launch_pass_dw: column table exceeds LDS");
   const int grid = pass_dw_grid(V, cfg);
+  if (cfg.dw_vec) {
+    switch (cfg.ncol * 100 + cfg.kch) {
+      case 102: return launch_dwv_n<T, MODE, 1, 2>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 104: return launch_dwv_n<T, MODE, 1, 4>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 201: return launch_dwv_n<T, MODE, 2, 1>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 202: return launch_dwv_n<T, MODE, 2, 2>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 204: return launch_dwv_n<T, MODE, 2, 4>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      default: throw Error("launch_pass_dw: unsupported columns per wave / hop chunk of the vector-gather form");
+    }
+  }
   const int code = cfg.ncol * 100 + cfg.kch;
   switch (code) {
     case 108: return launch_dw_n<T, MODE, 1, 8>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);

@@ -52,14 +52,22 @@ struct TwoPassCfg {
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
   int ncol = 1;         // columns per lane in the down pass (1 or 2)
   int kch = 16;         // hop chunk of the down pass (8, 16, 32 with 1 column; 8, 16 with 2)
+  int dw_vec = 0;       // 1: vector-gather down pass (16-byte lanes; R 16|32|64|128, 256 in fp32;
+                        //    (ncol, kch) in (1, 2|4), (2, 1|2|4))
 };
 
 // Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
 // LzChain::part_dw; blocks of an up-pass launch, the size of part_up.
 int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg);
+// Whether a launch with this schedule runs the vector-gather down pass
+// (padded layout and 16-byte aligned buffers; otherwise the 4-byte form).
+bool pass_dw_vec_ok(const TwoPassView& V, const TwoPassCfg& cfg, size_t elem, const void* const* bufs, int nbufs);
+// Dynamic LDS of a down-pass block (column table and offsets).
+size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg);
 int pass_up_grid(const TwoPassView& V, const TwoPassCfg& cfg);
 
 bool two_pass_ok(const Sector& s, size_t elem);
+
 // Column stride of the engine's own vectors: DimUp rounded up to a whole
 // number of 128-byte cache lines of fp32 (32 elements), so that a 32-row
 // fp32 panel segment (16-row fp64) of any column is exactly one line.

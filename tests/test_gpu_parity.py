@@ -180,6 +180,41 @@ def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, monkeypatch):
         assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
 
 
+@pytest.mark.parametrize("rpt,kch,R,dn,dk,vec", [
+    (1, 8, 64, 2, 8, 0), (2, 4, 32, 1, 16, 0), (2, 4, 64, 2, 4, 1), (1, 8, 64, 1, 2, 1), (1, 16, 32, 2, 1, 1),
+    (1, 8, 16, 2, 2, 1), (2, 8, 128, 1, 4, 1), (2, 4, 256, 2, 2, 1)])
+def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, monkeypatch):
+    """Schedules pinned through the override, among them the vector-gather
+    down pass (16-byte lanes, panels of 16 to 256 rows; the 8-site ring's
+    DimUp 70 leaves a quad that straddles DimUp): H x v on three sectors and a
+    mixed-precision ground state (the chain-mode passes) against the
+    oracle."""
+    tail = f"{R},256,32,{dn},{dk},{rpt},{kch},{vec}"
+    tail64 = f"{min(R, 128)},256,32,{dn},{dk},{rpt},{kch},{vec}"  # 256-row panels are fp32 only
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", "3,1024,256," + tail)
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", "1,512,512," + tail64)
+    for cfg, sec in [("ring8", (4, 4)), ("rect3x4", (7, 6)), ("rect3x4", (6, 5))]:
+        c = lattice.CONFIGS[cfg]
+        m = _use(ed, orc, c["H"](), [c["U"]])
+        isec = ed.sector_index(*sec)
+        dim, du, _ = ed.build_sector(isec)
+        print(f"pinned schedule {tail}: {cfg} {sec} DimUp {du}", flush=True)
+        for prec in ("fp32", "fp64"):
+            sched = ed.bench_passes(isec, prec, 1)[2]
+            rows = 128 if (R == 256 and prec == "fp64") else R
+            assert (sched["dw_rows"], sched["dw_vector_gather"], sched["up_rows_per_thread"]) == (rows, vec, rpt)
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    Hc = orc.hamiltonian_csr(m, 6, 6)
+    e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=200, refine_tol=1e-13)
+    assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
+
+
 def test_lanczos_segmented_rect3x4_vs_oracle(ed, orc, monkeypatch):
     """Ground state through the segmented step (phase B, beta, segmented up,
     down, alpha): FP64 E0 against the oracle's Lanczos from the same start
