@@ -214,6 +214,7 @@ def main():
                      "traffic_source": tsrc},
         "kernels_fp32": kern,
         "schedule_fp32": sched,
+        "schedule_fp64": sched64,
         "hxv_algorithmic_gbs_fp32": gbs_hxv,
         "wall_s_timed": wall,
         "hxv_per_s_fp64": 1e3 / ms64,
