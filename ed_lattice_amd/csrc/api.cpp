@@ -309,7 +309,11 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
       best.C = C;
     }
   }
-  const int shapes[4][3] = {{64, 256, 256}, {64, 1024, 64}, {64, 512, 128}, {32, 256, 256}};
+  // {R, threads, chunks, vector gathers}: 4-byte gathers with 2 columns per
+  // lane and hop chunk 8, vector gathers with 2 columns per wave and hop
+  // chunk 2 per lane group
+  const int shapes[8][4] = {{64, 256, 256, 0}, {64, 1024, 64, 0}, {64, 512, 128, 0}, {32, 256, 256, 0},
+                            {64, 1024, 128, 1}, {128, 1024, 128, 1}, {128, 1024, 256, 1}, {32, 1024, 128, 1}};
   double bd = 1e300;
   TwoPassCfg dw = best;
   for (const auto& sh : shapes) {
@@ -317,6 +321,10 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
     w.R = sh[0];
     w.threads_dw = sh[1];
     w.nchunk = std::max(1, std::min(sh[2], s.sp[1].dim));
+    w.dw_vec = sh[3];
+    w.ncol = 2;
+    w.kch = sh[3] ? 2 : 8;
+    if (pass_dw_lds_bytes(V, w) > (size_t)(160 * 1024 - 4096)) continue;
     double ms = 0.0;
     for (int r = 0; r < 2; ++r) {
       EventTimer tm(st);
@@ -324,8 +332,8 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
       ms = tm.stop();
     }
     if (verbose())
-      std::fprintf(stderr, "[edl] segmented sector down pass R %d x %d threads, %d chunks: %.3f ms\n", w.R,
-                   w.threads_dw, w.nchunk, ms);
+      std::fprintf(stderr, "[edl] segmented sector down pass vec %d R %d x %d threads, %d chunks: %.3f ms\n",
+                   w.dw_vec, w.R, w.threads_dw, w.nchunk, ms);
     if (ms < bd) {
       bd = ms;
       dw = w;

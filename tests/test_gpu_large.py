@@ -41,3 +41,28 @@ def test_obc3x6_full_size(orc):
         ref = orc.hxv_rows_real64(m, 9, 9, rows, vh)
         scale = np.max(np.abs(hvh[rows]))
         assert np.max(np.abs(hvh[rows] - ref)) < 5e-6 * scale
+
+
+def test_obc3x6_chain_layout_first_step():
+    """The chain's own layout (padded column stride: segmented up pass and the
+    down pass the tuner picks for it, Lanczos terms fused) against the
+    natural-layout operator at full size: for the start vector v (the fp64
+    random vector of the same seed), alpha_0 = <v,Hv>/<v,v> and
+    beta_1^2 = <Hv,Hv>/<v,v> - alpha_0^2."""
+    H = lattice.disordered_rect(3, 6)
+    with EDLattice(0) as ed:
+        ed.set_hamiltonian(H, [4.0])
+        isec = ed.sector_index(9, 9)
+        dim, _, _ = ed.build_sector(isec)
+        nb = dim * 8
+        v, hv = ed.alloc(nb), ed.alloc(nb)
+        ed.vec_random("fp64", v, dim, 0x5EED)
+        ed.hxv_device(isec, "fp64", v, hv)
+        vv, vhv, hh = ed.vec_dot("fp64", v, v, dim), ed.vec_dot("fp64", v, hv, dim), ed.vec_dot("fp64", hv, hv, dim)
+        a0 = vhv / vv
+        b1 = np.sqrt(hh / vv - a0 * a0)
+        r = ed.lanczos_eigh(isec, "fp64", niter=3, seed=0x5EED)
+        print(f"obc3x6 chain alpha0 {r['alpha'][0]:.15f} ref {a0:.15f} beta1 {r['beta'][1]:.15f} ref {b1:.15f}",
+              flush=True)
+        assert abs(r["alpha"][0] - a0) < 1e-11 * np.sqrt(hh / vv)
+        assert abs(r["beta"][1] - b1) < 1e-9 * b1
