@@ -82,12 +82,28 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   __syncthreads();
-  for (int r = threadIdx.x; r < dimup; r += blockDim.x) {
-    VT x;
+  // SB rows per thread at a time (about 32 registers of loads), all their
+  // loads in flight before the LDS writes: one HBM round trip per SB rows
+  constexpr int W = (int)(sizeof(T) / 4);
+  constexpr int SB = 32 / (C * W) > 16 ? 16 : (32 / (C * W) < 2 ? 2 : 32 / (C * W));
+  for (int r0 = threadIdx.x; r0 < dimup; r0 += SB * blockDim.x) {
+    T a[SB][C];
 #pragma unroll
-    for (int j = 0; j < C; ++j)
-      Vec<T, C>::set(x, j, (c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0);
-    P[r] = x;
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        a[h][j] = (r < dimup && c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0;
+    }
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+      if (r >= dimup) break;
+      VT x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, a[h][j]);
+      P[r] = x;
+    }
   }
   __syncthreads();
   up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, c0, ts);
@@ -164,9 +180,9 @@ static __host__ __device__ constexpr size_t lds_head_bytes(size_t sz) { return 2
 // the input is q.  With phase B of step k pending it is r = t - am * q_k
 // (t read from qn, r stored back over it unless `keep_t`), and the block's
 // share of |r|^2 is returned; x (natural layout) accumulates cx * q_k for
-// the eigenvector replay when given.  Two rows per thread, every load of
-// both rows in flight first.
-template <typename T, int C>
+// the eigenvector replay when given.  SB rows per thread at a time, every
+// load of those rows in flight first.
+template <typename T, int C, int SB = 2>
 static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typename Vec<T, C>::type* P, T* qn,
                                                      const T* __restrict__ qk, int c0, bool pend, bool keep_t,
                                                      T am, double* __restrict__ x, double cx) {
@@ -174,11 +190,13 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   double nrm = 0.0;
-  for (int r0 = threadIdx.x; r0 < dimup; r0 += 2 * blockDim.x) {
-    const int rr[2] = {r0, r0 + (int)blockDim.x};
-    T a[2][C], b[2][C];
+  for (int r0 = threadIdx.x; r0 < dimup; r0 += SB * blockDim.x) {
+    int rr[SB];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < SB; ++h) rr[h] = r0 + h * (int)blockDim.x;
+    T a[SB][C], b[SB][C];
+#pragma unroll
+    for (int h = 0; h < SB; ++h)
 #pragma unroll
       for (int j = 0; j < C; ++j) {
         const bool ok = rr[h] < dimup && c0 + j < dimdw;
@@ -187,7 +205,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
         b[h][j] = (ok && pend) ? qk[i] : (T)0;
       }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < SB; ++h) {
       if (rr[h] >= dimup) break;
       VT v;
 #pragma unroll
@@ -241,7 +259,10 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   double nrm = 0.0;
   for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
     __syncthreads();
-    nrm += stage_panel<T, C>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
+    // two loads per element with phase B pending: about 32 registers of loads
+    constexpr int W = (int)(sizeof(T) / 4);
+    constexpr int SB = 16 / (C * W) > 8 ? 8 : (16 / (C * W) < 2 ? 2 : 16 / (C * W));
+    nrm += stage_panel<T, C, SB>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
     __syncthreads();
     if (!last) up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, pan * C, (T)1);
   }
