@@ -50,7 +50,7 @@ __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chai
   const T* __restrict__ qk = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
   const T am = (T)(s.alpha_k / s.s_cur);
   __syncthreads();
-  const double nrm = stage_panel<T, C>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
+  const double nrm = stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
   __syncthreads();
   up_rows<T, C, RPT, UKCH>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
   if (!pend) return;

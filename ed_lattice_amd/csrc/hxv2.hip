@@ -54,6 +54,16 @@ struct Vec<T, 4> {
   static __device__ __forceinline__ void set(type& v, int j, T x) { v[j] = x; }
 };
 
+// Rows per thread of a panel-staging batch: as many as about `regs` 32-bit
+// registers of loads hold (`per` loads per element), 2 to `cap`.  All loads
+// of a batch are issued before its LDS writes, so staging a panel costs one
+// memory round trip per batch.
+template <typename T, int C>
+constexpr int stage_rows(int regs, int per, int cap) {
+  const int n = regs / (C * per * (int)(sizeof(T) / 4));
+  return n > cap ? cap : (n < 2 ? 2 : n);
+}
+
 static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t su, uint32_t sd) {
   const uint32_t both = su & sd;
   double e = 0.0;
@@ -82,10 +92,7 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   __syncthreads();
-  // SB rows per thread at a time (about 32 registers of loads), all their
-  // loads in flight before the LDS writes: one HBM round trip per SB rows
-  constexpr int W = (int)(sizeof(T) / 4);
-  constexpr int SB = 32 / (C * W) > 16 ? 16 : (32 / (C * W) < 2 ? 2 : 32 / (C * W));
+  constexpr int SB = stage_rows<T, C>(32, 1, 16);
   for (int r0 = threadIdx.x; r0 < dimup; r0 += SB * blockDim.x) {
     T a[SB][C];
 #pragma unroll
@@ -259,10 +266,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   double nrm = 0.0;
   for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
     __syncthreads();
-    // two loads per element with phase B pending: about 32 registers of loads
-    constexpr int W = (int)(sizeof(T) / 4);
-    constexpr int SB = 16 / (C * W) > 8 ? 8 : (16 / (C * W) < 2 ? 2 : 16 / (C * W));
-    nrm += stage_panel<T, C, SB>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
+    nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
     __syncthreads();
     if (!last) up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, pan * C, (T)1);
   }
@@ -829,11 +833,24 @@ __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, c
   const int c0 = pan * C;
   const size_t ld = (size_t)V.ld;
   for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
-  for (int r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-    VT x;
+  constexpr int SB = stage_rows<T, C>(32, 1, 16);
+  for (int r0 = lo + threadIdx.x; r0 < hi; r0 += SB * blockDim.x) {
+    T a[SB][C];
 #pragma unroll
-    for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, (c0 + j < dimdw) ? q[(size_t)(c0 + j) * ld + r] : (T)0);
-    P[r - lo] = x;
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) a[h][j] = (r < hi && c0 + j < dimdw) ? q[(size_t)(c0 + j) * ld + r] : (T)0;
+    }
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+      if (r >= hi) break;
+      VT x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, a[h][j]);
+      P[r - lo] = x;
+    }
   }
   __syncthreads();
   uint32_t sd[C];
