@@ -727,6 +727,24 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     tps.L.x = x;
     tps.L.ccoef = ccoef;
   };
+  // Stored basis (two-pass sectors, no deflation, when nmax vectors fit in
+  // 70 % of the free device memory; EDL_LANCZOS_BASIS=0 turns it off): the
+  // up pass copies each q_k it reads for phase B, and the eigenvector is one
+  // pass over the stored vectors instead of a second run of the chain.
+  const size_t vbytes = sizeof(T) * (size_t)ld * S.sp[1].dim;
+  bool use_basis = false;
+  if (d_x && two && !seg && nX == 0) {
+    const char* e = std::getenv("EDL_LANCZOS_BASIS");
+    size_t free_b = 0, total_b = 0;
+    EDL_HIP(hipMemGetInfo(&free_b, &total_b));
+    use_basis = !(e && e[0] == '0') && (double)vbytes * nmax <= 0.7 * (double)free_b;
+  }
+  DevBuf basis, bscale;
+  if (use_basis) {
+    basis = DevBuf(vbytes * (size_t)nmax);
+    bscale = DevBuf(sizeof(double) * (size_t)nmax);
+  }
+  T* last_q = nullptr;
   {
     ChainSet<T> cs;
     ChainAlloc<T> A;
@@ -736,12 +754,38 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
     cs.h.push_back(cd);
     chainset_finalize(c, cs, sch);
     bind(cd, nmax, nullptr, nullptr);
+    tps.L.basis = use_basis ? basis.p : nullptr;
+    tps.L.bscale = use_basis ? bscale.as<double>() : nullptr;
     r.hxv += run_steps(c, cs, nmax, check, tol, st, two ? &tps : nullptr);
+    tps.L.basis = nullptr;
+    tps.L.bscale = nullptr;
     r.out = read_chain<T>(cs.h[0], nmax, st);
+    if (use_basis && r.out.nsteps > 0) {
+      // q_{m-1} and its norm (the scalars keep the last completed step) are
+      // still in the chain's buffers
+      const int m = r.out.nsteps;
+      last_q = static_cast<T*>(((m - 1) & 1) ? cd.q1 : cd.q0);
+      EDL_HIP(hipMemcpyAsync(static_cast<char*>(basis.p) + vbytes * (size_t)(m - 1), last_q, vbytes,
+                             hipMemcpyDeviceToDevice, st));
+      EDL_HIP(hipMemcpyAsync(bscale.as<double>() + (m - 1), &cd.sc->s_cur, sizeof(double), hipMemcpyDeviceToDevice,
+                             st));
+      EDL_HIP(hipStreamSynchronize(st));
+    }
   }
   std::vector<double> cvec;
   r.e0 = lowest_ritz(r.out, d_x ? &cvec : nullptr);
-  if (d_x) {
+  if (d_x && use_basis && r.out.nsteps > 0) {
+    const int m = r.out.nsteps;
+    std::vector<double> sc(m), w(m);
+    EDL_HIP(hipMemcpy(sc.data(), bscale.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    for (int k = 0; k < m; ++k) w[k] = cvec[k] / sc[k];
+    DevBuf dw(sizeof(double) * m);
+    EDL_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+    launch_basis_combine<T>(static_cast<const T*>(basis.p), tps.V, dw.as<double>(), m, d_x, st);
+    const double n2 = device_dot(c, d_x, d_x, S.dim, st);
+    launch_scale(d_x, S.dim, 1.0 / std::sqrt(n2), st);
+    EDL_HIP(hipStreamSynchronize(st));
+  } else if (d_x) {
     const int m = r.out.nsteps;
     DevBuf dc(sizeof(double) * m);
     EDL_HIP(hipMemcpyAsync(dc.p, cvec.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));

@@ -187,12 +187,14 @@ static __host__ __device__ constexpr size_t lds_head_bytes(size_t sz) { return 2
 // the input is q.  With phase B of step k pending it is r = t - am * q_k
 // (t read from qn, r stored back over it unless `keep_t`), and the block's
 // share of |r|^2 is returned; x (natural layout) accumulates cx * q_k for
-// the eigenvector replay when given.  SB rows per thread at a time, every
-// load of those rows in flight first.
+// the eigenvector replay when given, and q_k is copied to `bstore` (same
+// layout) when given.  SB rows per thread at a time, every load of those
+// rows in flight first.
 template <typename T, int C, int SB = 2>
 static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typename Vec<T, C>::type* P, T* qn,
                                                      const T* __restrict__ qk, int c0, bool pend, bool keep_t,
-                                                     T am, double* __restrict__ x, double cx) {
+                                                     T am, double* __restrict__ x, double cx,
+                                                     T* __restrict__ bstore = nullptr) {
   typedef typename Vec<T, C>::type VT;
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
@@ -221,6 +223,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
         if (pend && c0 + j < dimdw) {
           e = a[h][j] - am * b[h][j];
           if (!keep_t) qn[(size_t)(c0 + j) * ld + rr[h]] = e;
+          if (bstore) bstore[(size_t)(c0 + j) * ld + rr[h]] = b[h][j];
           nrm += (double)e * (double)e;
           if (x) x[(size_t)(c0 + j) * dimup + rr[h]] += cx * (double)b[h][j];
         }
@@ -263,10 +266,13 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   const T am = (T)(s.alpha_k / s.s_cur);
   double* x = pend ? L.x : nullptr;
   const double cx = x ? L.ccoef[s.k] / s.s_cur : 0.0;
+  // stored basis: q_k (read for phase B) goes to basis[k] with its norm
+  T* bst = (pend && L.basis) ? static_cast<T*>(L.basis) + (size_t)s.k * V.ld * V.dimdw : nullptr;
+  if (bst && blockIdx.x == 0 && threadIdx.x == 0) L.bscale[s.k] = s.s_cur;
   double nrm = 0.0;
   for (int pan = blockIdx.x; pan < npan; pan += gridDim.x) {
     __syncthreads();
-    nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx);
+    nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx, bst);
     __syncthreads();
     if (!last) up_rows<T, C, RPT, UKCH>(V, I, lut, P, tout, pan * C, (T)1);
   }
@@ -597,6 +603,43 @@ __global__ __launch_bounds__(256) void k_lz_alpha(LzChain L, int nparts) {
     *L.sc = n;
   }
 }
+
+// ------------------------------------------------ eigenvector from a basis
+// x (natural layout, fp64) = sum_k w_k q_k over m stored vectors with column
+// stride ld, k in step order (the order in which the replay adds them).
+template <typename T>
+__global__ __launch_bounds__(256) void k_basis_combine(const T* __restrict__ basis, size_t stride,
+                                                       const double* __restrict__ w, int m, int dimup, int64_t n,
+                                                       int ld, double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t idw = i / dimup;
+    const size_t p = (size_t)idw * ld + (size_t)(i - idw * dimup);
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 8 <= m; k += 8) {
+      T v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = basis[(size_t)(k + j) * stride + p];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc += w[k + j] * (double)v[j];
+    }
+    for (; k < m; ++k) acc += w[k] * (double)basis[(size_t)k * stride + p];
+    x[i] = acc;
+  }
+}
+
+template <typename T>
+void launch_basis_combine(const T* basis, const TwoPassView& V, const double* w, int m, double* x, hipStream_t st) {
+  const int64_t n = (int64_t)V.dimup * V.dimdw;
+  const size_t stride = (size_t)V.ld * V.dimdw;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_basis_combine<T>, dim3((unsigned)blocks), dim3(256), 0, st, basis, stride, w, m, V.dimup, n,
+                     V.ld, x);
+  EDL_HIP(hipGetLastError());
+}
+template void launch_basis_combine<double>(const double*, const TwoPassView&, const double*, int, double*,
+                                           hipStream_t);
+template void launch_basis_combine<float>(const float*, const TwoPassView&, const double*, int, double*, hipStream_t);
 
 // ---------------------------------------------------------------- host side
 static int lds_limit_bytes() { return 160 * 1024; }

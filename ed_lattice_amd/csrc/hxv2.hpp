@@ -37,6 +37,8 @@ struct LzChain {
   double* part_dw;        // [pass_dw_grid]
   double* x;              // fp64 replay accumulator (natural layout), or null
   const double* ccoef;    // replay coefficients c_k, or null
+  void* basis;            // [nmax][ld * DimDw] stored q_k (two-pass up pass), or null
+  double* bscale;         // [nmax] their norms s_k
   int nmax;
   double brk_tol;         // relative breakdown threshold on beta
 };
@@ -67,6 +69,10 @@ size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg);
 int pass_up_grid(const TwoPassView& V, const TwoPassCfg& cfg);
 
 bool two_pass_ok(const Sector& s, size_t elem);
+// x (natural layout, fp64) = sum_{k<m} w_k basis_k for m vectors stored with
+// the view's column stride (basis_k at k * ld * DimDw).
+template <typename T>
+void launch_basis_combine(const T* basis, const TwoPassView& V, const double* w, int m, double* x, hipStream_t st);
 
 // Column stride of the engine's own vectors: DimUp rounded up to a whole
 // number of 128-byte cache lines of fp32 (32 elements), so that a 32-row

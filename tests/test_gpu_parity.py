@@ -283,6 +283,33 @@ def test_lanczos_gs_ring8_vs_dense(ed, orc):
         np.testing.assert_allclose(obs[k], ref[k], atol=1e-12)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_ground_state_vector_stored_basis_equals_replay(ed, orc, prec, monkeypatch):
+    """The eigenvector from the stored Lanczos basis (one combination pass)
+    equals the one from the second run of the chain (EDL_LANCZOS_BASIS=0),
+    and both equal the dense ground state."""
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    w, V = np.linalg.eigh(orc.hamiltonian_csr(m, 4, 4).toarray().real)
+    r1 = ed.lanczos_eigh(isec, prec, niter=120, want_vector=True)
+    monkeypatch.setenv("EDL_LANCZOS_BASIS", "0")
+    r0 = ed.lanczos_eigh(isec, prec, niter=120, want_vector=True)
+    assert r1["e0"] == r0["e0"]
+    assert np.max(np.abs(r1["gs"] - r0["gs"])) < 1e-12
+    tol = 1e-9 if prec != "fp32" else 1e-5
+    assert 1 - abs(r1["gs"] @ V[:, 0]) < tol
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    monkeypatch.delenv("EDL_LANCZOS_BASIS")
+    ed.lanczos_eigh(isec, prec, niter=150)
+    o1 = ed.observables()
+    monkeypatch.setenv("EDL_LANCZOS_BASIS", "0")
+    ed.lanczos_eigh(isec, prec, niter=150)
+    o0 = ed.observables()
+    for k in ("dens_up", "dens_dw", "docc"):
+        np.testing.assert_allclose(o1[k], o0[k], atol=1e-12)
+
+
 @pytest.mark.parametrize("cfg", ["ring8", "rect3x4"])
 def test_mixed_precision_e0(ed, orc, cfg):
     c = lattice.CONFIGS[cfg]
