@@ -60,10 +60,12 @@ struct SpinTable {
   float* coef32 = nullptr;     // same, rounded to fp32
   double* diag = nullptr;      // [dim] one-spin part of the diagonal
   // Packed table: [kmax][dim] words nbr | cid << 24, with the coefficient
-  // lut[cid] = Hij(pair) * sign (cid = 2*pair + (sign < 0), 255 = padding).
-  // Available when dim < 2^24 and there are at most 127 hopping pairs.
+  // lut[cid] = h * sign, h the v-th distinct nonzero Hij value of the spin
+  // (cid = 2*v + (sign < 0), 255 = padding).  Available when dim < 2^24 and
+  // there are at most 127 distinct hopping values.
   uint32_t* packed = nullptr;
   bool packed_ok = false;
+  int nvals = 0;               // distinct hopping values (cids 0 .. 2*nvals-1)
   double lut64[256] = {};
   float lut32[256] = {};
   int32_t* cnt = nullptr;      // [dim] number of hops of each string

@@ -6,6 +6,7 @@ namespace edl {
 
 struct TwoPassView {
   int dimup, dimdw, kup, kdw;
+  int nv_up;               // distinct up-spin hopping values (1: coefficients +-h, cids 0 and 1)
   int ld;                  // column stride of the vectors (>= dimup)
   const int32_t* map_up;
   const int32_t* map_dw;

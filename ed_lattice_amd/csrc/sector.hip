@@ -63,6 +63,7 @@ __global__ void k_unrank(int32_t* map, int dim, int N, int Ns) {
 
 struct HopPair {
   uint8_t p, q;
+  uint8_t v;  // index of h among the spin's distinct hopping values
   double h;
 };
 
@@ -106,7 +107,7 @@ __global__ void k_hop_fill(const int32_t* map, int dim, const HopPair* pairs, in
       nbr[o] = r;
       coef64[o] = v;
       coef32[o] = (float)v;
-      if (packed) packed[o] = ((uint32_t)r & 0xFFFFFFu) | ((uint32_t)(2 * k + (sg < 0.0 ? 1 : 0)) << 24);
+      if (packed) packed[o] = ((uint32_t)r & 0xFFFFFFu) | ((uint32_t)(2 * pairs[k].v + (sg < 0.0 ? 1 : 0)) << 24);
       ++c;
     }
   }
@@ -167,12 +168,18 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
   // (HxV_dw.f90:15), Hij(1,...) for the up spin (HxV_up.f90:16)
   const int s_idx = (spin == 0) ? 0 : (M.Nspin - 1);
   std::vector<HopPair> pairs;
+  std::vector<double> vals;  // distinct hopping values, in order of appearance
   for (int p = 0; p < Ns; ++p)
     for (int q = 0; q < Ns; ++q) {
       if (p == q) continue;
       double h = M.Hij[(size_t)s_idx + (size_t)M.Nspin * (p + (size_t)Ns * q)];
-      if (h != 0.0) pairs.push_back(HopPair{(uint8_t)p, (uint8_t)q, h});
+      if (h == 0.0) continue;
+      size_t v = 0;
+      while (v < vals.size() && vals[v] != h) ++v;
+      if (v == vals.size()) vals.push_back(h);
+      pairs.push_back(HopPair{(uint8_t)p, (uint8_t)q, (uint8_t)(v < 255 ? v : 255), h});
     }
+  t.nvals = (int)vals.size();
   HopPair* d_pairs = nullptr;
   int32_t* d_cnt = nullptr;
   const int np = (int)pairs.size();
@@ -190,13 +197,13 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
   EDL_HIP(hipMalloc(&t.nbr, sizeof(int32_t) * n));
   EDL_HIP(hipMalloc(&t.coef64, sizeof(double) * n));
   EDL_HIP(hipMalloc(&t.coef32, sizeof(float) * n));
-  t.packed_ok = (np <= 127) && (t.dim < (1 << 24));
+  t.packed_ok = (t.nvals <= 127) && (t.dim < (1 << 24));
   if (t.packed_ok) {
     EDL_HIP(hipMalloc(&t.packed, sizeof(uint32_t) * n));
     for (int k = 0; k < 256; ++k) t.lut64[k] = 0.0;
-    for (int k = 0; k < np; ++k) {
-      t.lut64[2 * k] = pairs[k].h;
-      t.lut64[2 * k + 1] = -pairs[k].h;
+    for (int v = 0; v < t.nvals; ++v) {
+      t.lut64[2 * v] = vals[v];
+      t.lut64[2 * v + 1] = -vals[v];
     }
     for (int k = 0; k < 256; ++k) t.lut32[k] = (float)t.lut64[k];
     EDL_HIP(hipMalloc(&t.d_lut64, sizeof(t.lut64)));
