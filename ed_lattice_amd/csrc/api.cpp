@@ -1852,10 +1852,12 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
   ms_pass[1] = (two || seg) ? dw / reps : 0.0;
-  const int32_t out[13] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+  // the up pass takes its coefficients from registers in fp32 on one-value lattices
+  const int32_t lreg = (two && sizeof(T) == 4 && s.sp[0].nvals == 1) ? 1 : 0;
+  const int32_t out[14] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
-                           cf.dw_vec};
-  for (int k = 0; k < 13; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
+                           cf.dw_vec,                lreg};
+  for (int k = 0; k < 14; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

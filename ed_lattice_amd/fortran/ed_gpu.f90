@@ -254,7 +254,7 @@ module ED_GPU
        type(c_ptr), value :: ctx
        integer(c_int), value :: isector, prec, reps
        real(c_double) :: ms_pass(2)
-       integer(c_int32_t) :: cfg(13)
+       integer(c_int32_t) :: cfg(14)
      end function edl_bench_passes
 
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
