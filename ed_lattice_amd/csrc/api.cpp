@@ -66,6 +66,16 @@ struct DevBuf {
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+  // An optional allocation: empty (and no error left pending) when the
+  // device cannot provide it.
+  static DevBuf try_alloc(size_t bytes) {
+    DevBuf b;
+    if (hipMalloc(&b.p, bytes ? bytes : 4) != hipSuccess) {
+      b.p = nullptr;
+      (void)hipGetLastError();
+    }
+    return b;
+  }
 };
 
 // Bump allocator over one device allocation: the buffers of a chain group
@@ -741,8 +751,10 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   }
   DevBuf basis, bscale;
   if (use_basis) {
-    basis = DevBuf(vbytes * (size_t)nmax);
-    bscale = DevBuf(sizeof(double) * (size_t)nmax);
+    // the second run of the chain stays the fallback if the memory is gone
+    basis = DevBuf::try_alloc(vbytes * (size_t)nmax);
+    use_basis = basis.p != nullptr;
+    if (use_basis) bscale = DevBuf(sizeof(double) * (size_t)nmax);
   }
   T* last_q = nullptr;
   {
