@@ -215,6 +215,26 @@ def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, monke
     assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
 
 
+@pytest.mark.parametrize("R", [64, 128])
+def test_hxv_vector_gather_degenerate_sectors(ed, orc, R, monkeypatch):
+    """The vector-gather down pass on sectors where one spin has a single
+    string (DimUp or DimDw = 1, and dim 1) or an odd DimUp (35: quads that
+    straddle DimUp in fp32 and fp64), pinned with one column chunk."""
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"3,1024,256,{R},256,1,2,2,1,8,1")
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", f"1,512,512,{R},256,1,2,2,1,8,1")
+    for H, U, sec in [(lattice.ring(8), 4.0, (0, 4)), (lattice.ring(8), 4.0, (4, 8)), (lattice.ring(8), 4.0, (8, 0)),
+                      (lattice.ring(8), 4.0, (1, 7)), (lattice.ring(7), 3.0, (3, 3))]:
+        m = _use(ed, orc, H, [U])
+        isec = ed.sector_index(*sec)
+        dim, du, dd = ed.build_sector(isec)
+        print(f"vector gather R {R}: sector {sec} DimUp {du} DimDw {dd}", flush=True)
+        assert ed.bench_passes(isec, "fp32", 1)[2]["dw_vector_gather"] == 1
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+
+
 def test_lanczos_segmented_rect3x4_vs_oracle(ed, orc, monkeypatch):
     """Ground state through the segmented step (phase B, beta, segmented up,
     down, alpha): FP64 E0 against the oracle's Lanczos from the same start
