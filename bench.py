@@ -99,8 +99,11 @@ def measured_traffic(case, kernel, sched):
         rec = json.load(f).get(case)
     if not rec or rec.get("kernel") != kernel:
         return None, None
-    same = all(rec.get("schedule", {}).get(k) == v for k, v in sched.items())
-    return rec["bytes_per_launch"], f"profiles/traffic.json[{case}] (schedule {'matches' if same else 'differs'})"
+    # the fields that shape the profiled (down-pass) kernel's memory traffic
+    keys = [k for k in sched if k.startswith("dw_") or k in ("two_pass", "ld")]
+    same = all(rec.get("schedule", {}).get(k) == sched[k] for k in keys)
+    return rec["bytes_per_launch"], (f"profiles/traffic.json[{case}] "
+                                     f"(down-pass schedule {'matches' if same else 'differs'})")
 
 
 def seeds_benchmark(ed, nseq=24):
