@@ -79,12 +79,12 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool L2 = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
 
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool L2 = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false>
 static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
                                                 typename Vec<T, C>::type* P, const T* __restrict__ q,
                                                 T* __restrict__ tout, int c0, T ts) {
@@ -113,16 +113,16 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
     }
   }
   __syncthreads();
-  up_rows<T, C, RPT, UKCH, L2>(V, I, lut, P, tout, c0, ts);
+  up_rows<T, C, RPT, UKCH, LREG>(V, I, lut, P, tout, c0, ts);
 }
 
 // The row loop of a panel already staged in P: RPT rows per thread, hop
 // words fetched UKCH at a time so that RPT*UKCH table loads are in flight
-// before the LDS gathers that depend on them.  L2: the spin has one
+// before the LDS gathers that depend on them.  LREG: the spin has one
 // distinct hopping value, the coefficient of cid 0 / 1 is lut[0] / lut[1]
 // (held in registers) and padding (cid 255) is 0, so a hop costs no LDS
 // read besides its gather.
-template <typename T, int C, int RPT, int UKCH, bool L2>
+template <typename T, int C, int RPT, int UKCH, bool LREG>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts) {
@@ -170,7 +170,7 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
 #pragma unroll
         for (int j = 0; j < UKCH; ++j) {
           const uint32_t cid = w[h][j] >> 24;
-          const T cf = L2 ? (cid == 0u ? l0 : (cid == 1u ? l1 : (T)0)) : lut[cid];
+          const T cf = LREG ? (cid == 0u ? l0 : (cid == 1u ? l1 : (T)0)) : lut[cid];
           const VT x = P[w[h][j] & 0xFFFFFFu];
 #pragma unroll
           for (int c = 0; c < C; ++c) acc[h][c] += cf * Vec<T, C>::get(x, c);
@@ -245,7 +245,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C, int RPT, int UKCH, bool L2 = false>
+template <typename T, int C, int RPT, int UKCH, bool LREG = false>
 __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   if (!chain) {
     for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
-      up_panel<T, C, RPT, UKCH, L2>(V, I, lut, P, qa, tout, pan * C, (T)1);
+      up_panel<T, C, RPT, UKCH, LREG>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
   }
   const LzScalars s = *L.sc;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     __syncthreads();
     nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx, bst);
     __syncthreads();
-    if (!last) up_rows<T, C, RPT, UKCH, L2>(V, I, lut, P, tout, pan * C, (T)1);
+    if (!last) up_rows<T, C, RPT, UKCH, LREG>(V, I, lut, P, tout, pan * C, (T)1);
   }
   if (pend && !last) {
     const double v = block_sum(nrm, sh);
@@ -683,17 +683,17 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   return v;
 }
 
-template <typename T, int C, int RPT, int UKCH, bool L2>
+template <typename T, int C, int RPT, int UKCH, bool LREG>
 static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
   const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
   static bool attr = false;
   if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C, RPT, UKCH, L2>,
+    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C, RPT, UKCH, LREG>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 256));
     attr = true;
   }
-  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, L2>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
+  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
                      L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
 }
