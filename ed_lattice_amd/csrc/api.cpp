@@ -275,6 +275,65 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           best.dw_vec = vec;
         }
       }
+  // The chain step's down pass (MODE 1) streams two more vectors and holds
+  // more registers; its best vector-gather shape is timed on its own, with
+  // the chain scalars in a middle-step state that repeated launches leave
+  // unchanged (beta = 0, norms 1).
+  {
+    LzScalars s0{};
+    s0.s_cur = 1.0;
+    s0.s_old = 1.0;
+    DevBuf scd(sizeof(LzScalars)), ab(sizeof(double) * 8);
+    EDL_HIP(hipMemcpyAsync(scd.p, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
+    launch_zero_bytes(ab.p, sizeof(double) * 8, st);
+    size_t maxpart = 1;
+    for (int R : {64, 128})
+      for (int nch : {64, 128, 256}) {
+        TwoPassCfg w = best;
+        w.R = R;
+        w.nchunk = std::min(nch, V.dimdw);
+        maxpart = std::max(maxpart, (size_t)pass_dw_grid(V, w));
+      }
+    DevBuf part(sizeof(double) * maxpart);
+    LzChain A{};
+    A.sc = scd.as<LzScalars>();
+    A.alpha = ab.as<double>();
+    A.beta = ab.as<double>() + 4;
+    A.part_dw = part.as<double>();
+    A.nmax = 4;
+    double bc = 1e300;
+    for (int R : {64, 128})
+      for (int nk : {102, 104, 201, 202, 204})
+        for (int th : {256, 512, 1024})
+          for (int nch : {64, 128, 256}) {
+            if (nch > V.dimdw) continue;
+            TwoPassCfg w = best;
+            w.R = R;
+            w.ncol = nk / 100;
+            w.kch = nk % 100;
+            w.threads_dw = th;
+            w.nchunk = nch;
+            w.dw_vec = 1;
+            if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
+            launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
+            EventTimer tm(st);
+            for (int r = 0; r < 3; ++r)
+              launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
+            const double ms = tm.stop();
+            if (verbose())
+              std::fprintf(stderr, "[edl] tune dw chain fp%d R %3d ncol %d kch %d threads %4d chunks %3d: %.4f ms\n",
+                           8 * (int)sizeof(T), R, w.ncol, w.kch, th, nch, ms / 3);
+            if (ms < bc) {
+              bc = ms;
+              best.ch_R = R;
+              best.ch_threads = th;
+              best.ch_nchunk = nch;
+              best.ch_ncol = w.ncol;
+              best.ch_kch = w.kch;
+              best.ch_vec = 1;
+            }
+          }
+  }
   tp_cache()[key] = best;
   return best;
 }
@@ -702,7 +761,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   // Two-pass chains keep their vectors with the padded column stride.
   const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
-    tps.cfg = seg ? seg_cfg<T>(c, S) : two_pass_cfg<T>(c, S);
+    tps.cfg = seg ? seg_cfg<T>(c, S) : chain_cfg(two_pass_cfg<T>(c, S));
     if (seg) {
       seg_bounds(S, sizeof(T), tps.cfg.C);
       tps.seg = &S;
@@ -1592,7 +1651,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       Sector& J = get_sector(c, targets[idx[k]]);
       const bool seg = seg_path(J, sizeof(T));
       TwoPassStep<T> tps{};
-      tps.cfg = seg ? seg_cfg<T>(c, J) : two_pass_cfg<T>(c, J);
+      tps.cfg = seg ? seg_cfg<T>(c, J) : chain_cfg(two_pass_cfg<T>(c, J));
       if (seg) {
         seg_bounds(J, sizeof(T), tps.cfg.C);
         tps.seg = &J;

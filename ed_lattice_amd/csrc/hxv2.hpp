@@ -57,7 +57,25 @@ struct TwoPassCfg {
   int kch = 16;         // hop chunk of the down pass (8, 16, 32 with 1 column; 8, 16 with 2)
   int dw_vec = 0;       // 1: vector-gather down pass (16-byte lanes; R 16|32|64|128, 256 in fp32;
                         //    (ncol, kch) in (1, 2|4), (2, 1|2|4))
+  // Down-pass shape of the Lanczos chains (the pass with the Lanczos terms
+  // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
+  int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0;
 };
+
+// The schedule a Lanczos chain runs: the operator's, with the down-pass
+// shape tuned for the chain step where there is one.
+inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
+  TwoPassCfg r = c;
+  if (c.ch_R > 0) {
+    r.R = c.ch_R;
+    r.threads_dw = c.ch_threads;
+    r.nchunk = c.ch_nchunk;
+    r.ncol = c.ch_ncol;
+    r.kch = c.ch_kch;
+    r.dw_vec = c.ch_vec;
+  }
+  return r;
+}
 
 // Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
 // LzChain::part_dw; blocks of an up-pass launch, the size of part_up.
