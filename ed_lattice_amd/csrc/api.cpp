@@ -333,6 +333,44 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               best.ch_vec = 1;
             }
           }
+    // The chain step's up pass with phase B pending and alpha = 0 (r = t,
+    // written back over t: repeated launches leave their input unchanged).
+    LzScalars s1 = s0;
+    s1.pad = 1;
+    EDL_HIP(hipMemcpyAsync(scd.p, &s1, sizeof(s1), hipMemcpyHostToDevice, st));
+    DevBuf pu(sizeof(double) * (size_t)c.num_cu * 2);
+    A.part_up = pu.as<double>();
+    double bu2 = 1e300;
+    for (int C : {1, 2, 3, 4})
+      for (int rk : {208, 204, 108, 116})
+        for (int th : {512, 1024})
+          for (int gm : {1, 2}) {
+            const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
+            if (need > (size_t)lds || (gm == 2 && 2 * need > (size_t)lds)) continue;
+            TwoPassCfg w = best;
+            w.C = C;
+            w.up_rpt = rk / 100;
+            w.up_kch = rk % 100;
+            w.threads_up = th;
+            w.grid_up = c.num_cu * gm;
+            // qa = q_k (q), qb = t (o): phase B reads both and rewrites t
+            launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), o.as<T>(), t.as<T>(), &A, w, st);
+            EventTimer tm(st);
+            for (int r = 0; r < 3; ++r)
+              launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), o.as<T>(), t.as<T>(), &A, w, st);
+            const double ms = tm.stop();
+            if (verbose())
+              std::fprintf(stderr, "[edl] tune up chain fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
+                           8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
+            if (ms < bu2) {
+              bu2 = ms;
+              best.ch_C = C;
+              best.ch_up_threads = th;
+              best.ch_grid_up = w.grid_up;
+              best.ch_up_rpt = w.up_rpt;
+              best.ch_up_kch = w.up_kch;
+            }
+          }
   }
   tp_cache()[key] = best;
   return best;

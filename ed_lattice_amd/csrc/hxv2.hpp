@@ -60,6 +60,8 @@ struct TwoPassCfg {
   // Down-pass shape of the Lanczos chains (the pass with the Lanczos terms
   // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
   int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0;
+  // Up-pass shape of the chains (phase B fused), likewise; ch_C = 0: the operator's.
+  int ch_C = 0, ch_up_threads = 0, ch_grid_up = 0, ch_up_rpt = 0, ch_up_kch = 0;
 };
 
 // The schedule a Lanczos chain runs: the operator's, with the down-pass
@@ -73,6 +75,13 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
     r.ncol = c.ch_ncol;
     r.kch = c.ch_kch;
     r.dw_vec = c.ch_vec;
+  }
+  if (c.ch_C > 0) {
+    r.C = c.ch_C;
+    r.threads_up = c.ch_up_threads;
+    r.grid_up = c.ch_grid_up;
+    r.up_rpt = c.ch_up_rpt;
+    r.up_kch = c.ch_up_kch;
   }
   return r;
 }
