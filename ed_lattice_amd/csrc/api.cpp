@@ -446,6 +446,63 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
       dw = w;
     }
   }
+  // The chain step's down pass (MODE 1) over a few vector-gather shapes,
+  // with the chain scalars in a middle-step state that repeated launches
+  // leave unchanged (beta = 0, norms 1), as for two-pass sectors.
+  {
+    LzScalars s0{};
+    s0.s_cur = 1.0;
+    s0.s_old = 1.0;
+    DevBuf scd(sizeof(LzScalars)), ab(sizeof(double) * 8);
+    EDL_HIP(hipMemcpyAsync(scd.p, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
+    launch_zero_bytes(ab.p, sizeof(double) * 8, st);
+    // {R, columns per wave, hop chunk, threads, chunks}
+    const int cshapes[6][5] = {{128, 1, 4, 1024, 256}, {128, 2, 2, 1024, 256}, {64, 1, 4, 1024, 128},
+                               {64, 2, 2, 1024, 128},  {32, 1, 4, 1024, 128},  {32, 2, 2, 1024, 128}};
+    size_t maxpart = 1;
+    for (const auto& sh : cshapes) {
+      TwoPassCfg w = dw;
+      w.R = sh[0];
+      w.nchunk = std::max(1, std::min(sh[4], s.sp[1].dim));
+      maxpart = std::max(maxpart, (size_t)pass_dw_grid(V, w));
+    }
+    DevBuf part(sizeof(double) * maxpart);
+    LzChain A{};
+    A.sc = scd.as<LzScalars>();
+    A.alpha = ab.as<double>();
+    A.beta = ab.as<double>() + 4;
+    A.part_dw = part.as<double>();
+    A.nmax = 4;
+    double bc = 1e300;
+    for (const auto& sh : cshapes) {
+      TwoPassCfg w = dw;
+      w.R = sh[0];
+      w.ncol = sh[1];
+      w.kch = sh[2];
+      w.threads_dw = sh[3];
+      w.nchunk = std::max(1, std::min(sh[4], s.sp[1].dim));
+      w.dw_vec = 1;
+      if (pass_dw_lds_bytes(V, w) > (size_t)(160 * 1024 - 4096)) continue;
+      double ms = 0.0;
+      for (int r = 0; r < 2; ++r) {
+        EventTimer tm(st);
+        launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
+        ms = tm.stop();
+      }
+      if (verbose())
+        std::fprintf(stderr, "[edl] segmented sector chain down pass R %d ncol %d kch %d x %d threads, %d chunks: %.3f ms\n",
+                     w.R, w.ncol, w.kch, w.threads_dw, w.nchunk, ms);
+      if (ms < bc) {
+        bc = ms;
+        dw.ch_R = w.R;
+        dw.ch_threads = w.threads_dw;
+        dw.ch_nchunk = w.nchunk;
+        dw.ch_ncol = w.ncol;
+        dw.ch_kch = w.kch;
+        dw.ch_vec = 1;
+      }
+    }
+  }
   seg_bounds(s, sizeof(T), dw.C);
   cache[key] = dw;
   return dw;
@@ -799,7 +856,7 @@ static GsRun gs_run(Ctx& c, Sector& S, const double* d_start, int niter, double 
   // Two-pass chains keep their vectors with the padded column stride.
   const int ld = two ? padded_ld(S.sp[0].dim) : S.sp[0].dim;
   if (two) {
-    tps.cfg = seg ? seg_cfg<T>(c, S) : chain_cfg(two_pass_cfg<T>(c, S));
+    tps.cfg = chain_cfg(seg ? seg_cfg<T>(c, S) : two_pass_cfg<T>(c, S));
     if (seg) {
       seg_bounds(S, sizeof(T), tps.cfg.C);
       tps.seg = &S;
@@ -1689,7 +1746,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       Sector& J = get_sector(c, targets[idx[k]]);
       const bool seg = seg_path(J, sizeof(T));
       TwoPassStep<T> tps{};
-      tps.cfg = seg ? seg_cfg<T>(c, J) : chain_cfg(two_pass_cfg<T>(c, J));
+      tps.cfg = chain_cfg(seg ? seg_cfg<T>(c, J) : two_pass_cfg<T>(c, J));
       if (seg) {
         seg_bounds(J, sizeof(T), tps.cfg.C);
         tps.seg = &J;
