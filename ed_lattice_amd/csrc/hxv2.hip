@@ -411,16 +411,17 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
   }
 }
 
-// Vector-gather down pass (R-row panels, R = 16, 32 or 64; padded layout:
-// ld % 32 == 0).  A lane loads VW = 16 / sizeof(T) consecutive rows with
-// one 16-byte load, so an R-row neighbour segment takes LPS = R / VW lanes
-// and one wave instruction fetches GPW = 64 / LPS segments: lane group g takes the hops
-// k = g, g + GPW, g + 2 GPW, ... of the wave's column, and the groups' sums
-// meet by lane shuffles.  Against one 4-byte element per lane this issues
-// GPW times fewer gather instructions for the same bytes (the address unit,
-// not HBM, bounds the 4-byte form).  Same block mapping, modes and LDS
-// column table as k_pass_dw.  A quad that straddles DimUp loads rows of the
-// column's padding (inside the column) and stores only rows < DimUp.
+// Vector-gather down pass (R-row panels, R = 16 to 128, 256 in fp32; padded
+// layout: ld % 32 == 0).  A lane loads VW = 16 / sizeof(T) consecutive rows
+// with one 16-byte load, so an R-row neighbour segment takes LPS = R / VW
+// lanes and one wave instruction fetches GPW = 64 / LPS segments: lane group
+// g takes the hops k = g, g + GPW, g + 2 GPW, ... of the wave's column, and
+// the groups' sums meet by lane shuffles.  Against one 4-byte element per
+// lane this issues GPW times fewer gather instructions for the same bytes,
+// and it admits panels taller than a wave (longer neighbour segments, the
+// shape that ran fastest).  Same block mapping, modes and LDS column table
+// as k_pass_dw.  A quad that straddles DimUp loads rows of the column's
+// padding (inside the column) and stores only rows < DimUp.
 template <typename T, int R, int MODE, int NCOL, int KCH>
 __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
                                                    const T* __restrict__ tin, T* outp, LzChain A, int nchunk) {
