@@ -245,11 +245,10 @@ BtSet::~BtSet() {
   if (d_chain_dw) (void)hipFree(d_chain_dw);
 }
 
-// LDS limits of every instantiation, set once and outside any graph capture.
+// LDS limits of every instantiation, set once per device and outside any
+// graph capture.
 template <typename T>
 static void bt_attributes() {
-  static bool done = false;
-  if (done) return;
   for (auto f : {(const void*)k_bt_up<T, 1, 2, 8>, (const void*)k_bt_up<T, 2, 2, 8>, (const void*)k_bt_up<T, 4, 2, 8>,
                  (const void*)k_bt_up<T, 1, 2, 4>, (const void*)k_bt_up<T, 2, 2, 4>, (const void*)k_bt_up<T, 4, 2, 4>,
                  (const void*)k_bt_up<T, 1, 1, 8>, (const void*)k_bt_up<T, 2, 1, 8>, (const void*)k_bt_up<T, 4, 1, 8>,
@@ -257,8 +256,7 @@ static void bt_attributes() {
                  (const void*)k_bt_dw<T, 8, 1>, (const void*)k_bt_dw<T, 16, 1>, (const void*)k_bt_dw<T, 32, 1>,
                  (const void*)k_bt_dw<T, 1, 2>, (const void*)k_bt_dw<T, 2, 2>, (const void*)k_bt_dw<T, 4, 2>,
                  (const void*)k_bt_dw<T, 8, 2>, (const void*)k_bt_dw<T, 16, 2>, (const void*)k_bt_dw<T, 32, 2>})
-    EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBtDwBudget));
-  done = true;
+    set_lds_limit(f, (int)kBtDwBudget);
 }
 
 template <typename T>

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -33,6 +35,20 @@ struct Error : std::exception {
   do {                                        \
     if (!(cond)) throw ::edl::Error(text);    \
   } while (0)
+
+// Raises a kernel's dynamic-LDS limit on the current device, once per
+// (device, kernel): the attribute belongs to the device, so a process with
+// contexts on several GPUs sets it on each.
+inline void set_lds_limit(const void* kernel, int bytes) {
+  static std::mutex m;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  EDL_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(m);
+  if (done.count({dev, kernel})) return;
+  EDL_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  done.insert({dev, kernel});
+}
 
 // Parameters of the diagonal that are evaluated per element: the density-
 // density interaction between the up and the down string of one basis state.

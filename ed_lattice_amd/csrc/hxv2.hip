@@ -688,12 +688,7 @@ template <typename T, int C, int RPT, int UKCH, bool LREG>
 static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
   const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
-  static bool attr = false;
-  if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_up<T, C, RPT, UKCH, LREG>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 256));
-    attr = true;
-  }
+  set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG>, lds_limit_bytes() - 256);
   hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
                      L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
@@ -759,12 +754,8 @@ int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
 template <typename T, int MODE, int NCOL, int KCH>
 static void launch_dw_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
                         const TwoPassCfg& cfg, size_t lds, int grid, hipStream_t st) {
-  static bool attr = false;
-  if (!attr) {
-    for (auto f : {(const void*)k_pass_dw<T, 32, MODE, NCOL, KCH>, (const void*)k_pass_dw<T, 64, MODE, NCOL, KCH>})
-      EDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
-    attr = true;
-  }
+  for (auto f : {(const void*)k_pass_dw<T, 32, MODE, NCOL, KCH>, (const void*)k_pass_dw<T, 64, MODE, NCOL, KCH>})
+    set_lds_limit(f, lds_limit_bytes() - 4096);
   if (cfg.R == 32)
     hipLaunchKernelGGL((k_pass_dw<T, 32, MODE, NCOL, KCH>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
                        tin, out, A, cfg.nchunk);
@@ -786,12 +777,7 @@ size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg) {
 template <typename T, int R, int MODE, int NCOL, int KCH>
 static void launch_dwv_k(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
                          const TwoPassCfg& cfg, size_t lds, int grid, hipStream_t st) {
-  static bool attr = false;
-  if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_pass_dwv<T, R, MODE, NCOL, KCH>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
-    attr = true;
-  }
+  set_lds_limit((const void*)k_pass_dwv<T, R, MODE, NCOL, KCH>, lds_limit_bytes() - 4096);
   hipLaunchKernelGGL((k_pass_dwv<T, R, MODE, NCOL, KCH>), dim3(grid), dim3(cfg.threads_dw), lds, st, V, lut, qa, qb,
                      tin, out, A, cfg.nchunk);
   EDL_HIP(hipGetLastError());
@@ -1148,12 +1134,7 @@ static void launch_up_seg_c(const TwoPassView& V, const InterParams& I, const T*
   for (int k = 0; k < nseg; ++k) rows = std::max(rows, s.seg_h[k + 1] - s.seg_h[k]);
   const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)rows * C * sizeof(T);
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_up_seg: segment exceeds LDS");
-  static bool attr = false;
-  if (!attr) {
-    EDL_HIP(hipFuncSetAttribute((const void*)k_up_seg<T, C, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                lds_limit_bytes() - 256));
-    attr = true;
-  }
+  set_lds_limit((const void*)k_up_seg<T, C, 8>, lds_limit_bytes() - 256);
   const int64_t ntiles = (int64_t)((V.dimdw + C - 1) / C) * nseg;
   const int64_t n8 = (ntiles + 7) / 8;
   EDL_CHECK(8 * n8 < ((int64_t)1 << 31), "launch_up_seg: too many tiles");
