@@ -1571,7 +1571,7 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
     return 0.0;
   };
   double bu = 1e300, bd = 1e300;
-  for (int C : {1, 2, 4}) {
+  for (int C : {1, 2, 3, 4}) {
     if (head + (size_t)C * max_up * sizeof(T) > budget) continue;
     for (int rk : {208, 204, 108})
       for (int th : {256, 512, 1024}) {

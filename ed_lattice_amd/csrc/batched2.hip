@@ -249,7 +249,8 @@ BtSet::~BtSet() {
 // graph capture.
 template <typename T>
 static void bt_attributes() {
-  for (auto f : {(const void*)k_bt_up<T, 1, 2, 8>, (const void*)k_bt_up<T, 2, 2, 8>, (const void*)k_bt_up<T, 4, 2, 8>,
+  for (auto f : {(const void*)k_bt_up<T, 3, 2, 8>, (const void*)k_bt_up<T, 3, 2, 4>, (const void*)k_bt_up<T, 3, 1, 8>,
+                 (const void*)k_bt_up<T, 1, 2, 8>, (const void*)k_bt_up<T, 2, 2, 8>, (const void*)k_bt_up<T, 4, 2, 8>,
                  (const void*)k_bt_up<T, 1, 2, 4>, (const void*)k_bt_up<T, 2, 2, 4>, (const void*)k_bt_up<T, 4, 2, 4>,
                  (const void*)k_bt_up<T, 1, 1, 8>, (const void*)k_bt_up<T, 2, 1, 8>, (const void*)k_bt_up<T, 4, 1, 8>,
                  (const void*)k_bt_dw<T, 1, 1>, (const void*)k_bt_dw<T, 2, 1>, (const void*)k_bt_dw<T, 4, 1>,
@@ -337,6 +338,7 @@ void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st, bool sc
   switch (bs.shape.C) {
     case 1: return bt_up_c<T, 1>(bs, I, st, scalars);
     case 2: return bt_up_c<T, 2>(bs, I, st, scalars);
+    case 3: return bt_up_c<T, 3>(bs, I, st, scalars);
     case 4: return bt_up_c<T, 4>(bs, I, st, scalars);
     default: throw Error("bt_launch_up: unsupported panel width");
   }
