@@ -148,10 +148,29 @@ static bool verbose() {
   return e && *e && *e != '0';
 }
 
-// Tuned two-pass schedule per (sector shape, precision).
+// Tuned two-pass schedule per (sector shape, precision); the chain shapes
+// (the ch_* fields) also on their own, for pinned operator schedules.
 static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg>& tp_cache() {
   static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg> m;
   return m;
+}
+static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg>& tp_chain_cache() {
+  static std::map<std::tuple<int, int, int, int, int>, TwoPassCfg> m;
+  return m;
+}
+// dst takes the chain shapes of src.
+static void with_chain_shapes(TwoPassCfg& dst, const TwoPassCfg& src) {
+  dst.ch_R = src.ch_R;
+  dst.ch_threads = src.ch_threads;
+  dst.ch_nchunk = src.ch_nchunk;
+  dst.ch_ncol = src.ch_ncol;
+  dst.ch_kch = src.ch_kch;
+  dst.ch_vec = src.ch_vec;
+  dst.ch_C = src.ch_C;
+  dst.ch_up_threads = src.ch_up_threads;
+  dst.ch_grid_up = src.ch_grid_up;
+  dst.ch_up_rpt = src.ch_up_rpt;
+  dst.ch_up_kch = src.ch_up_kch;
 }
 
 // Pick panel widths, block sizes and grids of the two passes by timing the
@@ -162,10 +181,13 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   // timed on the engine's own (padded) layout
   const int ld = padded_ld(s.sp[0].dim);
   const TwoPassView V = two_pass_view(s, ld);
+  TwoPassCfg best;
+  bool pinned = false;
   // EDL_TWO_PASS_CFG_F32 / _F64 =
   // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec]]]"
-  // pins the schedule of that precision (reproducible profiles of one
-  // configuration), ahead of any tuned schedule; no tuning then.
+  // pins the operator schedule of that precision (reproducible profiles of
+  // one configuration), ahead of any tuned schedule; the operator is not
+  // tuned then, its chain shapes still are (once per sector shape).
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
     const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
@@ -194,87 +216,97 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               "EDL_TWO_PASS_CFG_F32/_F64: bad block size, grid or chunk count");
     EDL_CHECK(256 * sizeof(T) + 16 + (size_t)V.dimup * p.C * sizeof(T) <= (size_t)(160 * 1024),
               "EDL_TWO_PASS_CFG_F32/_F64: up-pass columns exceed LDS");
-    return p;
+    pinned = true;
+    best = p;
+  } else {
+    auto it = tp_cache().find(key);
+    if (it != tp_cache().end()) return it->second;
   }
-  auto it = tp_cache().find(key);
-  if (it != tp_cache().end()) return it->second;
+  // The chain shapes do not depend on the operator's: a pinned operator
+  // schedule still runs its chains on their own tuned shapes.
+  auto ct = tp_chain_cache().find(key);
+  if (pinned && ct != tp_chain_cache().end()) {
+    with_chain_shapes(best, ct->second);
+    return best;
+  }
   hipStream_t st = c.stream;
   const int64_t D = s.dim, Dp = (int64_t)ld * s.sp[1].dim;
   DevBuf d64(sizeof(double) * D), q(sizeof(T) * Dp), t(sizeof(T) * Dp), o(sizeof(T) * Dp);
   launch_start_vector(d64.as<double>(), D, 0x5EEDull, st);
   launch_zero_bytes(q.p, sizeof(T) * Dp, st);
   launch_convert_ld<T>(d64.as<double>(), q.as<T>(), s.sp[0].dim, s.sp[1].dim, ld, st);
-  TwoPassCfg best;
-  best.grid_up = c.num_cu;
-  best.nchunk = std::min(32, V.dimdw);
-  double best_up = 1e300, best_dw = 1e300;
   const int lds = 160 * 1024;
-  for (int C : {1, 2, 3, 4})
-    for (int rk : {208, 204, 108, 116})
-    for (int th : {512, 1024})
-      for (int gm : {1, 2}) {
-        const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
-        if (need > (size_t)lds) continue;
-        if (gm == 2 && 2 * need > (size_t)lds) continue;
-        TwoPassCfg w = best;
-        w.C = C;
-        w.up_rpt = rk / 100;
-        w.up_kch = rk % 100;
-        w.threads_up = th;
-        w.grid_up = c.num_cu * gm;
-        launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
-        EventTimer tm(st);
-        for (int r = 0; r < 3; ++r)
+  if (!pinned) {
+    best.grid_up = c.num_cu;
+    best.nchunk = std::min(32, V.dimdw);
+    double best_up = 1e300, best_dw = 1e300;
+    for (int C : {1, 2, 3, 4})
+      for (int rk : {208, 204, 108, 116})
+      for (int th : {512, 1024})
+        for (int gm : {1, 2}) {
+          const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
+          if (need > (size_t)lds) continue;
+          if (gm == 2 && 2 * need > (size_t)lds) continue;
+          TwoPassCfg w = best;
+          w.C = C;
+          w.up_rpt = rk / 100;
+          w.up_kch = rk % 100;
+          w.threads_up = th;
+          w.grid_up = c.num_cu * gm;
           launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
-        const double ms = tm.stop();
-        if (verbose())
-          std::fprintf(stderr, "[edl] tune up  fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
-                       8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
-        if (ms < best_up) {
-          best_up = ms;
-          best.C = C;
-          best.up_rpt = w.up_rpt;
-          best.up_kch = w.up_kch;
-          best.threads_up = th;
-          best.grid_up = w.grid_up;
+          EventTimer tm(st);
+          for (int r = 0; r < 3; ++r)
+            launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
+          const double ms = tm.stop();
+          if (verbose())
+            std::fprintf(stderr, "[edl] tune up  fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
+                         8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
+          if (ms < best_up) {
+            best_up = ms;
+            best.C = C;
+            best.up_rpt = w.up_rpt;
+            best.up_kch = w.up_kch;
+            best.threads_up = th;
+            best.grid_up = w.grid_up;
+          }
         }
-      }
-  // 4-byte (8-byte) gathers, then 16-byte vector gathers
-  for (int vec : {0, 1})
-  for (int R : {16, 32, 64, 128, 256})
-    for (int nk : {108, 116, 132, 208, 216, 102, 104, 201, 202, 204})
-      for (int th : {256, 512, 1024})
-        for (int nch : {32, 64, 128, 256}) {
-        if (nch > V.dimdw) continue;
-        const int nc = nk / 100, kc = nk % 100;
-        const bool vk = kc <= 4;
-        if (vk != (vec == 1) || (!vec && (R == 16 || R > 64)) || (R == 256 && sizeof(T) != 4)) continue;
-        TwoPassCfg w = best;
-        w.R = R;
-        w.ncol = nc;
-        w.kch = kc;
-        w.threads_dw = th;
-        w.nchunk = nch;
-        w.dw_vec = vec;
-        if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
-        launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
-        EventTimer tm(st);
-        for (int r = 0; r < 3; ++r)
+    // 4-byte (8-byte) gathers, then 16-byte vector gathers
+    for (int vec : {0, 1})
+    for (int R : {16, 32, 64, 128, 256})
+      for (int nk : {108, 116, 132, 208, 216, 102, 104, 201, 202, 204})
+        for (int th : {256, 512, 1024})
+          for (int nch : {32, 64, 128, 256}) {
+          if (nch > V.dimdw) continue;
+          const int nc = nk / 100, kc = nk % 100;
+          const bool vk = kc <= 4;
+          if (vk != (vec == 1) || (!vec && (R == 16 || R > 64)) || (R == 256 && sizeof(T) != 4)) continue;
+          TwoPassCfg w = best;
+          w.R = R;
+          w.ncol = nc;
+          w.kch = kc;
+          w.threads_dw = th;
+          w.nchunk = nch;
+          w.dw_vec = vec;
+          if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
           launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
-        const double ms = tm.stop();
-        if (verbose())
-          std::fprintf(stderr, "[edl] tune dw  fp%d vec %d R %3d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
-                       8 * (int)sizeof(T), vec, R, nc, kc, th, nch, ms / 3);
-        if (ms < best_dw) {
-          best_dw = ms;
-          best.R = R;
-          best.ncol = nc;
-          best.kch = kc;
-          best.threads_dw = th;
-          best.nchunk = nch;
-          best.dw_vec = vec;
+          EventTimer tm(st);
+          for (int r = 0; r < 3; ++r)
+            launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+          const double ms = tm.stop();
+          if (verbose())
+            std::fprintf(stderr, "[edl] tune dw  fp%d vec %d R %3d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
+                         8 * (int)sizeof(T), vec, R, nc, kc, th, nch, ms / 3);
+          if (ms < best_dw) {
+            best_dw = ms;
+            best.R = R;
+            best.ncol = nc;
+            best.kch = kc;
+            best.threads_dw = th;
+            best.nchunk = nch;
+            best.dw_vec = vec;
+          }
         }
-      }
+  }
   // The chain step's down pass (MODE 1) streams two more vectors and holds
   // more registers; its best vector-gather shape is timed on its own, with
   // the chain scalars in a middle-step state that repeated launches leave
@@ -372,7 +404,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             }
           }
   }
-  tp_cache()[key] = best;
+  tp_chain_cache()[key] = best;
+  if (!pinned) tp_cache()[key] = best;
   return best;
 }
 
@@ -569,12 +602,13 @@ struct ChainAlloc {
 
 // d_start is in the natural layout; the stored vectors get column stride ld
 // (the padding rows are zero and no kernel writes them).
-template <typename T>
 // With d_ss set, the start vector's sum of squares goes to *d_ss on the
-// device (partials in `part`) and the scalars are left to
-// launch_init_scalars: no host round trip.
+// device (partials in `part`, which holds part_n doubles) and the scalars
+// are left to launch_init_scalars: no host round trip.
+template <typename T>
 static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, const double* d_start, double norm2,
-                       int nmax, hipStream_t st, int ld = 0, double* d_ss = nullptr, double* part = nullptr) {
+                       int nmax, hipStream_t st, int ld = 0, double* d_ss = nullptr, double* part = nullptr,
+                       int64_t part_n = 0) {
   if (ld == 0) ld = S.sp[0].dim;
   const int64_t D = (int64_t)ld * S.sp[1].dim;
   fill_sector_view(S, sizeof(T) == 4, cd);
@@ -592,6 +626,7 @@ static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, 
   launch_zero_bytes(beta, sizeof(double) * nmax, st);
   launch_zero_bytes(counter, 4, st);
   if (d_ss) {
+    EDL_CHECK(part_n >= partials_size(D), "init_chain: partial-sum buffer smaller than the launch grid");
     device_sumsq_to<T>(q0, D, part, d_ss, st);
   } else {
     const double ss = device_sumsq<T>(q0, D, st);
@@ -605,6 +640,8 @@ static void init_chain(Ctx& c, ChainDev& cd, ChainAlloc<T>& A, const Sector& S, 
     s.alpha_k = 0.0;
     s.norm2 = norm2;
     EDL_HIP(hipMemcpyAsync(sc, &s, sizeof(s), hipMemcpyHostToDevice, st));
+    // `s` lives on this stack frame: the copy must have read it before return
+    EDL_HIP(hipStreamSynchronize(st));
   }
   cd.q0 = q0;
   cd.q1 = q1;
@@ -985,14 +1022,32 @@ static std::map<const Ctx*, GsStats>& gs_stats() {
   return m;
 }
 
-static void install_gs(Ctx& c, int isector, int64_t dim) {
-  if (c.d_gs && c.gs_dim != dim) {
-    (void)hipFree(c.d_gs);
-    c.d_gs = nullptr;
-  }
-  if (!c.d_gs) EDL_HIP(hipMalloc(&c.d_gs, sizeof(double) * dim));
+// Drop the installed state (a new model invalidates it).
+static void clear_gs(Ctx& c) {
+  if (c.d_gs) (void)hipFree(c.d_gs);
+  c.d_gs = nullptr;
+  c.gs_dim = 0;
+  c.gs_sector = 0;
+  c.gs_energy = 0.0;
+}
+
+// Install a computed state: the context takes the buffer over (only after
+// the computation succeeded, so a failure leaves the previous state whole).
+static void adopt_gs(Ctx& c, int isector, int64_t dim, double energy, DevBuf&& buf) {
+  clear_gs(c);
+  c.d_gs = buf.as<double>();
+  buf.p = nullptr;
   c.gs_dim = dim;
   c.gs_sector = isector;
+  c.gs_energy = energy;
+}
+
+// The sector of the installed state, checked against the state's length.
+static Sector& gs_sector_of(Ctx& c, const char* who) {
+  EDL_CHECK(c.d_gs && c.gs_sector > 0, std::string(who) + ": no ground state installed");
+  Sector& S = get_sector(c, c.gs_sector);
+  EDL_CHECK(S.dim == c.gs_dim, std::string(who) + ": installed state does not match its sector");
+  return S;
 }
 
 }  // namespace edl
@@ -1080,6 +1135,8 @@ extern "C" int edl_set_hamiltonian(edl_ctx* ctx, int Ns, int Nspin, int Norb, co
   EDL_HIP(hipSetDevice(ctx->c.device));
   EDL_HIP(hipStreamSynchronize(ctx->c.stream));
   release_sectors(ctx->c);
+  // the installed state belongs to the previous model
+  clear_gs(ctx->c);
   Model& M = ctx->c.model;
   M.Ns = Ns;
   M.Nspin = Nspin;
@@ -1332,34 +1389,33 @@ extern "C" int edl_lanczos_gs(edl_ctx* ctx, int isector, int prec, int niter, do
   Sector& S = get_sector(c, isector);
   const int64_t D = S.dim;
   hipStream_t st = c.stream;
-  DevBuf start(sizeof(double) * D);
+  DevBuf start(sizeof(double) * D), x(sizeof(double) * D);
   launch_start_vector(start.as<double>(), D, seed, st);
-  install_gs(c, isector, D);
   GsStats stats;
   EventTimer tm(st);
   GsRun r;
   int64_t hxv = 0;
   if (prec == EDL_FP64) {
-    r = gs_run<double>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    r = gs_run<double>(c, S, start.as<double>(), niter, tol, 5, x.as<double>());
     stats.steps_hi = r.out.nsteps;
   } else if (prec == EDL_FP32) {
-    r = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    r = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, x.as<double>());
     stats.steps_lo = r.out.nsteps;
   } else {
-    GsRun lo = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, c.d_gs);
+    GsRun lo = gs_run<float>(c, S, start.as<double>(), niter, tol, 5, x.as<double>());
     stats.steps_lo = lo.out.nsteps;
     stats.e0_lo = lo.e0;
     hxv += lo.hxv;
-    // FP64 restart from the FP32 Ritz vector (now in c.d_gs)
-    EDL_HIP(hipMemcpyAsync(start.p, c.d_gs, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
+    // FP64 restart from the FP32 Ritz vector (now in x)
+    EDL_HIP(hipMemcpyAsync(start.p, x.p, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
     const double rt = refine_tol > 0.0 ? refine_tol : 1e-13;
-    r = gs_run<double>(c, S, start.as<double>(), niter, rt, 5, c.d_gs);
+    r = gs_run<double>(c, S, start.as<double>(), niter, rt, 5, x.as<double>());
     stats.steps_hi = r.out.nsteps;
   }
   hxv += r.hxv;
   c.last_ms = tm.stop();
   c.last_hxv = hxv;
-  c.gs_energy = r.e0;
+  adopt_gs(c, isector, D, r.e0, std::move(x));
   gs_stats()[&c] = stats;
   if (e0) *e0 = r.e0;
   if (nsteps) *nsteps = r.out.nsteps;
@@ -1430,11 +1486,11 @@ extern "C" int edl_lanczos_states(edl_ctx* ctx, int isector, int prec, int nstat
     lanczos_states_impl<float>(c, S, nstates, niter, tol, seed, X.as<double>(), energies, &hxv);
   c.last_ms = tm.stop();
   c.last_hxv = hxv;
-  install_gs(c, isector, D);
-  c.gs_energy = energies[0];
-  EDL_HIP(hipMemcpyAsync(c.d_gs, X.p, sizeof(double) * D, hipMemcpyDeviceToDevice, c.stream));
+  DevBuf x0(sizeof(double) * D);
+  EDL_HIP(hipMemcpyAsync(x0.p, X.p, sizeof(double) * D, hipMemcpyDeviceToDevice, c.stream));
   if (vecs) EDL_HIP(hipMemcpyAsync(vecs, X.p, sizeof(double) * (size_t)D * nstates, hipMemcpyDeviceToHost, c.stream));
   EDL_HIP(hipStreamSynchronize(c.stream));
+  adopt_gs(c, isector, D, energies[0], std::move(x0));
   API_END
 }
 
@@ -1453,17 +1509,17 @@ extern "C" int edl_set_state(edl_ctx* ctx, int isector, double energy, const dou
   Ctx& c = ctx->c;
   EDL_HIP(hipSetDevice(c.device));
   Sector& S = get_sector(c, isector);
-  install_gs(c, isector, S.dim);
-  c.gs_energy = energy;
-  EDL_HIP(hipMemcpyAsync(c.d_gs, vec, sizeof(double) * S.dim, hipMemcpyHostToDevice, c.stream));
+  DevBuf x(sizeof(double) * S.dim);
+  EDL_HIP(hipMemcpyAsync(x.p, vec, sizeof(double) * S.dim, hipMemcpyHostToDevice, c.stream));
   EDL_HIP(hipStreamSynchronize(c.stream));
+  adopt_gs(c, isector, S.dim, energy, std::move(x));
   API_END
 }
 
 extern "C" int edl_get_state(edl_ctx* ctx, double* vec) {
   API_BEGIN
   Ctx& c = ctx->c;
-  EDL_CHECK(c.d_gs, "edl_get_state: no state installed");
+  EDL_CHECK(c.d_gs && vec, "edl_get_state: no state installed or null output");
   EDL_HIP(hipMemcpyAsync(vec, c.d_gs, sizeof(double) * c.gs_dim, hipMemcpyDeviceToHost, c.stream));
   EDL_HIP(hipStreamSynchronize(c.stream));
   API_END
@@ -1671,9 +1727,19 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   }
   // Seeds are generated one after another into tmp (stream order keeps the
   // reuse safe); their norms stay on the device until every chain is set up.
+  // The partial sums serve the seed norm (J.dim elements) and the stored
+  // start vector's sum of squares (ld * DimDw elements, ld >= DimUp rounded
+  // up to 32 rows: up to 32x J.dim on sectors with one up string).
   DevBuf tmp(sizeof(double) * maxdim);
   const size_t nc = idx.size();
-  DevBuf sums(sizeof(double) * 2 * nc), part(sizeof(double) * (size_t)partials_size(std::max<int64_t>(maxdim, 1) * 2));
+  int64_t maxred = std::max<int64_t>(maxdim, 1);
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const Sector& J = get_sector(c, targets[idx[k]]);
+    const int ld = (bt || alone) ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
+    maxred = std::max<int64_t>(maxred, (int64_t)ld * J.sp[1].dim);
+  }
+  const int64_t part_n = partials_size(maxred);
+  DevBuf sums(sizeof(double) * 2 * nc), part(sizeof(double) * (size_t)part_n);
   double* d_n2 = sums.as<double>();
   double* d_ss = d_n2 + nc;
   for (size_t k = 0; k < nc; ++k) {
@@ -1682,18 +1748,19 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     if (host_seed) {
       EDL_HIP(hipMemcpyAsync(tmp.p, host_seed, sizeof(double) * J.dim, hipMemcpyHostToDevice, st));
     } else {
-      const Sector& I = get_sector(c, c.gs_sector);
+      const Sector& I = gs_sector_of(c, "edl_tridiag_batched");
       if (seeds[s].op == EDL_OP_J)
         launch_current_seed(current_terms(c.model, seeds[s].io), c.d_gs, J, tmp.as<double>(), st);
       else
         launch_seed(seeds[s], c.d_gs, I.sp[0].dim, J, tmp.as<double>(), st);
     }
+    EDL_CHECK(partials_size(J.dim) <= part_n, "tridiag_group: partial-sum buffer smaller than the launch grid");
     device_dot_to(tmp.as<double>(), tmp.as<double>(), J.dim, part.as<double>(), d_n2 + k, st);
     const int nmax = (int)std::min<int64_t>(nlanc, J.dim);
     ChainDev cd{};
     const int ld = (bt || alone) ? padded_ld(J.sp[0].dim) : J.sp[0].dim;
     allocs[k].ar = &ar;
-    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), 0.0, nmax, st, ld, d_ss + k, part.as<double>());
+    init_chain<T>(c, cd, allocs[k], J, tmp.as<double>(), 0.0, nmax, st, ld, d_ss + k, part.as<double>(), part_n);
     if (bt) {
       BtChain b{};
       b.V = two_pass_view(J, ld);
@@ -1828,8 +1895,8 @@ extern "C" int edl_tridiag_batched(edl_ctx* ctx, int nseeds, const edl_seed_desc
   EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_tridiag_batched: prec must be EDL_FP64 or EDL_FP32");
   EDL_CHECK(nlanc >= 1, "edl_tridiag_batched: nlanc must be >= 1");
   Ctx& c = ctx->c;
-  EDL_CHECK(c.d_gs, "edl_tridiag_batched: no ground state installed");
   EDL_HIP(hipSetDevice(c.device));
+  gs_sector_of(c, "edl_tridiag_batched");
   const int Ns = c.model.Ns;
   std::vector<SeedSpec> sp(nseeds);
   std::vector<int> targets(nseeds, 0), active;
@@ -1901,10 +1968,9 @@ extern "C" int edl_lanczos_poles(int nlanc, const double* alpha, const double* b
 extern "C" int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, double* docc) {
   API_BEGIN
   Ctx& c = ctx->c;
-  EDL_CHECK(c.d_gs, "edl_observables: no ground state installed");
   EDL_CHECK(dens_up && dens_dw && docc, "edl_observables: null output");
   EDL_HIP(hipSetDevice(c.device));
-  Sector& S = get_sector(c, c.gs_sector);
+  Sector& S = gs_sector_of(c, "edl_observables");
   device_observables(c, S, c.d_gs, dens_up, dens_dw, docc);
   API_END
 }
@@ -1912,10 +1978,9 @@ extern "C" int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, d
 extern "C" int edl_local_energy(edl_ctx* ctx, double* e) {
   API_BEGIN
   Ctx& c = ctx->c;
-  EDL_CHECK(c.d_gs, "edl_local_energy: no ground state installed");
   EDL_CHECK(e, "edl_local_energy: null output");
   EDL_HIP(hipSetDevice(c.device));
-  Sector& S = get_sector(c, c.gs_sector);
+  Sector& S = gs_sector_of(c, "edl_local_energy");
   device_energy(c, S, c.d_gs, e);
   API_END
 }

@@ -589,6 +589,8 @@ void chainset_finalize(Ctx& c, ChainSet<T>& cs, const Schedule& want) {
   cs.owned.push_back(cs.d_tile0);
   EDL_HIP(hipMemcpyAsync(cs.d, cs.h.data(), sizeof(ChainDev) * cs.h.size(), hipMemcpyHostToDevice, c.stream));
   EDL_HIP(hipMemcpyAsync(cs.d_tile0, t0.data(), sizeof(int64_t) * t0.size(), hipMemcpyHostToDevice, c.stream));
+  // t0 is local: the copies must have read their host sources before return
+  EDL_HIP(hipStreamSynchronize(c.stream));
 }
 
 template <typename T>

@@ -693,8 +693,11 @@ int or_observables(int Ns, int Nup, int Ndw, const double *state, double *dens_u
  * each row the candidate sources are its hop partners; a source's scatter
  * contributions are kept when their target (found by binary_search as in the
  * reference) is the row.  Non-local (Jhflag) terms refused. */
-int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t *rows, const float *v,
-                       double *out) {
+/* Element k of the input vector: fp32 (vf) or fp64 (vd) storage. */
+static inline double vec_at(const float *vf, const double *vd, int64_t k) { return vf ? (double)vf[k] : vd[k]; }
+
+static int hxv_rows_real64_impl(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t *rows, const float *vf,
+                                const double *vd, double *out) {
   or_model m;
   int rc = model_init(&m, OR_MODEL_PASS);
   if (rc) return rc;
@@ -719,7 +722,7 @@ int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t
       S1.Dim = 1;
       S1.mup = &su;
       S1.mdw = &sd;
-      acc += local_htmp(&m, &S1, 1, Hdiag) * (double)v[i];
+      acc += local_htmp(&m, &S1, 1, Hdiag) * vec_at(vf, vd, i);
     }
     int bits[OR_MAXNS + 1];
     /* up hops: sources jup in column idw */
@@ -744,7 +747,7 @@ int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t
             or_c(jo, mup[jup - 1], &t1, &s1);
             or_cdg(io, t1, &t2, &s2);
             if (or_binary_search(mup, (int)dimup, t2) == iup)
-              acc += h * s1 * s2 * (double)v[(int64_t)(jup - 1) + (int64_t)(idw - 1) * dimup];
+              acc += h * s1 * s2 * vec_at(vf, vd, (int64_t)(jup - 1) + (int64_t)(idw - 1) * dimup);
           }
       }
     /* down hops: sources jdw in row iup */
@@ -769,7 +772,7 @@ int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t
             or_c(jo, mdw[jdw - 1], &t1, &s1);
             or_cdg(io, t1, &t2, &s2);
             if (or_binary_search(mdw, (int)dimdw, t2) == idw)
-              acc += h * s1 * s2 * (double)v[(int64_t)(iup - 1) + (int64_t)(jdw - 1) * dimup];
+              acc += h * s1 * s2 * vec_at(vf, vd, (int64_t)(iup - 1) + (int64_t)(jdw - 1) * dimup);
           }
       }
     out[r] = acc;
@@ -777,4 +780,16 @@ int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t
   free(mup);
   free(mdw);
   return 0;
+}
+
+int or_hxv_rows_real64(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t *rows, const float *v,
+                       double *out) {
+  return hxv_rows_real64_impl(OR_MODEL_PASS, Nup, Ndw, nrows, rows, v, NULL, out);
+}
+
+/* The same rows for a real fp64 input vector (checks of full-size fp64
+ * eigenvectors). */
+int or_hxv_rows_real64d(OR_MODEL_ARGS, int Nup, int Ndw, int nrows, const int64_t *rows, const double *v,
+                        double *out) {
+  return hxv_rows_real64_impl(OR_MODEL_PASS, Nup, Ndw, nrows, rows, NULL, v, out);
 }

@@ -195,6 +195,23 @@ def hxv_rows_real64(model, nup, ndw, rows, v32):
     return out
 
 
+def hxv_rows_real64d(model, nup, ndw, rows, v64):
+    """hxv_rows_real64 for a real fp64 vector (full-size eigenvector checks)."""
+    L = lib()
+    L.or_hxv_rows_real64d.argtypes = [c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), POINTER(c_double),
+                                      POINTER(c_double), c_double, c_double, c_double, c_double, c_double, c_int,
+                                      c_int, c_int, c_int, c_int, POINTER(c_int64), POINTER(c_double),
+                                      POINTER(c_double)]
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    v = np.ascontiguousarray(v64, dtype=np.float64)
+    out = np.zeros(rows.size)
+    rc = L.or_hxv_rows_real64d(*model.args(), nup, ndw, rows.size, rows.ctypes.data_as(POINTER(c_int64)), _dp(v),
+                               _dp(out))
+    if rc:
+        raise RuntimeError(f"or_hxv_rows_real64d failed ({rc})")
+    return out
+
+
 def seed_c_cdg(Ns, op, ipos, ispin, nI, nJ, state):
     """apply_op_C (op=1) / apply_op_CDG (op=2) loop of GF_ELECTRONS.f90:241-245."""
     st = np.ascontiguousarray(state, dtype=np.complex128)

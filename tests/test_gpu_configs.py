@@ -1,0 +1,174 @@
+"""The benchmark workloads at full size (GPU).
+
+4x4 PBC U=8 (8,8), dim 165 636 900: FP64 and mixed-precision Lanczos of 200
+steps; the mixed E0 within 1e-10 of FP64, the density and double-occupancy
+differences stated and bounded, and both eigenpairs pinned against the
+oracle's restated directMatVec through residual rows (H x - E0 x)_i.
+
+3x6 OBC U=4 with seeded disorder (9,9), dim 2 363 904 400: the mixed ground
+state through residual rows of the oracle's 64-bit row evaluator, then the 18
+S^z_i seeds of the spin susceptibility with 100 Lanczos steps each, checked
+by the sum rule norm2_i = <(S^z_i)^2> = (n_up + n_dw - 2 docc)/4 of the same
+state and by poles E_j - E0 >= 0 (Ritz values of the N-sector lie above its
+ground state).
+
+3x4 PBC U=6 (6,6): all 312 GF seeds of both spins (78 i <= j pairs x
+particle/hole x spin; targets (7,6), (5,6), (6,7), (6,5) of dim 731 808) in
+one batched call, against one-at-a-time chains and the oracle's Lanczos."""
+import numpy as np
+import pytest
+
+from ed_lattice_amd import EDLattice, lattice
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(77)
+
+
+def _residual_rows(orc, m, nup, ndw, x, e0, nrows=300):
+    """max_i |(H x)_i - E0 x_i| over sampled rows (oracle operator), and the
+    scale |E0| * rms(x) it is compared with."""
+    dim = x.size
+    rows = np.concatenate([[0, dim - 1], RNG.integers(0, dim, nrows)])
+    hx = orc.hxv_rows_real64d(m, nup, ndw, rows, x)
+    res = np.abs(hx - e0 * x[rows])
+    return float(res.max()), abs(e0) / np.sqrt(dim), rows, hx
+
+
+def test_sq4x4_fp64_and_mixed_ground_state(orc):
+    H = lattice.square(4, 4)
+    m = orc.Model(H, [8.0])
+    with EDLattice(0) as ed:
+        ed.set_hamiltonian(H, [8.0])
+        isec = ed.sector_index(8, 8)
+        dim, _, _ = ed.build_sector(isec)
+        assert dim == 165636900
+        print("sq4x4 fp64 Lanczos 200 steps", flush=True)
+        r64 = ed.lanczos_eigh(isec, "fp64", niter=200, want_vector=True)
+        o64 = ed.observables()
+        print("sq4x4 mixed Lanczos 200 fp32 steps + fp64 refinement", flush=True)
+        rmx = ed.lanczos_eigh(isec, "mixed", niter=200, want_vector=True, refine_tol=1e-13)
+        omx = ed.observables()
+        x64, xmx = r64["gs"], rmx["gs"]
+        de = abs(rmx["e0"] - r64["e0"]) / abs(r64["e0"])
+        ddocc = np.max(np.abs(omx["docc"] - o64["docc"]))
+        ddens = np.max(np.abs(omx["dens"] - o64["dens"]))
+        print(f"sq4x4 E0 fp64 {r64['e0']:.14f} mixed {rmx['e0']:.14f} (fp32 alone {rmx['e0_fp32']:.14f}); "
+              f"|dE0|/|E0| {de:.2e}; docc fp64 {o64['docc'].mean():.12f} mixed {omx['docc'].mean():.12f} "
+              f"max|ddocc| {ddocc:.2e} max|ddens| {ddens:.2e}; E0 + U N/2 = {r64['e0'] + 64.0:.10f}", flush=True)
+        assert de <= 1e-10
+        assert ddocc <= 1e-8 and ddens <= 1e-8
+        # half filling, one spin-symmetric ground state: n = 1 on every site
+        np.testing.assert_allclose(o64["dens"], 1.0, atol=1e-10)
+        np.testing.assert_allclose(o64["dens_up"], o64["dens_dw"], atol=1e-10)
+        # both eigenpairs are eigenpairs of the oracle's operator (sampled rows)
+        for name, x, e0 in (("fp64", x64, r64["e0"]), ("mixed", xmx, rmx["e0"])):
+            assert abs(np.linalg.norm(x) - 1.0) < 1e-12
+            rmax, scale, rows, hx = _residual_rows(orc, m, 8, 8, x, e0)
+            print(f"sq4x4 {name}: residual rows max {rmax:.3e} (|E0| rms(x) {scale:.3e})", flush=True)
+            assert rmax < 1e-5 * scale
+        # the GPU operator on the eigenvector: same rows as the oracle, and a
+        # small global residual; Rayleigh quotient = E0
+        hx_gpu = ed.direct_matvec(isec, x64, "fp64")
+        assert np.max(np.abs(hx_gpu[rows] - hx)) < 1e-12 * np.max(np.abs(hx))
+        rnorm = np.linalg.norm(hx_gpu - r64["e0"] * x64)
+        rq = float(x64 @ hx_gpu)
+        print(f"sq4x4 fp64 |H x - E0 x| = {rnorm:.3e}, <x|H|x> - E0 = {rq - r64['e0']:.3e}", flush=True)
+        assert rnorm < 1e-4 * abs(r64["e0"])
+        assert abs(rq - r64["e0"]) < 1e-11 * abs(r64["e0"])
+
+
+@pytest.fixture(scope="module")
+def obc3x6(orc):
+    """The 3x6 mixed ground state, installed on a context kept for the seeds."""
+    H = lattice.disordered_rect(3, 6)
+    m = orc.Model(H, [4.0])
+    ed = EDLattice(0)
+    ed.set_hamiltonian(H, [4.0])
+    isec = ed.sector_index(9, 9)
+    dim, _, _ = ed.build_sector(isec)
+    print("obc3x6 mixed ground state", flush=True)
+    r = ed.lanczos_eigh(isec, "mixed", niter=150, tol=1e-9, refine_tol=1e-12)
+    print(f"obc3x6 E0 {r['e0']:.12f} (fp32 {r['e0_fp32']:.12f}), steps fp32 {r['steps_fp32']} "
+          f"fp64 {r['steps_fp64']}, {r['ms'] / 1e3:.1f} s", flush=True)
+    yield ed, m, isec, dim, r
+    ed.close()
+
+
+def test_obc3x6_mixed_ground_state(obc3x6, orc):
+    ed, m, isec, dim, r = obc3x6
+    assert dim == 2363904400
+    x = ed.get_state(dim)
+    assert abs(np.linalg.norm(x) - 1.0) < 1e-10
+    rmax, scale, _, _ = _residual_rows(orc, m, 9, 9, x, r["e0"], nrows=200)
+    print(f"obc3x6 residual rows max {rmax:.3e} (|E0| rms(x) {scale:.3e})", flush=True)
+    assert rmax < 1e-4 * scale
+    assert abs(r["e0"] - r["e0_fp32"]) < 1e-5 * abs(r["e0"])
+
+
+def test_obc3x6_sz_seeds_100_steps(obc3x6):
+    ed, m, isec, dim, r = obc3x6
+    obs = ed.observables()
+    seeds = [("sz", 1, i, 0) for i in range(1, 19)]
+    print("obc3x6 18 S^z seeds x 100 steps (fp32 chains)", flush=True)
+    rs = ed.tridiag_batched(seeds, 100, "fp32")
+    print(f"obc3x6 seeds: {rs['ms'] / 1e3:.1f} s for {rs['hxv']} chain steps "
+          f"({rs['ms'] / max(rs['hxv'], 1):.1f} ms per step)", flush=True)
+    assert np.all(rs["isector"] == isec)
+    assert np.all(rs["nsteps"] == 100)
+    sz2 = 0.25 * (obs["dens_up"] + obs["dens_dw"] - 2.0 * obs["docc"])
+    np.testing.assert_allclose(rs["norm2"], sz2, rtol=1e-10, atol=1e-14)
+    for k in range(18):
+        p, w = ed.lanczos_poles(rs["alpha"][k], rs["beta"][k], rs["norm2"][k], r["e0"], 1)
+        assert abs(w.sum() - rs["norm2"][k]) < 1e-10 * rs["norm2"][k]
+        # Ritz values of a chain in the ground state's own sector are >= E0
+        # (fp32 chains: a few ulps of |E| below at most)
+        assert p.min() > -1e-5 * abs(r["e0"]), (k, p.min())
+
+
+def _both_spin_seeds():
+    seeds = []
+    for ispin in (1, 2):
+        for i in range(1, 13):
+            for j in range(i, 13):
+                jo = 0 if i == j else j
+                seeds += [("cdg", ispin, i, jo), ("c", ispin, i, jo)]
+    return seeds
+
+
+def test_rect3x4_both_spins_batched(orc):
+    """312 seeds (both spins) in one batched call: every chain equals the
+    same seed run alone (leading coefficients), spin-down chains equal the
+    spin-up chains of the mirrored seed (the model is spin symmetric), and
+    a particle and a hole seed of spin down match the oracle's Lanczos."""
+    H = lattice.square(3, 4)
+    m = orc.Model(H, [6.0])
+    with EDLattice(0) as ed:
+        ed.set_hamiltonian(H, [6.0])
+        isec = ed.sector_index(6, 6)
+        r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
+        seeds = _both_spin_seeds()
+        assert len(seeds) == 312
+        rb = ed.tridiag_batched(seeds, 150, "fp64")
+        print(f"rect3x4 312 seeds batched: {rb['ms']:.1f} ms", flush=True)
+        assert set(rb["isector"][:156]) == {ed.sector_index(7, 6), ed.sector_index(5, 6)}
+        assert set(rb["isector"][156:]) == {ed.sector_index(6, 7), ed.sector_index(6, 5)}
+        pick = [0, 1, 40, 155, 156, 157, 200, 311]
+        rs = ed.tridiag_batched([seeds[k] for k in pick], 150, "fp64", group=1)
+        for n, k in enumerate(pick):
+            np.testing.assert_allclose(rb["alpha"][k, :10], rs["alpha"][n, :10], rtol=1e-9, atol=1e-9)
+            np.testing.assert_allclose(rb["beta"][k, 1:10], rs["beta"][n, 1:10], rtol=1e-9, atol=1e-9)
+            assert abs(rb["norm2"][k] - rs["norm2"][n]) < 1e-12
+        # spin flip: the (6,6) ground state is spin symmetric up to a sign, so
+        # the norms and the leading chain coefficients of the mirrored seeds agree
+        np.testing.assert_allclose(rb["norm2"][:156], rb["norm2"][156:], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(rb["alpha"][:156, :8], rb["alpha"][156:, :8], rtol=1e-8, atol=1e-8)
+        for k in (156, 157):
+            op, ispin, io, jo = seeds[k]
+            opc = 2 if op == "cdg" else 1
+            d = 1 if opc == 2 else -1
+            s = orc.seed_c_cdg(12, opc, io, 2, (6, 6), (6, 6 + d), r["gs"]).real
+            assert abs(rb["norm2"][k] - s @ s) < 1e-10
+            ao, bo, _ = orc.lanczos_tridiag(orc.hamiltonian_csr(m, 6, 6 + d), s, 150)
+            np.testing.assert_allclose(rb["alpha"][k, :10], ao[:10], rtol=1e-9, atol=1e-9)
+            np.testing.assert_allclose(rb["beta"][k, 1:10], bo[1:10], rtol=1e-9, atol=1e-9)

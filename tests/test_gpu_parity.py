@@ -470,6 +470,39 @@ def test_tridiag_seed_vs_oracle(ed, orc, batched_kernels):
     np.testing.assert_allclose(_gz(p, w, z), _gz(po, wo, z), atol=1e-8 * n2)
 
 
+def test_tridiag_one_up_string_sectors_vs_oracle(ed, orc, batched_kernels):
+    """Sectors with a few up strings, whose padded column stride (32 rows)
+    is many times DimUp: 3x4 (1,6) (DimUp 12) through tridiag_Hv_sector, and
+    a batched group from the ring8 (2,4) ground state whose c seeds land in
+    (1,4) (DimUp 8) and c^dg seeds in (3,4)."""
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(1, 6)
+    dim, du, _ = ed.build_sector(isec)
+    assert du == 12
+    seed = RNG.standard_normal(dim)
+    a, b, n2 = ed.tridiag_Hv_sector(isec, seed, 60, "fp64")
+    assert abs(n2 - seed @ seed) < 1e-10 * n2
+    ao, bo, _ = orc.lanczos_tridiag(orc.hamiltonian_csr(m, 1, 6), seed, 60)
+    np.testing.assert_allclose(a[:30], ao[:30], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(b[1:30], bo[1:30], rtol=1e-8, atol=1e-8)
+    m = _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(2, 4)
+    r = ed.lanczos_eigh(isec, "fp64", niter=200, want_vector=True)
+    seeds = [("c", 1, 1, 0), ("c", 1, 3, 5), ("cdg", 1, 2, 0), ("c", 2, 4, 0)]
+    rb = ed.tridiag_batched(seeds, 80, "fp64")
+    assert list(rb["isector"]) == [ed.sector_index(1, 4)] * 2 + [ed.sector_index(3, 4), ed.sector_index(2, 3)]
+    for k, (op, ispin, io, jo) in enumerate(seeds):
+        opc = 2 if op == "cdg" else 1
+        d = 1 if opc == 2 else -1
+        nJ = (2 + d, 4) if ispin == 1 else (2, 4 + d)
+        s = orc.seed_c_cdg(8, opc, io, ispin, (2, 4), nJ, r["gs"]).real
+        if jo:
+            s = s + orc.seed_c_cdg(8, opc, jo, ispin, (2, 4), nJ, r["gs"]).real
+        assert abs(rb["norm2"][k] - s @ s) < 1e-10
+        ao, bo, _ = orc.lanczos_tridiag(orc.hamiltonian_csr(m, *nJ), s, 80)
+        _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, 10, 1e-9)
+
+
 @pytest.mark.parametrize("prec,tol,conv_tol,lead,nlead,window", [("fp64", 1e-8, 1e-10, 1e-9, 10, 1e-6),
                                                                  ("fp32", 2e-4, 1e-5, 1e-4, 6, 1e-4)])
 def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, window, batched_kernels):
@@ -702,3 +735,21 @@ def test_lanczos_states_rect3x4(ed, orc):
     for j in range(3):
         hx = orc.matvec(m, 6, 6, X[j], nthreads=8).real
         assert np.linalg.norm(hx - e[j] * X[j]) < 1e-5 * abs(e[j]), j
+
+
+def test_new_model_drops_installed_state(ed, orc):
+    """A new Hamiltonian invalidates the installed ground state: the calls
+    that read it refuse until a state of the new model is installed."""
+    from ed_lattice_amd.solver import EDError
+
+    _use(ed, orc, lattice.ring(8), [4.0])
+    isec = ed.sector_index(4, 4)
+    ed.lanczos_eigh(isec, "fp64", niter=100)
+    ed.observables()
+    _use(ed, orc, lattice.ring(6), [4.0])
+    with pytest.raises(EDError):
+        ed.observables()
+    with pytest.raises(EDError):
+        ed.tridiag_batched([("c", 1, 1, 0)], 20, "fp64")
+    r = ed.lanczos_eigh(ed.sector_index(3, 3), "fp64", niter=100)
+    assert np.isfinite(ed.observables()["docc"]).all() and r["e0"] < 0

@@ -105,6 +105,10 @@ def test_rows_real64_equal_matvec(orc):
     ref = orc.matvec(m, nup, ndw, v.astype(np.float64)).real
     rows = np.arange(D)
     np.testing.assert_allclose(orc.hxv_rows_real64(m, nup, ndw, rows, v), ref, atol=1e-12)
+    # fp64 input: the same rows of the faithful product of that vector
+    v64 = RNG.standard_normal(D)
+    ref64 = orc.matvec(m, nup, ndw, v64).real
+    np.testing.assert_allclose(orc.hxv_rows_real64d(m, nup, ndw, rows, v64), ref64, atol=1e-12)
 
 
 def test_dim_overflow_refused(orc):
