@@ -57,6 +57,7 @@ SIGNATURES = {
     "edl_lanczos_poles": (c_int, [c_int, _dp, _dp, c_double, c_double, c_int, _dp, _dp]),
     "edl_observables": (c_int, [c_void_p, _dp, _dp, _dp]),
     "edl_local_energy": (c_int, [c_void_p, _dp]),
+    "edl_correlations": (c_int, [c_void_p, _dp, _dp]),
     "edl_last_timing": (c_int, [c_void_p, _dp, _i64p]),
     "edl_bench_hxv": (c_int, [c_void_p, c_int, c_int, c_int, _dp]),
     "edl_bench_passes": (c_int, [c_void_p, c_int, c_int, c_int, _dp, _ip]),

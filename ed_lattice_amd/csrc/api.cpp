@@ -35,6 +35,12 @@ Sector& get_sector(Ctx& c, int isector) {
   s->Ndw = (isector - 1) % (Ns + 1);
   s->Nup = ((isector - 1) / (Ns + 1)) % (Ns + 1);
   build_sector_tables(c, *s);
+  try {
+    build_up_color_tables(*s, c.stream);
+  } catch (...) {
+    free_sector(*s);
+    throw;
+  }
   Sector& ref = *s;
   c.sectors[isector] = std::move(s);
   return ref;
@@ -171,6 +177,7 @@ static void with_chain_shapes(TwoPassCfg& dst, const TwoPassCfg& src) {
   dst.ch_grid_up = src.ch_grid_up;
   dst.ch_up_rpt = src.ch_up_rpt;
   dst.ch_up_kch = src.ch_up_kch;
+  dst.ch_up_col = src.ch_up_col;
 }
 
 // Pick panel widths, block sizes and grids of the two passes by timing the
@@ -184,16 +191,20 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   TwoPassCfg best;
   bool pinned = false;
   // EDL_TWO_PASS_CFG_F32 / _F64 =
-  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec]]]"
+  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col]]]]"
   // pins the operator schedule of that precision (reproducible profiles of
   // one configuration), ahead of any tuned schedule; the operator is not
   // tuned then, its chain shapes still are (once per sector shape).
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
-                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec);
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
+                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec,
+                              &p.up_col);
+    EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T)),
+              "EDL_TWO_PASS_CFG_F32/_F64: up_col needs fp32 panels of 1 or 3 columns on a sector with the "
+              "bank-scheduled table");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
-                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec]]]");
+                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
               "EDL_TWO_PASS_CFG_F32/_F64: (up_rpt, up_kch) must be (1, 8|16) or (2, 4|8)");
     EDL_CHECK(pass_dw_lds_bytes(V, p) <= (size_t)(160 * 1024 - 4096),
@@ -240,6 +251,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     best.grid_up = c.num_cu;
     best.nchunk = std::min(32, V.dimdw);
     double best_up = 1e300, best_dw = 1e300;
+    for (int col : {0, 1})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
       for (int th : {512, 1024})
@@ -247,8 +259,10 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
           if (need > (size_t)lds) continue;
           if (gm == 2 && 2 * need > (size_t)lds) continue;
+          if (col && !up_col_ok(V, C, sizeof(T))) continue;
           TwoPassCfg w = best;
           w.C = C;
+          w.up_col = col;
           w.up_rpt = rk / 100;
           w.up_kch = rk % 100;
           w.threads_up = th;
@@ -259,11 +273,12 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), q.as<T>(), t.as<T>(), nullptr, w, st);
           const double ms = tm.stop();
           if (verbose())
-            std::fprintf(stderr, "[edl] tune up  fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
-                         8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
+            std::fprintf(stderr, "[edl] tune up  fp%d C %d col %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
+                         8 * (int)sizeof(T), C, col, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
           if (ms < best_up) {
             best_up = ms;
             best.C = C;
+            best.up_col = col;
             best.up_rpt = w.up_rpt;
             best.up_kch = w.up_kch;
             best.threads_up = th;
@@ -373,14 +388,17 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     DevBuf pu(sizeof(double) * (size_t)c.num_cu * 2);
     A.part_up = pu.as<double>();
     double bu2 = 1e300;
+    for (int col : {0, 1})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
         for (int th : {512, 1024})
           for (int gm : {1, 2}) {
             const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
             if (need > (size_t)lds || (gm == 2 && 2 * need > (size_t)lds)) continue;
+            if (col && !up_col_ok(V, C, sizeof(T))) continue;
             TwoPassCfg w = best;
             w.C = C;
+            w.up_col = col;
             w.up_rpt = rk / 100;
             w.up_kch = rk % 100;
             w.threads_up = th;
@@ -392,11 +410,12 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), q.as<T>(), o.as<T>(), t.as<T>(), &A, w, st);
             const double ms = tm.stop();
             if (verbose())
-              std::fprintf(stderr, "[edl] tune up chain fp%d C %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
-                           8 * (int)sizeof(T), C, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
+              std::fprintf(stderr, "[edl] tune up chain fp%d C %d col %d rpt %d kch %2d threads %4d grid %5d: %.4f ms\n",
+                           8 * (int)sizeof(T), C, col, w.up_rpt, w.up_kch, th, w.grid_up, ms / 3);
             if (ms < bu2) {
               bu2 = ms;
               best.ch_C = C;
+              best.ch_up_col = col;
               best.ch_up_threads = th;
               best.ch_grid_up = w.grid_up;
               best.ch_up_rpt = w.up_rpt;
@@ -1975,6 +1994,16 @@ extern "C" int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, d
   API_END
 }
 
+extern "C" int edl_correlations(edl_ctx* ctx, double* sz2, double* n2) {
+  API_BEGIN
+  Ctx& c = ctx->c;
+  EDL_CHECK(sz2 && n2, "edl_correlations: null output");
+  EDL_HIP(hipSetDevice(c.device));
+  Sector& S = gs_sector_of(c, "edl_correlations");
+  device_correlations(c, S, c.d_gs, sz2, n2);
+  API_END
+}
+
 extern "C" int edl_local_energy(edl_ctx* ctx, double* e) {
   API_BEGIN
   Ctx& c = ctx->c;
@@ -2085,10 +2114,10 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   ms_pass[1] = (two || seg) ? dw / reps : 0.0;
   // the up pass takes its coefficients from registers in fp32 on one-value lattices
   const int32_t lreg = (two && sizeof(T) == 4 && s.sp[0].nvals == 1) ? 1 : 0;
-  const int32_t out[14] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+  const int32_t out[15] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
-                           cf.dw_vec,                lreg};
-  for (int k = 0; k < 14; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
+                           cf.dw_vec,                lreg,      two ? cf.up_col : 0};
+  for (int k = 0; k < 15; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

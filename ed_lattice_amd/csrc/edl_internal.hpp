@@ -65,6 +65,20 @@ struct InterParams {
   double Jx, Jp;
 };
 
+// Bank-scheduled hop table of the up pass: per block of 64 strings, slots
+// [ns[b]][64] of packed words from word off[b] on, ordered so that in every
+// slot the 32 lanes of each half-wave read LDS words in distinct banks (bank
+// = W * nbr mod 32 for a staged row of W dwords; a bipartite edge colouring
+// of lanes x banks, which needs no more slots than the larger of the
+// busiest lane's and the busiest bank's hop count).  Slots a lane does not
+// use hold cid 255 and the address of a bank that slot leaves free.
+struct ColorTable {
+  uint32_t* words = nullptr;
+  int32_t* off = nullptr;      // [nblocks]
+  int32_t* ns = nullptr;       // [nblocks]
+  int64_t slots = 0;           // sum of ns (table size / 64 words)
+};
+
 // One spin species of one sector: sorted strings, hopping table, diagonal.
 struct SpinTable {
   int N = 0;           // electrons of this spin
@@ -91,6 +105,9 @@ struct SpinTable {
   int32_t* off = nullptr;      // [dim+1]
   double* d_lut64 = nullptr;   // device copies of the coefficient lut
   float* d_lut32 = nullptr;
+  // Bank-scheduled copies of the packed table for the up pass, one per LDS
+  // element width W (dwords per staged row: 1 or 3), built on first use.
+  ColorTable pkc[4];
 };
 
 struct Sector {

@@ -79,12 +79,12 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
 
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false>
 static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
                                                 typename Vec<T, C>::type* P, const T* __restrict__ q,
                                                 T* __restrict__ tout, int c0, T ts) {
@@ -113,7 +113,7 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
     }
   }
   __syncthreads();
-  up_rows<T, C, RPT, UKCH, LREG>(V, I, lut, P, tout, c0, ts);
+  up_rows<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, tout, c0, ts);
 }
 
 // The row loop of a panel already staged in P: RPT rows per thread, hop
@@ -121,14 +121,19 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
 // before the LDS gathers that depend on them.  LREG: the spin has one
 // distinct hopping value, the coefficient of cid 0 / 1 is lut[0] / lut[1]
 // (held in registers) and padding (cid 255) is 0, so a hop costs no LDS
-// read besides its gather.
-template <typename T, int C, int RPT, int UKCH, bool LREG>
+// read besides its gather.  COL: the hops come in the slots of the bank-
+// scheduled table (ColorTable, width W = C * sizeof(T) / 4): in every slot
+// the 32 lanes of a half-wave gather from distinct LDS banks; every row of a
+// 64-row block runs the block's slot count (unused slots: coefficient 0).
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts) {
   typedef typename Vec<T, C>::type VT;
+  constexpr int W = C * (int)sizeof(T) / 4;
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
+  const size_t tstride = COL ? 64 : (size_t)dimup;
   uint32_t sd[C];
   double dgd[C];
 #pragma unroll
@@ -141,6 +146,7 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
   const int stride = RPT * blockDim.x;
   for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
     int row[RPT], nh[RPT];
+    const uint32_t* tb[RPT];
     bool okr[RPT];
     int nmax = 0;
     T acc[RPT][C];
@@ -149,7 +155,14 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       row[h] = i0 + h * (int)blockDim.x;
       okr[h] = row[h] < dimup;
       if (!okr[h]) row[h] = i0;
-      nh[h] = okr[h] ? V.cnt_up[row[h]] : 0;
+      if (COL) {
+        const int b = row[h] >> 6;
+        nh[h] = okr[h] ? V.pkc_ns[W][b] : 0;
+        tb[h] = V.pkc_up[W] + V.pkc_off[W][b] + (row[h] & 63);
+      } else {
+        nh[h] = okr[h] ? V.cnt_up[row[h]] : 0;
+        tb[h] = V.pk_up + row[h];
+      }
       nmax = nh[h] > nmax ? nh[h] : nmax;
       const uint32_t su = (uint32_t)V.map_up[row[h]];
       const double dgu = V.dg_up[row[h]];
@@ -164,7 +177,7 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       for (int h = 0; h < RPT; ++h)
 #pragma unroll
         for (int j = 0; j < UKCH; ++j)
-          w[h][j] = (k0 + j < nh[h]) ? V.pk_up[(size_t)(k0 + j) * dimup + row[h]] : (255u << 24);
+          w[h][j] = (k0 + j < nh[h]) ? tb[h][(size_t)(k0 + j) * tstride] : (255u << 24);
 #pragma unroll
       for (int h = 0; h < RPT; ++h)
 #pragma unroll
@@ -245,7 +258,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C, int RPT, int UKCH, bool LREG = false>
+template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = false>
 __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
@@ -257,7 +270,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   if (!chain) {
     for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
-      up_panel<T, C, RPT, UKCH, LREG>(V, I, lut, P, qa, tout, pan * C, (T)1);
+      up_panel<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
   }
   const LzScalars s = *L.sc;
@@ -279,7 +292,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     __syncthreads();
     nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx, bst);
     __syncthreads();
-    if (!last) up_rows<T, C, RPT, UKCH, LREG>(V, I, lut, P, tout, pan * C, (T)1);
+    if (!last) up_rows<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, tout, pan * C, (T)1);
   }
   if (pend && !last) {
     const double v = block_sum(nrm, sh);
@@ -650,6 +663,149 @@ template void launch_basis_combine<float>(const float*, const TwoPassView&, cons
 // ---------------------------------------------------------------- host side
 static int lds_limit_bytes() { return 160 * 1024; }
 
+// Edge colouring of a bipartite multigraph (lanes x banks) with as many
+// colours as its largest degree (Konig's theorem): each edge takes a colour
+// free at both ends, after swapping two colours along an alternating path
+// when the free colours differ.  color[e] receives the slot of edge e.
+static int color_bipartite(const std::vector<int>& el, const std::vector<int>& ek, int nl, int nk,
+                           std::vector<int>& color) {
+  const int ne = (int)el.size();
+  std::vector<int> dl(nl, 0), dk(nk, 0);
+  for (int e = 0; e < ne; ++e) {
+    ++dl[el[e]];
+    ++dk[ek[e]];
+  }
+  int D = 0;
+  for (int v : dl) D = std::max(D, v);
+  for (int v : dk) D = std::max(D, v);
+  color.assign(ne, -1);
+  if (D == 0) return 0;
+  std::vector<int> al((size_t)nl * D, -1), ak((size_t)nk * D, -1);
+  std::vector<int> path;
+  for (int e = 0; e < ne; ++e) {
+    const int l = el[e], k = ek[e];
+    int a = 0, b = 0;
+    while (al[(size_t)l * D + a] >= 0) ++a;
+    while (ak[(size_t)k * D + b] >= 0) ++b;
+    if (ak[(size_t)k * D + a] >= 0) {
+      // the a/b alternating path from bank k: k -a- l1 -b- k2 -a- ...
+      path.clear();
+      int at = k, col = a;
+      bool on_bank = true;
+      for (;;) {
+        const int f = on_bank ? ak[(size_t)at * D + col] : al[(size_t)at * D + col];
+        if (f < 0) break;
+        path.push_back(f);
+        at = on_bank ? el[f] : ek[f];
+        on_bank = !on_bank;
+        col = (col == a) ? b : a;
+      }
+      for (int f : path) {
+        al[(size_t)el[f] * D + color[f]] = -1;
+        ak[(size_t)ek[f] * D + color[f]] = -1;
+      }
+      for (int f : path) {
+        color[f] = (color[f] == a) ? b : a;
+        al[(size_t)el[f] * D + color[f]] = f;
+        ak[(size_t)ek[f] * D + color[f]] = f;
+      }
+    }
+    EDL_CHECK(al[(size_t)l * D + a] < 0 && ak[(size_t)k * D + a] < 0, "color_bipartite: no common free colour");
+    color[e] = a;
+    al[(size_t)l * D + a] = e;
+    ak[(size_t)k * D + a] = e;
+  }
+  return D;
+}
+
+// The bank-scheduled table of W-dword rows (see ColorTable): each half-wave
+// group of 32 strings is coloured on its own, the block of 64 takes the
+// larger slot count of its two halves.
+static void build_color_table(SpinTable& t, int W, hipStream_t st) {
+  ColorTable& ct = t.pkc[W];
+  if (ct.words || !t.packed_ok || t.dim < 1) return;
+  const int dim = t.dim, kmax = t.kmax;
+  std::vector<uint32_t> pk((size_t)std::max(kmax, 1) * dim);
+  std::vector<int32_t> cnt(dim);
+  EDL_HIP(hipMemcpyAsync(pk.data(), t.packed, sizeof(uint32_t) * pk.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(cnt.data(), t.cnt, sizeof(int32_t) * dim, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  const int nb = (dim + 63) / 64;
+  std::vector<int32_t> off(nb), ns(nb);
+  std::vector<uint32_t> words;
+  std::vector<int> el, ek, color;
+  std::vector<uint32_t> ew;
+  for (int b = 0; b < nb; ++b) {
+    int nsh[2] = {0, 0};
+    std::vector<int> col[2];
+    std::vector<uint32_t> wds[2];
+    std::vector<int> lanes[2];
+    for (int h = 0; h < 2; ++h) {
+      el.clear();
+      ek.clear();
+      ew.clear();
+      for (int l = 0; l < 32; ++l) {
+        const int r = b * 64 + h * 32 + l;
+        if (r >= dim) break;
+        for (int k = 0; k < cnt[r]; ++k) {
+          const uint32_t w = pk[(size_t)k * dim + r];
+          el.push_back(l);
+          ek.push_back((int)(((int64_t)W * (w & 0xFFFFFFu)) % 32));
+          ew.push_back(w);
+        }
+      }
+      nsh[h] = color_bipartite(el, ek, 32, 32, color);
+      col[h] = color;
+      wds[h] = ew;
+      lanes[h] = el;
+    }
+    const int nsb = std::max(nsh[0], nsh[1]);
+    off[b] = (int32_t)words.size();
+    ns[b] = nsb;
+    EDL_CHECK(words.size() + (size_t)nsb * 64 < ((size_t)1 << 31), "build_color_table: table exceeds 2^31 words");
+    words.resize(words.size() + (size_t)nsb * 64, 0u);
+    uint32_t* blk = words.data() + off[b];
+    for (int h = 0; h < 2; ++h) {
+      std::vector<char> used((size_t)nsb * 32, 0);
+      std::vector<char> filled((size_t)nsb * 32, 0);
+      for (size_t e = 0; e < wds[h].size(); ++e) {
+        const int s = col[h][e], l = lanes[h][e];
+        blk[(size_t)s * 64 + h * 32 + l] = wds[h][e];
+        filled[(size_t)s * 32 + l] = 1;
+        used[(size_t)s * 32 + (int)(((int64_t)W * (wds[h][e] & 0xFFFFFFu)) % 32)] = 1;
+      }
+      for (int s = 0; s < nsb; ++s) {
+        // padding: cid 255 at one address of a bank this slot leaves free
+        int freebank = 0;
+        while (freebank < 32 && used[(size_t)s * 32 + freebank]) ++freebank;
+        uint32_t padrow = 0;
+        if (freebank < 32)
+          for (int r = 0; r < std::min(dim, 32 * W + 32); ++r)
+            if ((W * r) % 32 == freebank) {
+              padrow = (uint32_t)r;
+              break;
+            }
+        for (int l = 0; l < 32; ++l)
+          if (!filled[(size_t)s * 32 + l]) blk[(size_t)s * 64 + h * 32 + l] = padrow | (255u << 24);
+      }
+    }
+  }
+  ct.slots = (int64_t)words.size() / 64;
+  EDL_HIP(hipMalloc(&ct.words, sizeof(uint32_t) * std::max<size_t>(words.size(), 1)));
+  EDL_HIP(hipMalloc(&ct.off, sizeof(int32_t) * nb));
+  EDL_HIP(hipMalloc(&ct.ns, sizeof(int32_t) * nb));
+  EDL_HIP(hipMemcpyAsync(ct.words, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice, st));
+  EDL_HIP(hipMemcpyAsync(ct.off, off.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+  EDL_HIP(hipMemcpyAsync(ct.ns, ns.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+  EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
+}
+
+void build_up_color_tables(Sector& s, hipStream_t st) {
+  if (s.nonlocal || !s.sp[0].packed_ok || s.sp[0].dim < 64) return;
+  build_color_table(s.sp[0], 1, st);
+  build_color_table(s.sp[0], 3, st);
+}
+
 static size_t up_lds_bytes(int dimup, int C, size_t sz) {
   const size_t head = 256 * sz;
   return head + (16 - head % 16) % 16 + (size_t)dimup * C * sz;
@@ -681,39 +837,58 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   v.dg_dw = s.sp[1].diag;
   v.csr_dw = s.sp[1].csr;
   v.off_dw = s.sp[1].off;
+  for (int W = 0; W < 4; ++W) {
+    v.pkc_up[W] = s.sp[0].pkc[W].words;
+    v.pkc_off[W] = s.sp[0].pkc[W].off;
+    v.pkc_ns[W] = s.sp[0].pkc[W].ns;
+  }
   return v;
 }
 
-template <typename T, int C, int RPT, int UKCH, bool LREG>
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL>
 static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
   const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
-  set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG>, lds_limit_bytes() - 256);
-  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
-                     L ? *L : LzChain{}, L ? 1 : 0);
+  set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG, COL>, lds_limit_bytes() - 256);
+  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG, COL>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb,
+                     tout, L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
+}
+
+bool up_col_ok(const TwoPassView& V, int C, size_t elem) {
+  const int W = C * (int)elem / 4;
+  return elem == 4 && (C == 1 || C == 3) && V.pkc_up[W] != nullptr;
 }
 
 template <typename T, int C, int RPT, int UKCH>
 static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                        const LzChain* L, int grid, int threads, hipStream_t st) {
+                        const LzChain* L, int grid, int threads, bool col, hipStream_t st) {
   // fp32 only: in fp64 the two coefficient registers cost more than the
   // LDS reads they save (measured slower)
-  if constexpr (sizeof(T) == 4) {
-    if (V.nv_up == 1) return launch_up_l<T, C, RPT, UKCH, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+  if constexpr (sizeof(T) == 4 && (C == 1 || C == 3)) {
+    if (col) {
+      EDL_CHECK(up_col_ok(V, C, sizeof(T)), "launch_pass_up: no bank-scheduled table for this panel width");
+      if (V.nv_up == 1) return launch_up_l<T, C, RPT, UKCH, true, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+      return launch_up_l<T, C, RPT, UKCH, false, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+    }
   }
-  launch_up_l<T, C, RPT, UKCH, false>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+  EDL_CHECK(!col, "launch_pass_up: the bank-scheduled table takes fp32 panels of 1 or 3 columns");
+  if constexpr (sizeof(T) == 4) {
+    if (V.nv_up == 1) return launch_up_l<T, C, RPT, UKCH, true, false>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+  }
+  launch_up_l<T, C, RPT, UKCH, false, false>(V, I, lut, qa, qb, tout, L, grid, threads, st);
 }
 
 template <typename T, int C>
 static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, const TwoPassCfg& cfg, hipStream_t st) {
   const int th = cfg.threads_up;
+  const bool col = cfg.up_col != 0;
   switch (cfg.up_rpt * 100 + cfg.up_kch) {
-    case 108: return launch_up_k<T, C, 1, 8>(V, I, lut, qa, qb, tout, L, grid, th, st);
-    case 116: return launch_up_k<T, C, 1, 16>(V, I, lut, qa, qb, tout, L, grid, th, st);
-    case 208: return launch_up_k<T, C, 2, 8>(V, I, lut, qa, qb, tout, L, grid, th, st);
-    case 204: return launch_up_k<T, C, 2, 4>(V, I, lut, qa, qb, tout, L, grid, th, st);
+    case 108: return launch_up_k<T, C, 1, 8>(V, I, lut, qa, qb, tout, L, grid, th, col, st);
+    case 116: return launch_up_k<T, C, 1, 16>(V, I, lut, qa, qb, tout, L, grid, th, col, st);
+    case 208: return launch_up_k<T, C, 2, 8>(V, I, lut, qa, qb, tout, L, grid, th, col, st);
+    case 204: return launch_up_k<T, C, 2, 4>(V, I, lut, qa, qb, tout, L, grid, th, col, st);
     default: throw Error("launch_pass_up: unsupported rows per thread / hop chunk");
   }
 }

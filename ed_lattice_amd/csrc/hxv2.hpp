@@ -18,6 +18,10 @@ struct TwoPassView {
   const double* dg_dw;
   const uint32_t* csr_dw;  // down-spin packed words grouped by column (no padding)
   const int32_t* off_dw;   // [dimdw+1] their offsets
+  // bank-scheduled up-spin tables by staged row width W (SpinTable::pkc), or null
+  const uint32_t* pkc_up[4];
+  const int32_t* pkc_off[4];
+  const int32_t* pkc_ns[4];
 };
 
 // One Lanczos chain driven by the two passes.  A step is four launches:
@@ -50,6 +54,7 @@ struct TwoPassCfg {
   int grid_up = 256;
   int up_rpt = 2;       // rows per thread in the up pass (1 or 2)
   int up_kch = 8;       // hop chunk of the up pass ((1, 8|16) or (2, 4|8))
+  int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3)
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
@@ -61,7 +66,7 @@ struct TwoPassCfg {
   // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
   int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0;
   // Up-pass shape of the chains (phase B fused), likewise; ch_C = 0: the operator's.
-  int ch_C = 0, ch_up_threads = 0, ch_grid_up = 0, ch_up_rpt = 0, ch_up_kch = 0;
+  int ch_C = 0, ch_up_threads = 0, ch_grid_up = 0, ch_up_rpt = 0, ch_up_kch = 0, ch_up_col = 0;
 };
 
 // The schedule a Lanczos chain runs: the operator's, with the down-pass
@@ -82,9 +87,16 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
     r.grid_up = c.ch_grid_up;
     r.up_rpt = c.ch_up_rpt;
     r.up_kch = c.ch_up_kch;
+    r.up_col = c.ch_up_col;
   }
   return r;
 }
+
+// Whether the bank-scheduled up-pass table exists for panels of C columns of
+// T (staged rows of an odd number of dwords: fp32 with C = 1 or 3).
+bool up_col_ok(const TwoPassView& V, int C, size_t elem);
+// Builds the bank-scheduled up-spin tables of a two-pass sector (once).
+void build_up_color_tables(Sector& s, hipStream_t st);
 
 // Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
 // LzChain::part_dw; blocks of an up-pass launch, the size of part_up.

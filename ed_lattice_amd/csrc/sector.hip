@@ -301,6 +301,11 @@ void free_sector(Sector& s) {
     if (t.off) (void)hipFree(t.off);
     if (t.d_lut64) (void)hipFree(t.d_lut64);
     if (t.d_lut32) (void)hipFree(t.d_lut32);
+    for (auto& ct : t.pkc) {
+      if (ct.words) (void)hipFree(ct.words);
+      if (ct.off) (void)hipFree(ct.off);
+      if (ct.ns) (void)hipFree(ct.ns);
+    }
     t = SpinTable{};
   }
 }

@@ -200,6 +200,86 @@ void device_observables(Ctx& c, const Sector& s, const double* gs, double* dens_
   }
 }
 
+// A[idw * 32 + p] = sum over the rows of column idw of |gs|^2 * (bit p of
+// the up string): the up/down cross term of <n_i n_j> and <Sz_i Sz_j>.
+template <int NS>
+__global__ void k_colbits(const double* __restrict__ gs, const int32_t* __restrict__ map_up, int dimup,
+                          double* __restrict__ A) {
+  __shared__ double sh[NS][16];
+  const int idw = blockIdx.x;
+  double acc[NS];
+#pragma unroll
+  for (int p = 0; p < NS; ++p) acc[p] = 0.0;
+  for (int i = threadIdx.x; i < dimup; i += blockDim.x) {
+    const double g = gs[(size_t)idw * dimup + i];
+    const uint32_t su = (uint32_t)map_up[i];
+#pragma unroll
+    for (int p = 0; p < NS; ++p) acc[p] += (double)((su >> p) & 1u) * g * g;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int p = 0; p < NS; ++p) {
+    double v = acc[p];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[p][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += sh[threadIdx.x][w];
+    A[(size_t)idw * NS + threadIdx.x] = t;
+  }
+}
+
+// <Sz_i Sz_j> and <n_i n_j> of the installed state (the sz2 and n2 of
+// lanc_observables, OBSERVABLES.f90:142-150) from three contractions of
+// W = |gs|^2: UU_ij = sum_up rowsum * u_i u_j, DD_ij = sum_dw colsum * d_i d_j
+// and UD_ij = sum W u_i d_j; n2 = UU + DD + UD + UD^T, sz2 = (UU + DD - UD -
+// UD^T) / 4.  Outputs Ns x Ns, column-major.
+void device_correlations(Ctx& c, const Sector& s, const double* gs, double* sz2, double* n2) {
+  const int Ns = c.model.Ns;
+  hipStream_t st = c.stream;
+  const int dimup = s.sp[0].dim, dimdw = s.sp[1].dim;
+  struct Buf {  // freed on every exit, errors included
+    double* p = nullptr;
+    explicit Buf(size_t n) { EDL_HIP(hipMalloc(&p, sizeof(double) * n)); }
+    ~Buf() { if (p) (void)hipFree(p); }
+  } rs(dimup), cs(dimdw), A((size_t)32 * dimdw);
+  hipLaunchKernelGGL(k_rowsum, dim3((dimup + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gs, dimup, dimdw, rs.p);
+  hipLaunchKernelGGL(k_colsum, dim3(dimdw), dim3(kBlock), 0, st, gs, dimup, dimdw, cs.p);
+  hipLaunchKernelGGL(k_colbits<32>, dim3(dimdw), dim3(kBlock), 0, st, gs, s.sp[0].map, dimup, A.p);
+  EDL_HIP(hipGetLastError());
+  std::vector<double> hr(dimup), hc(dimdw), ha((size_t)32 * dimdw);
+  std::vector<int32_t> mu(dimup), md(dimdw);
+  EDL_HIP(hipMemcpyAsync(hr.data(), rs.p, sizeof(double) * dimup, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(hc.data(), cs.p, sizeof(double) * dimdw, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(ha.data(), A.p, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(mu.data(), s.sp[0].map, sizeof(int32_t) * dimup, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(md.data(), s.sp[1].map, sizeof(int32_t) * dimdw, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  std::vector<double> uu((size_t)Ns * Ns, 0.0), dd((size_t)Ns * Ns, 0.0), ud((size_t)Ns * Ns, 0.0);
+  for (int a = 0; a < dimup; ++a)
+    for (int i = 0; i < Ns; ++i)
+      if ((mu[a] >> i) & 1)
+        for (int j = 0; j < Ns; ++j)
+          if ((mu[a] >> j) & 1) uu[(size_t)i * Ns + j] += hr[a];
+  for (int b = 0; b < dimdw; ++b)
+    for (int i = 0; i < Ns; ++i) {
+      const double ai = ha[(size_t)b * 32 + i];
+      for (int j = 0; j < Ns; ++j)
+        if ((md[b] >> j) & 1) {
+          ud[(size_t)i * Ns + j] += ai;
+          if ((md[b] >> i) & 1) dd[(size_t)i * Ns + j] += hc[b];
+        }
+    }
+  for (int j = 0; j < Ns; ++j)
+    for (int i = 0; i < Ns; ++i) {
+      const size_t ij = (size_t)i * Ns + j, ji = (size_t)j * Ns + i;
+      n2[(size_t)j * Ns + i] = uu[ij] + dd[ij] + ud[ij] + ud[ji];
+      sz2[(size_t)j * Ns + i] = 0.25 * (uu[ij] + dd[ij] - ud[ij] - ud[ji]);
+    }
+}
+
 // ------------------------------------------------------------- energies
 struct EnergyParams {
   InterParams I;                 // U masks, same-site pairs, Ust, Jx, Jp

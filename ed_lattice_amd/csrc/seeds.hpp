@@ -28,6 +28,8 @@ struct CurrentSeed {
 };
 void launch_current_seed(const CurrentSeed& J, const double* gs, const Sector& s, double* out, hipStream_t st);
 void device_observables(Ctx& c, const Sector& s, const double* gs, double* dens_up, double* dens_dw, double* docc);
+// <Sz_i Sz_j> and <n_i n_j> (Ns x Ns, column-major) of one state of weight 1.
+void device_correlations(Ctx& c, const Sector& s, const double* gs, double* sz2, double* n2);
 // lanc_energy (ENERGY.f90:70-353) of one state of weight 1, in the column
 // order of energy_last.ed (ENERGY.f90:654-657): <K>, <Hi>, <V> = <Hi> - Ehf,
 // <Eloc>, <Ehf>, Dust, Dund, Dse, Dph, Dkxy, Dkz (the last two are never

@@ -21,6 +21,7 @@ module ED_GPU
   public :: edl_device_alloc, edl_copy_h2d, edl_copy_d2h, edl_vec_random, edl_vec_dot
   public :: edl_lanczos_gs, edl_lanczos_stats, edl_set_state, edl_get_state
   public :: edl_tridiag, edl_tridiag_batched, edl_lanczos_poles, edl_observables, edl_local_energy
+  public :: edl_correlations
   public :: edl_lanczos_states
   public :: edl_last_timing, edl_bench_hxv, edl_bench_passes
   public :: edl_check, edl_error_message
@@ -228,6 +229,12 @@ module ED_GPU
        real(c_double) :: energies(*)
        type(c_ptr), value :: vecs
      end function edl_lanczos_states
+
+     integer(c_int) function edl_correlations(ctx, sz2, n2) bind(C, name="edl_correlations")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: sz2(*), n2(*)
+     end function edl_correlations
 
      integer(c_int) function edl_local_energy(ctx, e) bind(C, name="edl_local_energy")
        import :: c_int, c_ptr, c_double

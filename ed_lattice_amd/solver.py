@@ -220,12 +220,12 @@ class EDLattice:
     def bench_passes(self, isector, prec="fp32", reps=10):
         """Mean ms of each kernel launch of the operator and its schedule."""
         ms = np.zeros(2)
-        cfg = np.zeros(14, np.int32)
+        cfg = np.zeros(15, np.int32)
         self._ck(self.lib.edl_bench_passes(self.h, isector, PREC[prec], reps, ms.ctypes.data_as(_lib._dp),
                                            cfg.ctypes.data_as(_lib._ip)))
         keys = ("two_pass", "ld", "up_cols", "up_threads", "up_grid", "dw_rows", "dw_threads", "dw_chunks",
                 "dw_cols_per_lane", "dw_hop_chunk", "up_rows_per_thread", "up_hop_chunk", "dw_vector_gather",
-                "up_coef_registers")
+                "up_coef_registers", "up_bank_scheduled")
         return float(ms[0]), float(ms[1]), dict(zip(keys, (int(x) for x in cfg)))
 
     def last_timing(self):
@@ -273,6 +273,14 @@ class EDLattice:
         v = np.zeros(dim)
         self._ck(self.lib.edl_get_state(self.h, _dp(v)))
         return v
+
+    def correlations(self):
+        """<Sz_i Sz_j> and <n_i n_j> (Ns x Ns) of the installed state."""
+        sz2 = np.zeros((self.Ns, self.Ns), order="F")
+        n2 = np.zeros((self.Ns, self.Ns), order="F")
+        self._ck(self.lib.edl_correlations(self.h, sz2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           n2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(sz2=sz2, n2=n2)
 
     def observables(self):
         """dens_up, dens_dw, docc per site of the installed ground state."""

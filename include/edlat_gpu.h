@@ -159,6 +159,11 @@ int edl_lanczos_poles(int nlanc, const double* alpha, const double* beta, double
  * dens_up, dens_dw, docc per site (weight 1). */
 int edl_observables(edl_ctx* ctx, double* dens_up, double* dens_dw, double* docc);
 
+/* The two-site correlations of lanc_observables (OBSERVABLES.f90:142-150,
+ * written to Sz_corr.ed and N_corr.ed at :337-347) of the installed state
+ * (weight 1): sz2[i + j*Ns] = <Sz_i Sz_j>, n2[i + j*Ns] = <n_i n_j>. */
+int edl_correlations(edl_ctx* ctx, double* sz2, double* n2);
+
 /* lanc_energy (ENERGY.f90:70-353) of the installed ground state (weight 1):
  * e[11] in the column order of energy_last.ed (ENERGY.f90:654-657): <K>,
  * <Hi>, <V> = <Hi> - <Ehf>, <Eloc>, <Ehf>, Dust, Dund, Dse, Dph, Dkxy, Dkz
@@ -178,14 +183,15 @@ int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_
 /* Per-kernel split of the same operator: ms_pass[0] and ms_pass[1] are the
  * mean durations of the up-pass and down-pass launches (HIP events around
  * each launch on the launch stream; a single-pass sector reports its one
- * kernel in ms_pass[0] and 0 in ms_pass[1]).  cfg[14] receives the tuned
+ * kernel in ms_pass[0] and 0 in ms_pass[1]).  cfg[15] receives the tuned
  * schedule: {two_pass (1 = LDS column panels, 2 = segmented up pass for
  * columns larger than LDS, 0 = single pass), column stride, up columns per
  * block, up threads, up grid, down panel rows, down threads, down column
  * chunks, down columns per lane, down hop chunk, up rows per thread, up hop
  * chunk, down-pass gathers of 16 bytes per lane (1) or one element per lane
  * (0), up-pass hop coefficients from registers (1: fp32, one distinct
- * hopping value) or from the LDS table (0)}. */
+ * hopping value) or from the LDS table (0), up-pass hops in the bank-scheduled
+ * order (1) or in table order (0)}. */
 int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg);
 
 #ifdef __cplusplus

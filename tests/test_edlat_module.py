@@ -1,0 +1,273 @@
+"""The EDLAT Fortran module (ed_lattice_amd/fortran/edlat.f90) through its
+driver: the reference's entry points (EDlattice.f90:1-51, MAIN.f90:34-108)
+and outputs, against exact Lehmann sums of the oracle's dense sectors.
+
+CPU: the module exports the reference's API.
+GPU (6-site ring, fp64 engine):
+  - off-diagonal G_ij = (G_mix - G_ii - G_jj)/2 (GF_ELECTRONS.f90:160-180)
+    on the Matsubara and real axes (G(w + i eps), :472-512), chi_ii(tau)
+    with the Ltau/2 symmetrisation (CHI_SPIN_ELECTRONS.f90:435-441), chiT.ed
+    (CHI_FUNCTIONS.f90:57-67), and both restart files read back with the
+    record layout of read_formatted_gfmatrix (VARS_GLOBAL.f90:611-632);
+  - the T = 0 sector loop with gs_threshold (DIAG.f90:264-280) on a
+    degenerate ground manifold, LANC_METHOD=arpack (two states per sector)
+    and ED_TWIN (DIAG.f90:132-133, EIGENSPACE.f90:198-247) on and off;
+  - the temperature list (MAIN.f90:81-96, SETUP.f90:123-142)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ed_lattice_amd import lattice
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FDIR = os.path.join(ROOT, "ed_lattice_amd", "fortran", "_build")
+EXE = os.path.join(FDIR, "edlat_gpu")
+
+
+def _input(**kw):
+    base = dict(NORB=1, NSITES=6, NSPIN=1, ULOC=3.0, XMU=0.0, HFMODE="T", ED_FILLING=0, ED_SPARSE_H="F",
+                ED_METHOD="lanczos", LANC_METHOD="lanczos", LANC_NSTATES_SECTOR=1, LANC_NITER=400,
+                LANC_NGFITER=300, GF_FLAG="F", OFFDIAG_GF_FLAG="F", CHISPIN_FLAG="F", LMATS=200, LREAL=200,
+                HIJ_FILE="hij.in", ED_PRECISION="fp64")
+    base.update(kw)
+    return "\n".join(f"{k}={v}" for k, v in base.items()) + "\n"
+
+
+def _run(tmp_path, text, H):
+    (tmp_path / "inputED.conf").write_text(text)
+    lattice.write_hij(str(tmp_path / "hij.in"), H)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    return r.stdout
+
+
+class FortranRecords:
+    """List-directed reads as Fortran does them: every read starts a new
+    record and takes records until it has its values (blank records are
+    skipped; a read of zero values consumes one record); r*c repeats."""
+
+    def __init__(self, path):
+        with open(path) as f:
+            self.lines = f.read().split("\n")
+        self.i = 0
+
+    def read(self, n):
+        vals, first = [], True
+        while first or len(vals) < n:
+            first = False
+            for tok in self.lines[self.i].replace(",", " ").split():
+                if "*" in tok:
+                    r, c = tok.split("*")
+                    vals += [c] * int(r)
+                else:
+                    vals.append(tok)
+            self.i += 1
+        return vals[:n]
+
+    def logical(self):
+        return self.read(1)[0].upper().lstrip(".").startswith("T")
+
+    def int(self):
+        return int(self.read(1)[0])
+
+    def reals(self, n):
+        return np.array([float(t.replace("D", "E").replace("d", "e")) for t in self.read(n)])
+
+
+def read_gfmatrix(rd):
+    """read_formatted_gfmatrix (VARS_GLOBAL.f90:611-632): None when the
+    status record is F, else [state][channel] -> (poles, weights)."""
+    if not rd.logical():
+        return None
+    out = []
+    for _ in range(rd.int()):
+        chans = []
+        for _ in range(rd.int()):
+            n = rd.int()
+            chans.append((rd.reals(n), rd.reals(n)))
+        out.append(chans)
+    return out
+
+
+def _sector_eigh(m, orc, nup, ndw):
+    return np.linalg.eigh(orc.hamiltonian_csr(m, nup, ndw).toarray().real)
+
+
+def test_module_exports_reference_api():
+    obj = os.path.join(FDIR, "edlat.o")
+    assert os.path.exists(obj), "build() compiles the EDLAT module"
+    syms = subprocess.run(["nm", obj], capture_output=True, text=True).stdout.lower()
+    for name in ("ed_read_input", "ed_hij_init", "ed_hij_add_link", "ed_hij_read", "ed_hij_info", "ed_hij_write",
+                 "ed_hij_get_g0inv", "ed_hij_get_g0func", "ed_init_solver", "ed_solve", "ed_get_gimp_matsubara",
+                 "ed_get_gimp_realaxis", "ed_get_dens", "ed_get_mag", "ed_get_docc", "ed_get_energy",
+                 "ed_get_doubles"):
+        # flang's module-procedure names (_QM<module>P<procedure>)
+        assert f"_qmedlatp{name}" in syms, name
+    assert os.path.exists(os.path.join(FDIR, "edlat.mod"))
+
+
+@pytest.mark.gpu
+def test_offdiag_gf_chi_and_restart_files_ring6(tmp_path, orc):
+    H, U = lattice.ring(6), 3.0
+    _run(tmp_path, _input(GF_FLAG="T", OFFDIAG_GF_FLAG="T", CHISPIN_FLAG="T", SECTOR_NUP=3, SECTOR_NDW=3), H)
+    m = orc.Model(H, [U])
+    w0, V0 = _sector_eigh(m, orc, 3, 3)
+    e0, gs = w0[0], V0[:, 0]
+    assert w0[1] - w0[0] > 1e-6   # non-degenerate ground state
+    # particle and hole amplitudes <n|c+_i|gs>, <m|c_i|gs> in the N+-1 sectors
+    wp, Vp = _sector_eigh(m, orc, 4, 3)
+    wh, Vh = _sector_eigh(m, orc, 2, 3)
+    A = np.array([Vp.T @ orc.seed_c_cdg(6, 2, i, 1, (3, 3), (4, 3), gs).real for i in range(1, 7)])
+    B = np.array([Vh.T @ orc.seed_c_cdg(6, 1, i, 1, (3, 3), (2, 3), gs).real for i in range(1, 7)])
+    eps, temp = 0.01, 0.001
+    for axis, z in (("iw", None), ("realw", None)):
+        for i in range(1, 7):
+            for j in range(1, 7):
+                g = np.loadtxt(tmp_path / f"G_i{i:04d}j{j:04d}_l1m1_s1_{axis}.ed")
+                x = g[:, 0]
+                zz = 1j * x if axis == "iw" else x + 1j * eps
+                ref = (A[i - 1] * A[j - 1] / (zz[:, None] - (wp - e0))).sum(1) + \
+                      (B[i - 1] * B[j - 1] / (zz[:, None] + (wh - e0))).sum(1)
+                got = g[:, 2] + 1j * g[:, 1]
+                tol = 1e-9 if axis == "iw" else 1e-7
+                assert np.max(np.abs(got - ref)) < tol * max(1.0, np.max(np.abs(ref))), (axis, i, j)
+    # chi_ii(tau), Ltau/2 symmetrised, beta = 1/temp
+    ws = w0 - e0
+    beta = 1.0 / temp
+    chiT = np.loadtxt(tmp_path / "chiT.ed", ndmin=2)[-1]
+    assert abs(chiT[0] - temp) < 1e-12
+    for i in range(1, 7):
+        d = np.loadtxt(tmp_path / f"spinChi_i{i:04d}_l1_tau.ed")
+        tau, chi = d[:, 0], d[:, 1]
+        ltau = tau.size - 1
+        amp = (V0.T @ orc.seed_sz(6, i, (3, 3), gs).real) ** 2
+        ref = np.empty_like(tau)
+        for k, t in enumerate(tau):
+            if k < ltau // 2:
+                ref[k] = (amp * np.exp(-t * ws)).sum()
+            elif k == ltau // 2:
+                ref[k] = (amp * 0.5 * (np.exp(-t * ws) + np.exp(-(beta - t) * ws))).sum()
+            else:
+                ref[k] = (amp * np.exp(-(beta - t) * ws)).sum()
+        assert np.max(np.abs(chi - ref)) < 1e-9, i
+        h = beta / ltau
+        assert abs(chiT[i] - h * (chi.sum() - 0.5 * (chi[0] + chi[-1]))) < 1e-9 * abs(chiT[i])
+    # gfmatrix.restart: Nspin x Ns x Ns records, two channels per state;
+    # channel 1 particle, 2 hole; weights sum to norm2 of the seed
+    rd = FortranRecords(tmp_path / "gfmatrix.restart")
+    gfm = [[read_gfmatrix(rd) for j in range(6)] for i in range(6)]
+    for i in range(6):
+        for j in range(6):
+            rec = gfm[i][j]
+            assert rec is not None and len(rec) == 1 and len(rec[0]) == 2
+            (pp, wpp), (ph, wph) = rec[0]
+            assert pp.size == 300 and ph.size == 300
+            if i == j:
+                assert abs(wpp.sum() - (A[i] ** 2).sum()) < 1e-10
+                assert abs(wph.sum() - (B[i] ** 2).sum()) < 1e-10
+                assert np.all(ph <= 1e-9) and np.all(pp >= -1e-9)
+            else:
+                assert abs(wpp.sum() - ((A[i] + A[j]) ** 2).sum()) < 1e-10
+            np.testing.assert_array_equal(rec[0][0][0], gfm[j][i][0][0][0])
+    # ChiSpinMatrix.restart: write_gfmatrix_all2 over (Ns, Ns), F off the diagonal
+    rd = FortranRecords(tmp_path / "ChiSpinMatrix.restart")
+    for i in range(6):
+        for j in range(6):
+            rec = read_gfmatrix(rd)
+            if i != j:
+                assert rec is None
+                continue
+            assert len(rec) == 1 and len(rec[0]) == 1
+            p, w = rec[0][0]
+            amp = (V0.T @ orc.seed_sz(6, i + 1, (3, 3), gs).real) ** 2
+            assert abs(w.sum() - amp.sum()) < 1e-10
+    assert rd.i >= len([ln for ln in rd.lines if ln.strip()])
+
+
+def _degenerate_case(twin):
+    return _input(ED_FILLING=5, LANC_METHOD="arpack", LANC_NSTATES_SECTOR=2, ED_TWIN=twin, GF_FLAG="T",
+                  LANC_NGFITER=200, GS_THRESHOLD=1e-9)
+
+
+@pytest.mark.gpu
+def test_zero_temperature_degenerate_sectors_and_twin(tmp_path, orc):
+    """6-site ring, U = 3, five electrons: the ground level is a spin doublet
+    (Nup, Ndw) = (3,2) / (2,3), each twofold degenerate in momentum, so the
+    T = 0 list keeps four states (Z = 4) from two sectors; with ED_TWIN the
+    (2,3) sector is not solved and its states are the spin flips of the
+    (3,2) ones.  Both runs give the same energies, densities and G, equal
+    to the dense ground manifold."""
+    H, U = lattice.ring(6), 3.0
+    m = orc.Model(H, [U])
+    levels = {}
+    for nup in range(6):
+        ndw = 5 - nup
+        if 0 <= ndw <= 6:
+            levels[(nup, ndw)] = _sector_eigh(m, orc, nup, ndw)
+    emin = min(w[0] for w, _ in levels.values())
+    manifold = [(s, k) for s, (w, _) in levels.items() for k in range(2) if abs(w[k] - emin) <= 1e-9]
+    assert len(manifold) == 4 and {s for s, _ in manifold} == {(3, 2), (2, 3)}
+    dens = np.zeros(6)
+    for (nup, ndw), k in manifold:
+        o = orc.observables(6, nup, ndw, levels[(nup, ndw)][1][:, k])
+        dens += (o["dens_up"] + o["dens_dw"]) / 4
+    outs = {}
+    for twin in ("F", "T"):
+        d = tmp_path / twin
+        d.mkdir()
+        out = _run(d, _degenerate_case(twin), H)
+        rows = [ln.split() for ln in out.splitlines() if ln.strip().startswith("Egs =")]
+        assert len(rows) == 4
+        np.testing.assert_allclose([float(r[2]) for r in rows], emin, atol=1e-9)
+        lst = np.loadtxt(d / "state_list.ed", ndmin=2)
+        assert lst.shape[0] == 4
+        got = np.array((d / "dens.ed").read_text().split(), float)
+        np.testing.assert_allclose(got, dens, atol=1e-8)
+        outs[twin] = d
+    solved = {twin: open(outs[twin] / "eigenvalues_list.ed").read().count("Sector") for twin in outs}
+    assert solved["F"] == 6 and solved["T"] == 3
+    for f in ("G_i0001_l1_s1_iw.ed", "G_i0003_l1_s1_realw.ed"):
+        a, b = np.loadtxt(outs["F"] / f), np.loadtxt(outs["T"] / f)
+        np.testing.assert_allclose(a, b, atol=1e-8)
+
+
+@pytest.mark.gpu
+def test_temperature_list_ring6(tmp_path, orc):
+    """ED_FINITE_TEMP with temperature.restart: the states are found once at
+    the largest T, then every temperature (largest first) trims the list and
+    appends its observables; each line equals the dense thermal average."""
+    H, U = lattice.ring(6), 3.0
+    (tmp_path / "temperature.restart").write_text("0.3\n0.6\n")
+    out = _run(tmp_path, _input(ED_FINITE_TEMP="T", LANC_METHOD="arpack", LANC_NSTATES_SECTOR=2, CUTOFF=1e-9,
+                                LANC_NITER=200, TEMP=0.1), H)
+    assert "Solving for temperature" in out
+    used = np.loadtxt(tmp_path / "temperature.used")
+    np.testing.assert_allclose(used, [0.6, 0.3])
+    m = orc.Model(H, [U])
+    kept = []
+    for nup in range(7):
+        for ndw in range(7):
+            w, V = _sector_eigh(m, orc, nup, ndw)
+            for j in range(min(2, len(w))):
+                kept.append((w[j], nup, ndw, V[:, j]))
+    kept.sort(key=lambda t: t[0])
+    kept = kept[:2 * 49 + 10]
+    dens_lines = np.loadtxt(tmp_path / "dens.ed", ndmin=2)
+    docc_lines = np.loadtxt(tmp_path / "docc.ed", ndmin=2)
+    assert dens_lines.shape[0] == 2
+    e_first = kept[0][0]
+    kept = [k for k in kept if np.exp(-(k[0] - e_first) / 0.6) > 1e-9]
+    for row, T in enumerate((0.6, 0.3)):
+        sub = [k for k in kept if np.exp(-(k[0] - e_first) / T) > 1e-9]
+        bw = np.array([np.exp(-(k[0] - e_first) / T) for k in sub])
+        Z = bw.sum()
+        ntot = dc = 0.0
+        for (e, nup, ndw, v), b in zip(sub, bw):
+            o = orc.observables(6, nup, ndw, v)
+            ntot += b / Z * (o["dens_up"] + o["dens_dw"]).sum()
+            dc += b / Z * o["docc"].sum()
+        assert abs(dens_lines[row].sum() - ntot) < 1e-8, T
+        assert abs(docc_lines[row].sum() - dc) < 1e-8, T

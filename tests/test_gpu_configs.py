@@ -62,11 +62,14 @@ def test_sq4x4_fp64_and_mixed_ground_state(orc):
         np.testing.assert_allclose(o64["dens"], 1.0, atol=1e-10)
         np.testing.assert_allclose(o64["dens_up"], o64["dens_dw"], atol=1e-10)
         # both eigenpairs are eigenpairs of the oracle's operator (sampled rows)
+        sampled = {}
         for name, x, e0 in (("fp64", x64, r64["e0"]), ("mixed", xmx, rmx["e0"])):
             assert abs(np.linalg.norm(x) - 1.0) < 1e-12
-            rmax, scale, rows, hx = _residual_rows(orc, m, 8, 8, x, e0)
+            rmax, scale, rows_x, hx_x = _residual_rows(orc, m, 8, 8, x, e0)
+            sampled[name] = (rows_x, hx_x)
             print(f"sq4x4 {name}: residual rows max {rmax:.3e} (|E0| rms(x) {scale:.3e})", flush=True)
             assert rmax < 1e-5 * scale
+        rows, hx = sampled["fp64"]
         # the GPU operator on the eigenvector: same rows as the oracle, and a
         # small global residual; Rayleigh quotient = E0
         hx_gpu = ed.direct_matvec(isec, x64, "fp64")

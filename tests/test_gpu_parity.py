@@ -180,18 +180,20 @@ def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, monkeypatch):
         assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
 
 
-@pytest.mark.parametrize("rpt,kch,R,dn,dk,vec", [
-    (1, 8, 64, 2, 8, 0), (2, 4, 32, 1, 16, 0), (2, 4, 64, 2, 4, 1), (1, 8, 64, 1, 2, 1), (1, 16, 32, 2, 1, 1),
-    (1, 8, 16, 2, 2, 1), (2, 8, 128, 1, 4, 1), (2, 4, 256, 2, 2, 1)])
-def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, monkeypatch):
+@pytest.mark.parametrize("rpt,kch,R,dn,dk,vec,col", [
+    (1, 8, 64, 2, 8, 0, 0), (2, 4, 32, 1, 16, 0, 0), (2, 4, 64, 2, 4, 1, 0), (1, 8, 64, 1, 2, 1, 0),
+    (1, 16, 32, 2, 1, 1, 0), (1, 8, 16, 2, 2, 1, 0), (2, 8, 128, 1, 4, 1, 0), (2, 4, 256, 2, 2, 1, 0),
+    (1, 8, 128, 2, 2, 1, 1), (2, 4, 64, 1, 4, 1, 1)])
+def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, monkeypatch):
     """Schedules pinned through the override, among them the vector-gather
     down pass (16-byte lanes, panels of 16 to 256 rows; the 8-site ring's
-    DimUp 70 leaves a quad that straddles DimUp): H x v on three sectors and a
-    mixed-precision ground state (the chain-mode passes) against the
-    oracle."""
+    DimUp 70 leaves a quad that straddles DimUp) and the bank-scheduled up
+    pass (col = 1, fp32; its last 64-row block is partial on every sector
+    here): H x v on three sectors and a mixed-precision ground state (the
+    chain-mode passes) against the oracle."""
     tail = f"{R},256,32,{dn},{dk},{rpt},{kch},{vec}"
     tail64 = f"{min(R, 128)},256,32,{dn},{dk},{rpt},{kch},{vec}"  # 256-row panels are fp32 only
-    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", "3,1024,256," + tail)
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", "3,1024,256," + tail + f",{col}")
     monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", "1,512,512," + tail64)
     for cfg, sec in [("ring8", (4, 4)), ("rect3x4", (7, 6)), ("rect3x4", (6, 5))]:
         c = lattice.CONFIGS[cfg]
@@ -203,6 +205,7 @@ def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, monke
             sched = ed.bench_passes(isec, prec, 1)[2]
             rows = 128 if (R == 256 and prec == "fp64") else R
             assert (sched["dw_rows"], sched["dw_vector_gather"], sched["up_rows_per_thread"]) == (rows, vec, rpt)
+            assert sched["up_bank_scheduled"] == (col if prec == "fp32" else 0)
         v = RNG.standard_normal(dim)
         ref = orc.matvec(m, *sec, v, nthreads=8).real
         assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
@@ -301,6 +304,20 @@ def test_lanczos_gs_ring8_vs_dense(ed, orc):
     ref = orc.observables(8, 4, 4, r["gs"])
     for k in ("dens_up", "dens_dw", "docc"):
         np.testing.assert_allclose(obs[k], ref[k], atol=1e-12)
+    # <Sz_i Sz_j>, <n_i n_j> (OBSERVABLES.f90:142-150) from the bits of the
+    # basis strings, element by element
+    mu, md = orc.build_map(8, 4), orc.build_map(8, 4)
+    w = (r["gs"] ** 2).reshape(md.size, mu.size)          # [idw, iup]
+    bu = (mu[:, None] >> np.arange(8)) & 1                 # [iup, site]
+    bd = (md[:, None] >> np.arange(8)) & 1
+    n = bu[None, :, :] + bd[:, None, :]                    # [idw, iup, site]
+    sz = 0.5 * (bu[None, :, :] - bd[:, None, :])
+    n2 = np.einsum("ab,abi,abj->ij", w, n, n)
+    sz2 = np.einsum("ab,abi,abj->ij", w, sz, sz)
+    cor = ed.correlations()
+    np.testing.assert_allclose(cor["n2"], n2, atol=1e-12)
+    np.testing.assert_allclose(cor["sz2"], sz2, atol=1e-12)
+    np.testing.assert_allclose(np.diag(cor["n2"]), obs["dens_up"] + obs["dens_dw"] + 2 * obs["docc"], atol=1e-12)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
