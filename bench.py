@@ -155,12 +155,25 @@ def seeds_benchmark(ed):
                 seeds += [("cdg", ispin, i, jo), ("c", ispin, i, jo)]
     out = {"workload": "3x4 PBC U=6, GS (6,6); 312 GF seeds (both spins, 78 i<=j pairs x particle/hole) in "
                        "(7,6)/(5,6)/(6,7)/(6,5), dim 731808, 150 Lanczos steps each, fp64"}
-    # first calls tune the tile/wave schedules (cached per chain-set shape)
+    # first calls tune the tile/wave schedules and pick the persistent or
+    # the tile form (cached per chain-set shape); both forms are timed
     ed.tridiag_batched(seeds, 150, "fp64")
     ed.tridiag_batched(seeds[:1], 150, "fp64", group=1)
     rb = ed.tridiag_batched(seeds, 150, "fp64")
+    forms = {}
+    prev = os.environ.get("EDL_BATCHED_KERNELS")
+    try:
+        for form in ("tiles", "persist"):
+            os.environ["EDL_BATCHED_KERNELS"] = form
+            forms[form] = ed.tridiag_batched(seeds, 150, "fp64")["ms"]
+    finally:
+        if prev is None:
+            os.environ.pop("EDL_BATCHED_KERNELS", None)
+        else:
+            os.environ["EDL_BATCHED_KERNELS"] = prev
     rs = ed.tridiag_batched(seeds, 150, "fp64", group=1)
     out.update(nseeds=len(seeds), batched_ms=rb["ms"], sequential_ms=rs["ms"], speedup=rs["ms"] / rb["ms"],
+               batched_ms_tiles=forms["tiles"], batched_ms_persistent=forms["persist"],
                hxv_batched=rb["hxv"], hxv_per_s_batched=rb["hxv"] / (rb["ms"] * 1e-3))
     ed.tridiag_batched(seeds, 150, "fp32")
     rb32 = ed.tridiag_batched(seeds, 150, "fp32")
@@ -221,6 +234,8 @@ def small_sector_gpu(ed):
 
 def dw_kernel_name(t, sched):
     """Name of the down-pass kernel a schedule runs (as rocprofv3 prints it)."""
+    if sched.get("dw_range_sites"):
+        return f"k_pass_dwr<{t}, {sched['dw_rows']}, 0>"
     if sched.get("dw_vector_gather"):
         return f"k_pass_dwv<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>"
     return f"k_pass_dw<{t}, {sched['dw_rows']}, 0, {sched['dw_cols_per_lane']}, {sched['dw_hop_chunk']}>"

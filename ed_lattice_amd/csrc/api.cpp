@@ -37,6 +37,7 @@ Sector& get_sector(Ctx& c, int isector) {
   build_sector_tables(c, *s);
   try {
     build_up_color_tables(*s, c.stream);
+    build_dw_range_tables(*s, c.stream);
   } catch (...) {
     free_sector(*s);
     throw;
@@ -178,6 +179,7 @@ static void with_chain_shapes(TwoPassCfg& dst, const TwoPassCfg& src) {
   dst.ch_up_rpt = src.ch_up_rpt;
   dst.ch_up_kch = src.ch_up_kch;
   dst.ch_up_col = src.ch_up_col;
+  dst.ch_rng = src.ch_rng;
 }
 
 // Pick panel widths, block sizes and grids of the two passes by timing the
@@ -191,23 +193,25 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   TwoPassCfg best;
   bool pinned = false;
   // EDL_TWO_PASS_CFG_F32 / _F64 =
-  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col]]]]"
+  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]"
   // pins the operator schedule of that precision (reproducible profiles of
   // one configuration), ahead of any tuned schedule; the operator is not
   // tuned then, its chain shapes still are (once per sector shape).
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
                               &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec,
-                              &p.up_col);
+                              &p.up_col, &p.dw_rng);
+    EDL_CHECK(p.dw_rng == 0 || pass_dwr_ok(V, p.R, p.dw_rng, sizeof(T)),
+              "EDL_TWO_PASS_CFG_F32/_F64: dw_rng needs range tables for that many top sites and R 32|64 within LDS");
     EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T)),
               "EDL_TWO_PASS_CFG_F32/_F64: up_col needs fp32 panels of 1 or 3 columns on a sector with the "
               "bank-scheduled table");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
-                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col]]]]");
+                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
               "EDL_TWO_PASS_CFG_F32/_F64: (up_rpt, up_kch) must be (1, 8|16) or (2, 4|8)");
-    EDL_CHECK(pass_dw_lds_bytes(V, p) <= (size_t)(160 * 1024 - 4096),
+    EDL_CHECK(p.dw_rng > 0 || pass_dw_lds_bytes(V, p) <= (size_t)(160 * 1024 - 4096),
               "EDL_TWO_PASS_CFG_F32/_F64: down-pass column table exceeds LDS");
     EDL_CHECK(p.dw_vec == 0 || p.dw_vec == 1, "EDL_TWO_PASS_CFG_F32/_F64: dw_vec must be 0 or 1");
     if (p.dw_vec)
@@ -236,8 +240,21 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   // The chain shapes do not depend on the operator's: a pinned operator
   // schedule still runs its chains on their own tuned shapes.
   auto ct = tp_chain_cache().find(key);
+  // A pinned range-blocked down pass (dw_rng > 0) pins the chain's down
+  // pass to the same form, so that its Lanczos terms are tested pinned too.
+  auto pin_chain_range = [&](TwoPassCfg& b) {
+    if (!pinned || b.dw_rng == 0) return;
+    b.ch_R = b.R;
+    b.ch_threads = b.threads_dw;
+    b.ch_nchunk = b.nchunk;
+    b.ch_ncol = b.ncol;
+    b.ch_kch = b.kch;
+    b.ch_vec = 0;
+    b.ch_rng = b.dw_rng;
+  };
   if (pinned && ct != tp_chain_cache().end()) {
     with_chain_shapes(best, ct->second);
+    pin_chain_range(best);
     return best;
   }
   hipStream_t st = c.stream;
@@ -319,6 +336,31 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             best.threads_dw = th;
             best.nchunk = nch;
             best.dw_vec = vec;
+            best.dw_rng = 0;
+          }
+        }
+    // range-blocked form: R-row slabs of column ranges sharing b top sites
+    for (int b = 1; b <= kMaxRangeBits; ++b)
+      for (int R : {32, 64})
+        for (int th : {256, 512, 1024}) {
+          if (!pass_dwr_ok(V, R, b, sizeof(T))) continue;
+          TwoPassCfg w = best;
+          w.R = R;
+          w.threads_dw = th;
+          w.dw_rng = b;
+          launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+          EventTimer tm(st);
+          for (int r = 0; r < 3; ++r)
+            launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
+          const double ms = tm.stop();
+          if (verbose())
+            std::fprintf(stderr, "[edl] tune dw  fp%d ranges b %d (%d) R %3d threads %4d: %.4f ms\n",
+                         8 * (int)sizeof(T), b, V.rng_n[b], R, th, ms / 3);
+          if (ms < best_dw) {
+            best_dw = ms;
+            best.R = R;
+            best.threads_dw = th;
+            best.dw_rng = b;
           }
         }
   }
@@ -339,6 +381,15 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
         TwoPassCfg w = best;
         w.R = R;
         w.nchunk = std::min(nch, V.dimdw);
+        w.dw_rng = 0;
+        maxpart = std::max(maxpart, (size_t)pass_dw_grid(V, w));
+      }
+    for (int b = 1; b <= kMaxRangeBits; ++b)
+      for (int R : {32, 64}) {
+        if (!pass_dwr_ok(V, R, b, sizeof(T))) continue;
+        TwoPassCfg w = best;
+        w.R = R;
+        w.dw_rng = b;
         maxpart = std::max(maxpart, (size_t)pass_dw_grid(V, w));
       }
     DevBuf part(sizeof(double) * maxpart);
@@ -378,8 +429,34 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               best.ch_ncol = w.ncol;
               best.ch_kch = w.kch;
               best.ch_vec = 1;
+              best.ch_rng = 0;
             }
           }
+    for (int b = 1; b <= kMaxRangeBits; ++b)
+      for (int R : {32, 64})
+        for (int th : {256, 512, 1024}) {
+          if (!pass_dwr_ok(V, R, b, sizeof(T))) continue;
+          TwoPassCfg w = best;
+          w.R = R;
+          w.threads_dw = th;
+          w.dw_rng = b;
+          w.dw_vec = 0;
+          launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
+          EventTimer tm(st);
+          for (int r = 0; r < 3; ++r)
+            launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
+          const double ms = tm.stop();
+          if (verbose())
+            std::fprintf(stderr, "[edl] tune dw chain fp%d ranges b %d R %3d threads %4d: %.4f ms\n",
+                         8 * (int)sizeof(T), b, R, th, ms / 3);
+          if (ms < bc) {
+            bc = ms;
+            best.ch_R = R;
+            best.ch_threads = th;
+            best.ch_rng = b;
+            best.ch_vec = 0;
+          }
+        }
     // The chain step's up pass with phase B pending and alpha = 0 (r = t,
     // written back over t: repeated launches leave their input unchanged).
     LzScalars s1 = s0;
@@ -425,6 +502,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   }
   tp_chain_cache()[key] = best;
   if (!pinned) tp_cache()[key] = best;
+  pin_chain_range(best);
   return best;
 }
 
@@ -577,6 +655,7 @@ static void apply_h(Ctx& c, Sector& s, const T* v, T* out, int ld, hipStream_t s
     const TwoPassView V = two_pass_view(s, ld);
     T* vin = const_cast<T*>(v);  // operator mode only reads it
     launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, cfg.C, st);
+    launch_nonlocal<T>(V, c.model.inter, vin, vin, tmp, nullptr, st);
     launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, out, nullptr, cfg, st);
   } else if (two_pass_ok(s, sizeof(T))) {
     const TwoPassCfg cfg = two_pass_cfg<T>(c, s);
@@ -584,6 +663,7 @@ static void apply_h(Ctx& c, Sector& s, const T* v, T* out, int ld, hipStream_t s
     const TwoPassView V = two_pass_view(s, ld);
     T* vin = const_cast<T*>(v);  // operator mode only reads it
     launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, cfg, st);
+    launch_nonlocal<T>(V, c.model.inter, vin, vin, tmp, nullptr, st);
     launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), const_cast<T*>(v), const_cast<T*>(v), tmp, out, nullptr, cfg, st);
   } else {
     EDL_CHECK(ld == s.sp[0].dim, "apply_h: the single-pass operator takes the natural layout");
@@ -783,6 +863,7 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
       launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
     else
       launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
+    launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
     launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
     launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
   } else if (tp) {
@@ -791,6 +872,7 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
     T* q1 = static_cast<T*>(cs.h[0].q1);
     launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
     launch_lz_beta(tp->L, tp->V, tp->cfg, st);
+    launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
     launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
     launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
   } else {
@@ -1695,6 +1777,85 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
   return best;
 }
 
+// Persistent or tile launches for a batched group, and the persistent shape:
+// both forms run the group's first step (no phase B pending yet) on copies of
+// the chains with nmax = 1, which writes only q_1, T, alpha_0 and the
+// scalars; the scalars are put back after each timing, and the real first
+// step rewrites the rest before reading it.  EDL_BATCHED_KERNELS=persist or
+// =tiles forces the form.  Cached per group shape.
+static std::map<std::tuple<int, int, int, int>, std::pair<bool, BtShape>>& bt_persist_cache() {
+  static std::map<std::tuple<int, int, int, int>, std::pair<bool, BtShape>> m;
+  return m;
+}
+
+template <typename T>
+static bool bt_pick_persist(Ctx& c, const std::vector<BtChain>& chains, const BtShape& tiles, int max_up, int max_dw,
+                            hipStream_t st, BtShape* pshape) {
+  const char* e = std::getenv("EDL_BATCHED_KERNELS");
+  const bool force_p = e && std::strcmp(e, "persist") == 0, force_t = e && std::strcmp(e, "tiles") == 0;
+  const auto key = std::make_tuple((int)sizeof(T), max_up, max_dw, (int)chains.size());
+  auto it = bt_persist_cache().find(key);
+  if (it == bt_persist_cache().end()) {
+    std::vector<LzScalars> snap(chains.size());
+    for (size_t k = 0; k < chains.size(); ++k)
+      EDL_HIP(hipMemcpyAsync(&snap[k], chains[k].sc, sizeof(LzScalars), hipMemcpyDeviceToHost, st));
+    EDL_HIP(hipStreamSynchronize(st));
+    auto restore = [&]() {
+      for (size_t k = 0; k < chains.size(); ++k)
+        EDL_HIP(hipMemcpyAsync(chains[k].sc, &snap[k], sizeof(LzScalars), hipMemcpyHostToDevice, st));
+      EDL_HIP(hipStreamSynchronize(st));  // snap must outlive the copies
+    };
+    auto one_step = [&](const BtShape& shp, bool persist) {
+      BtSet bs;
+      bs.shape = shp;
+      bs.h = bt_resized(chains, shp);
+      for (BtChain& b : bs.h) b.nmax = 1;
+      bt_finalize<T>(bs);
+      double best = 1e300;
+      for (int r = 0; r < 2; ++r) {
+        restore();
+        EventTimer tm(st);
+        if (persist) {
+          bt_launch_persist<T>(bs, c.model.inter, st);
+        } else {
+          bt_launch_up<T>(bs, c.model.inter, st);
+          bt_launch_dw<T>(bs, st);
+        }
+        best = std::min(best, tm.stop());
+      }
+      restore();
+      return best;
+    };
+    const double tt = one_step(tiles, false);
+    double tp = 1e300;
+    BtShape bp = tiles;
+    for (int C : {2, 4})
+      for (int RS : {8, 16, 32}) {
+        if (!bt_persist_shape_ok(C, RS, sizeof(T)) ||
+            bt_persist_lds_bytes(max_up, max_dw, C, RS, sizeof(T)) > (size_t)150 * 1024)
+          continue;
+        BtShape w = tiles;
+        w.C = C;
+        w.RS = RS;
+        w.up_rpt = 2;
+        w.up_kch = 8;
+        w.dw_nc = 2;
+        const double ms = one_step(w, true);
+        if (verbose())
+          std::fprintf(stderr, "[edl] batched persistent C %d RS %d: %.3f ms per step (tiles %.3f ms)\n", C, RS, ms, tt);
+        if (ms < tp) {
+          tp = ms;
+          bp = w;
+        }
+      }
+    it = bt_persist_cache().emplace(key, std::make_pair(tp < tt, bp)).first;
+  }
+  *pshape = it->second.second;
+  if (force_t) return false;
+  if (force_p) return bt_persist_shape_ok(pshape->C, pshape->RS, sizeof(T));
+  return it->second.first;
+}
+
 template <typename T>
 static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const std::vector<int>& idx,
                              const std::vector<int>& targets, int nlanc, double* alpha, double* beta, double* norm2,
@@ -1713,7 +1874,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
     maxdim = std::max<int64_t>(maxdim, J.dim);
     max_up = std::max(max_up, J.sp[0].dim);
     max_dw = std::max(max_dw, J.sp[1].dim);
-    packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok && !J.nonlocal;
+    packed = packed && J.sp[0].packed_ok && J.sp[1].packed_ok;
   }
   // batched two-pass kernels when every target sector fits them; chains on
   // sectors large enough to fill the GPU alone: one at a time through the
@@ -1818,14 +1979,21 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   cs.h = chains;
   BtShape shp{};
   BtSet bs;
+  bool persist = false;
   if (bt) {
     shp = bt_tune<T>(c, btchains, max_up, max_dw, st);
+    BtShape pshp{};
+    persist = bt_pick_persist<T>(c, btchains, shp, max_up, max_dw, st, &pshp);
+    if (persist) shp = pshp;
     bs.shape = shp;
     bs.h = bt_resized(btchains, shp);
     bt_finalize<T>(bs);
   }
   EventTimer run_tm(st);
-  if (bt) {
+  if (bt && persist) {
+    bt_launch_persist<T>(bs, c.model.inter, st);
+    EDL_HIP(hipStreamSynchronize(st));
+  } else if (bt) {
     run_steps_bt<T>(c, bs, steps, st);
   } else if (alone) {
     for (size_t k = 0; k < chains.size(); ++k) {
@@ -1866,7 +2034,7 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   if (verbose()) {
     const double ms = run_tm.stop();
     std::fprintf(stderr, "[edl] batched group: %zu chains, %s kernels (C %d, RS %d), %d steps in %.3f ms (%.1f us/step)\n",
-                 chains.size(), bt ? "two-pass" : (alone ? "one-chain two-pass" : "single-pass"), bt ? shp.C : 0,
+                 chains.size(), bt ? (persist ? "persistent two-pass" : "two-pass tile") : (alone ? "one-chain two-pass" : "single-pass"), bt ? shp.C : 0,
                  bt ? shp.RS : 0, steps, ms, 1e3 * ms / steps);
   }
   int64_t hxv = 0;
@@ -2075,6 +2243,7 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
     for (int r = 0; r < reps; ++r) {
       EDL_HIP(hipEventRecord(ev[3 * r], st));
       launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, cf.C, st);
+      launch_nonlocal<T>(V, c.model.inter, v.as<T>(), v.as<T>(), tmp, nullptr, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
       launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
@@ -2086,6 +2255,7 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
     for (int r = 0; r < reps; ++r) {
       EDL_HIP(hipEventRecord(ev[3 * r], st));
       launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, cf, st);
+      launch_nonlocal<T>(V, c.model.inter, v.as<T>(), v.as<T>(), tmp, nullptr, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
       launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 2], st));
@@ -2114,10 +2284,10 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   ms_pass[1] = (two || seg) ? dw / reps : 0.0;
   // the up pass takes its coefficients from registers in fp32 on one-value lattices
   const int32_t lreg = (two && sizeof(T) == 4 && s.sp[0].nvals == 1) ? 1 : 0;
-  const int32_t out[15] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
+  const int32_t out[16] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
-                           cf.dw_vec,                lreg,      two ? cf.up_col : 0};
-  for (int k = 0; k < 15; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
+                           cf.dw_vec,                lreg,      two ? cf.up_col : 0,          cf.dw_rng};
+  for (int k = 0; k < 16; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 
 extern "C" int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg) {

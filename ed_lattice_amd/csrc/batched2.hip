@@ -96,6 +96,24 @@ __global__ __launch_bounds__(256) void k_bt_beta(const BtChain* __restrict__ cha
   }
 }
 
+// After the beta launch, for models with the non-local Kanamori block: T +=
+// N q_k of every chain (blockIdx.y = chain).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bt_nonlocal(const BtChain* __restrict__ chains, InterParams I) {
+  const BtChain& B = chains[blockIdx.y];
+  const LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const TwoPassView V = B.V;
+  const T* q = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+  T* tb = reinterpret_cast<T*>(B.tb);
+  const int64_t n = (int64_t)V.dimup * V.dimdw;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int idw = (int)(e / V.dimup), iup = (int)(e % V.dimup);
+    const size_t i = (size_t)idw * V.ld + iup;
+    tb[i] += (T)nonlocal_gather<T>(I, (uint32_t)V.map_up[iup], (uint32_t)V.map_dw[idw], q, V.ld, I.Jx, I.Jp);
+  }
+}
+
 // One block per chain, after the down launch: alpha_k from the slab
 // partials; phase B of step k becomes pending.
 __global__ __launch_bounds__(256) void k_bt_alpha(const BtChain* __restrict__ chains) {
@@ -115,39 +133,23 @@ __global__ __launch_bounds__(256) void k_bt_alpha(const BtChain* __restrict__ ch
   }
 }
 
+// Down tile of one slab (rows r0 .. r0+RS-1 of every column): the slab of q
+// is staged in S (the hop counts are already in cntS), then t = (T + H_dw q)
+// / s_k - beta_k q_{k-1} / s_{k-1} over q_{k-1}; returns this thread's share
+// of <t, q_k> (times s_k).  Both the tile kernel and the persistent kernel
+// run it.
 template <typename T, int RS, int NC>
-__global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
-                                                const int32_t* __restrict__ tchain, int64_t Tall, int64_t n8) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* lut = reinterpret_cast<T*>(smem);
-  T* S = reinterpret_cast<T*>(smem + lds_head_bytes(sizeof(T)));  // [dimdw][RS]
-  __shared__ double sh[16];
-  const int64_t t = xcd_tile(n8);
-  if (t >= Tall) return;
-  const int c = tchain[t];
-  const BtChain& B = chains[c];
-  const LzScalars s = *B.sc;
-  if (s.stopped) return;
-  const TwoPassView V = B.V;
-  const T* __restrict__ q = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
-  T* __restrict__ qo = reinterpret_cast<T*>((s.k & 1) ? B.q0 : B.q1);
-  const T* __restrict__ tb = reinterpret_cast<const T*>(B.tb);
-  const T* lsrc = reinterpret_cast<const T*>(B.lut_dw);
-  const int sl = (int)(t - tile0[c]);
-  const int r0 = sl * RS;
+static __device__ __forceinline__ double bt_dw_slab(const TwoPassView& V, const T* lut, T* S, const int* cntS,
+                                                    const T* __restrict__ q, T* __restrict__ qo,
+                                                    const T* __restrict__ tb, int r0, T ts, T bo) {
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
-  int* cntS = reinterpret_cast<int*>(S + (size_t)dimdw * RS);  // [dimdw] hop counts
 #pragma unroll 4
   for (int i = threadIdx.x; i < dimdw * RS; i += blockDim.x) {
     const int j = i / RS, r = i % RS;
     S[i] = (r0 + r < dimup) ? q[(size_t)j * ld + r0 + r] : (T)0;
   }
-  for (int j = threadIdx.x; j < dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
   __syncthreads();
-  const double inv_s = 1.0 / s.s_cur;
-  const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
   // lane = row of the slab (fastest) x column: the global accesses of a wave
   // are 64 / RS whole slab segments
   const int r = threadIdx.x % RS, jl = threadIdx.x / RS, jstep = blockDim.x / RS;
@@ -197,8 +199,140 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
       }
     }
   }
+  return part;
+}
+
+template <typename T, int RS, int NC>
+__global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
+                                                const int32_t* __restrict__ tchain, int64_t Tall, int64_t n8) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut = reinterpret_cast<T*>(smem);
+  T* S = reinterpret_cast<T*>(smem + lds_head_bytes(sizeof(T)));  // [dimdw][RS]
+  __shared__ double sh[16];
+  const int64_t t = xcd_tile(n8);
+  if (t >= Tall) return;
+  const int c = tchain[t];
+  const BtChain& B = chains[c];
+  const LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const TwoPassView V = B.V;
+  const T* __restrict__ q = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+  T* __restrict__ qo = reinterpret_cast<T*>((s.k & 1) ? B.q0 : B.q1);
+  const T* __restrict__ tb = reinterpret_cast<const T*>(B.tb);
+  const T* lsrc = reinterpret_cast<const T*>(B.lut_dw);
+  const int sl = (int)(t - tile0[c]);
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  int* cntS = reinterpret_cast<int*>(S + (size_t)V.dimdw * RS);  // [dimdw] hop counts
+  for (int j = threadIdx.x; j < V.dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
+  const double inv_s = 1.0 / s.s_cur;
+  const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
+  const double part = bt_dw_slab<T, RS, NC>(V, lut, S, cntS, q, qo, tb, sl * RS, ts, bo);
   const double v = block_sum(part * inv_s, sh);
   if (threadIdx.x == 0) B.partD[sl] = v;
+}
+
+// Every step of one chain in one block (a persistent launch: one block per
+// chain, all Lanczos steps before it exits).  The step is the four launches
+// of the tile path done by one block in turn -- the up pass over every
+// panel (phase B fused), beta, the down pass over every slab (phase A
+// fused), alpha -- with the reductions inside the block, so no launch or
+// grid-wide synchronisation separates the phases: __syncthreads() orders the
+// block's own global writes (r, T, t) before the phase that reads them.
+template <typename T, int C, int RPT, int UKCH, int RS, int NC>
+__global__ __launch_bounds__(1024) void k_bt_persist(const BtChain* __restrict__ chains, InterParams I) {
+  typedef typename Vec<T, C>::type VT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lut_u = reinterpret_cast<T*>(smem);
+  T* lut_d = lut_u + 256;
+  unsigned char* body = smem + 2 * lds_head_bytes(sizeof(T));
+  VT* P = reinterpret_cast<VT*>(body);  // up phase: [dimup] panel rows
+  T* S = reinterpret_cast<T*>(body);    // down phase: [dimdw][RS] slab
+  __shared__ double sh[16];
+  __shared__ double bc;
+  const BtChain& B = chains[blockIdx.x];
+  LzScalars s = *B.sc;
+  if (s.stopped) return;
+  const TwoPassView V = B.V;
+  const int dimdw = V.dimdw;
+  int* cntS = reinterpret_cast<int*>(body + sizeof(T) * (size_t)dimdw * RS);  // [dimdw], after the slab
+  const T* lu = reinterpret_cast<const T*>(B.lut_up);
+  const T* ldw = reinterpret_cast<const T*>(B.lut_dw);
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    lut_u[k] = lu[k];
+    lut_d[k] = ldw[k];
+  }
+  // (no __restrict__ at this scope: the buffers change roles between phases)
+  T* tb = reinterpret_cast<T*>(B.tb);
+  const int npan = (dimdw + C - 1) / C;
+  for (;;) {
+    // ---- up: phase B of the previous step where pending, T = [D + H_up] q
+    const bool pend = s.pad != 0;
+    T* qn = reinterpret_cast<T*>(((pend ? s.k + 1 : s.k) & 1) ? B.q1 : B.q0);
+    const T* qk = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+    const T am = (T)(s.alpha_k / s.s_cur);
+    double nrm = 0.0;
+    for (int pan = 0; pan < npan; ++pan) {
+      __syncthreads();
+      nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, false, am, nullptr, 0.0);
+      __syncthreads();
+      up_rows<T, C, RPT, UKCH>(V, I, lut_u, P, tb, pan * C, (T)1);
+    }
+    if (pend) {
+      // ---- beta_{k+1} = |r|, the scalars advance (k_bt_beta)
+      const double s2 = block_sum(nrm, sh);
+      if (threadIdx.x == 0) bc = s2;
+      __syncthreads();
+      const double b = sqrt(bc);
+      const bool more = (s.k + 1 < B.nmax);
+      if (more && threadIdx.x == 0) B.beta[s.k + 1] = b;
+      s.pad = 0;
+      const bool brk = !(b > B.brk_tol * (fabs(s.alpha_k) + s.beta_k));
+      if (!more || brk) {
+        s.stopped = 1;
+        break;
+      }
+      s.s_old = s.s_cur;
+      s.s_cur = b;
+      s.beta_k = b;
+      s.k = s.k + 1;
+    }
+    // ---- down: t = (T + H_dw q)/s_k - beta_k q_{k-1}/s_{k-1}, alpha (k_bt_dw, k_bt_alpha)
+    __syncthreads();  // r and T of every panel written before any slab reads them
+    if (I.Jx != 0.0 || I.Jp != 0.0) {
+      // T += N q_k, the non-local Kanamori block (k_bt_nonlocal)
+      const T* qc = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+      const int64_t n = (int64_t)V.dimup * dimdw;
+      for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const int idw = (int)(e / V.dimup), iup = (int)(e % V.dimup);
+        const size_t i = (size_t)idw * V.ld + iup;
+        tb[i] += (T)nonlocal_gather<T>(I, (uint32_t)V.map_up[iup], (uint32_t)V.map_dw[idw], qc, V.ld, I.Jx, I.Jp);
+      }
+      __syncthreads();
+    }
+    for (int j = threadIdx.x; j < dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
+    const T* q = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
+    T* qo = reinterpret_cast<T*>((s.k & 1) ? B.q0 : B.q1);
+    const double inv_s = 1.0 / s.s_cur;
+    const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
+    double part = 0.0;
+    for (int sl = 0; sl < B.nslab; ++sl) {
+      __syncthreads();  // the previous slab's gathers are done with S
+      part += bt_dw_slab<T, RS, NC>(V, lut_d, S, cntS, q, qo, tb, sl * RS, ts, bo);
+    }
+    const double a0 = block_sum(part * inv_s, sh);
+    if (threadIdx.x == 0) bc = a0;
+    __syncthreads();
+    const double a = bc;
+    if (threadIdx.x == 0) B.alpha[s.k] = a;
+    s.alpha_k = a;
+    s.nsteps = s.k + 1;
+    s.pad = 1;
+    if (s.k + 1 >= B.nmax) {
+      s.stopped = 1;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) *B.sc = s;
 }
 
 // ---------------------------------------------------------------- host side
@@ -307,6 +441,13 @@ static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st, bool 
   if (!scalars) return;
   hipLaunchKernelGGL(k_bt_beta, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
   EDL_HIP(hipGetLastError());
+  if (I.Jx != 0.0 || I.Jp != 0.0) {
+    int64_t n = 1;
+    for (const BtChain& b : bs.h) n = std::max<int64_t>(n, (int64_t)b.V.dimup * b.V.dimdw);
+    const unsigned gx = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_bt_nonlocal<T>, dim3(gx, (unsigned)bs.h.size()), dim3(256), 0, st, bs.d, I);
+    EDL_HIP(hipGetLastError());
+  }
 }
 
 template <typename T, int RS, int NC>
@@ -367,6 +508,51 @@ void bt_launch_dw(const BtSet& bs, hipStream_t st, bool scalars) {
     throw Error("bt_launch_dw: unsupported columns per thread");
 }
 
+// ---- persistent launches
+size_t bt_persist_lds_bytes(int max_dimup, int max_dimdw, int C, int RS, size_t elem) {
+  const size_t up = (size_t)C * max_dimup * elem;
+  const size_t dw = (size_t)RS * max_dimdw * elem + sizeof(int) * (size_t)max_dimdw;
+  return 2 * lds_head_bytes(elem) + std::max(up, dw);
+}
+
+bool bt_persist_shape_ok(int C, int RS, size_t elem) {
+  return (C == 2 || C == 4) && (RS == 8 || RS == 16 || (RS == 32 && elem == 4));
+}
+
+template <typename T, int C, int RS>
+static void bt_persist_k(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  const size_t lds = bt_persist_lds_bytes(max_dim(bs, 0), max_dim(bs, 1), C, RS, sizeof(T));
+  EDL_CHECK(lds <= kBtDwBudget, "bt_launch_persist: panel or slab exceeds LDS");
+  set_lds_limit((const void*)k_bt_persist<T, C, 2, 8, RS, 2>, (int)kBtDwBudget);
+  hipLaunchKernelGGL((k_bt_persist<T, C, 2, 8, RS, 2>), dim3((unsigned)bs.h.size()), dim3(1024), lds, st, bs.d, I);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int C>
+static void bt_persist_c(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  switch (bs.shape.RS) {
+    case 8: return bt_persist_k<T, C, 8>(bs, I, st);
+    case 16: return bt_persist_k<T, C, 16>(bs, I, st);
+    case 32:
+      if constexpr (sizeof(T) == 4) return bt_persist_k<T, C, 32>(bs, I, st);
+      [[fallthrough]];
+    default: throw Error("bt_launch_persist: unsupported slab height");
+  }
+}
+
+template <typename T>
+void bt_launch_persist(const BtSet& bs, const InterParams& I, hipStream_t st) {
+  EDL_CHECK(bt_persist_shape_ok(bs.shape.C, bs.shape.RS, sizeof(T)), "bt_launch_persist: unsupported shape");
+  for (const BtChain& b : bs.h)
+    EDL_CHECK(b.nslab == (b.V.dimup + bs.shape.RS - 1) / bs.shape.RS, "bt_launch_persist: slab count mismatch");
+  if (bs.shape.C == 2)
+    bt_persist_c<T, 2>(bs, I, st);
+  else
+    bt_persist_c<T, 4>(bs, I, st);
+}
+
+template void bt_launch_persist<double>(const BtSet&, const InterParams&, hipStream_t);
+template void bt_launch_persist<float>(const BtSet&, const InterParams&, hipStream_t);
 template void bt_finalize<double>(BtSet&);
 template void bt_finalize<float>(BtSet&);
 template void bt_launch_up<double>(const BtSet&, const InterParams&, hipStream_t, bool);

@@ -76,4 +76,13 @@ void bt_launch_up(const BtSet& bs, const InterParams& I, hipStream_t st, bool sc
 template <typename T>
 void bt_launch_dw(const BtSet& bs, hipStream_t st, bool scalars = true);
 
+// Persistent form: one block of 1024 threads per chain runs every remaining
+// step of its chain (up pass over all panels, beta, down pass over all
+// slabs, alpha) in a single launch, the reductions inside the block.  Shapes:
+// C 2 or 4 columns per panel, RS 8, 16 (or 32 in fp32) rows per slab.
+bool bt_persist_shape_ok(int C, int RS, size_t elem);
+size_t bt_persist_lds_bytes(int max_dimup, int max_dimdw, int C, int RS, size_t elem);
+template <typename T>
+void bt_launch_persist(const BtSet& bs, const InterParams& I, hipStream_t st);
+
 }  // namespace edl

@@ -79,6 +79,23 @@ struct ColorTable {
   int64_t slots = 0;           // sum of ns (table size / 64 words)
 };
 
+// Column ranges of the down spin for the range-blocked down pass: the
+// strings sorted ascending share the occupation of their top b sites over
+// contiguous index ranges lo[g] .. lo[g+1]-1; the hops of column j are
+// regrouped as its hops inside j's range first (word = local index in the
+// range | cid << 24, nloc[j] of them) then the others (global index),
+// words[off[j] .. off[j+1]).
+struct RangeTable {
+  int n = 0;                   // ranges
+  int maxcols = 0;             // columns of the largest range
+  int maxwords = 0;            // hop words of the largest range
+  int32_t* lo = nullptr;       // [n+1]
+  uint32_t* words = nullptr;   // [total hops]
+  int32_t* off = nullptr;      // [dimdw+1]
+  int32_t* nloc = nullptr;     // [dimdw]
+};
+constexpr int kMaxRangeBits = 8;
+
 // One spin species of one sector: sorted strings, hopping table, diagonal.
 struct SpinTable {
   int N = 0;           // electrons of this spin
@@ -106,8 +123,12 @@ struct SpinTable {
   double* d_lut64 = nullptr;   // device copies of the coefficient lut
   float* d_lut32 = nullptr;
   // Bank-scheduled copies of the packed table for the up pass, one per LDS
-  // element width W (dwords per staged row: 1 or 3), built on first use.
+  // element width W (dwords per staged row: 1 or 3), built with the sector.
   ColorTable pkc[4];
+  // Column ranges by top-site count b (the down spin's, for the
+  // range-blocked down pass), built with the sector for b = 1 .. 8 when
+  // the ranges are at least a few columns wide.
+  RangeTable rng[kMaxRangeBits + 1];
 };
 
 struct Sector {

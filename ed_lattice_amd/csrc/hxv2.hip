@@ -569,6 +569,134 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
   }
 }
 
+// ------------------------------------------ non-local Kanamori block (Jx, Jp)
+// T += N q between the up and the down pass: N (HxV_non_local.f90:1-88) moves
+// one up and one down electron between two orbitals of a site, so it is
+// neither a row nor a column operation; each element gathers its moves
+// (nonlocal_gather, the coefficient of the move that starts at it: the block
+// is real symmetric).  In a chain q is the current vector q_k (after the
+// step's beta, as the down pass reads it).
+template <typename T>
+__global__ __launch_bounds__(256) void k_nonlocal_add(TwoPassView V, InterParams I, T* qa, T* qb, T* tout, LzChain L,
+                                                      int chain) {
+  const T* q = qa;
+  if (chain) {
+    const LzScalars s = *L.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+  }
+  const int64_t n = (int64_t)V.dimup * V.dimdw;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int idw = (int)(e / V.dimup), iup = (int)(e % V.dimup);
+    const size_t i = (size_t)idw * V.ld + iup;
+    tout[i] += (T)nonlocal_gather<T>(I, (uint32_t)V.map_up[iup], (uint32_t)V.map_dw[idw], q, V.ld, I.Jx, I.Jp);
+  }
+}
+
+// ---------------------------------------------- pass D, range-blocked form
+// Tiles (row panel of R rows, column range g): the columns of a range share
+// the occupation of the top b sites of the down string, so the hops that
+// leave those sites alone (most of them: 53 % of the 4x4 sector's at b = 5)
+// stay inside the range.  A block stages the R-row slab of every column of
+// its range in LDS, with the range's hop words regrouped per column (inside
+// first, local indices); a lane is a row of one column (R = 64: a wave is
+// one column, R = 32: two), so the inside hops are conflict-free LDS reads
+// of consecutive rows, and the others are R-row segments of the same panel
+// from L2, where the other ranges' blocks of the panel (on the same XCD:
+// block b = 8 i + x takes range i % n of panel 8 (i / n) + x) staged them.
+// MODE as k_pass_dw.  Traffic per launch: q, T and out once.
+template <typename T, int R, int MODE>
+__global__ __launch_bounds__(1024) void k_pass_dwr(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
+                                                   const T* __restrict__ tin, T* outp, LzChain A, int b) {
+  constexpr int KCH = 8;
+  constexpr int CPW = 64 / R;  // columns per wave
+  __shared__ T lut[256];
+  __shared__ double sh[16];
+  extern __shared__ __align__(16) unsigned char dsm[];
+  LzScalars s{};
+  const T* __restrict__ q = qa;
+  T* __restrict__ out = outp;
+  if (MODE == 1) {
+    s = *A.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+    out = (s.k & 1) ? qa : qb;
+  }
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_dw[k];
+  const int dimup = V.dimup;
+  const size_t ld = (size_t)V.ld;
+  const int ng = V.rng_n[b];
+  const int blk = blockIdx.x, xcd = blk % 8, ib = blk / 8;
+  const int p = 8 * (ib / ng) + xcd, g = ib % ng;
+  const int npan = (dimup + R - 1) / R;
+  const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
+  const T ts = (T)inv_s;
+  const T bo = (MODE == 1) ? (T)(s.beta_k / s.s_old) : (T)0;
+  double part = 0.0;
+  if (p < npan) {
+    const int lo = V.rng_lo[b][g], nc = V.rng_lo[b][g + 1] - lo;
+    const int w0 = V.rng_off[b][lo], nw = V.rng_off[b][lo + nc] - w0;
+    T* S = reinterpret_cast<T*>(dsm);                                  // [nc][R]
+    uint32_t* tw = reinterpret_cast<uint32_t*>(S + (size_t)nc * R);  // [nw]
+    int* toff = reinterpret_cast<int*>(tw + nw);                      // [nc + 1]
+    int* tnl = toff + nc + 1;                                         // [nc]
+    const int r0 = p * R;
+    for (int e = threadIdx.x; e < nc * R; e += blockDim.x) {
+      const int jl = e / R, r = e % R;
+      S[e] = (r0 + r < dimup) ? q[(size_t)(lo + jl) * ld + r0 + r] : (T)0;
+    }
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) tw[i] = __builtin_nontemporal_load(&V.rng_words[b][w0 + i]);
+    for (int c = threadIdx.x; c <= nc; c += blockDim.x) {
+      toff[c] = V.rng_off[b][lo + c] - w0;
+      if (c < nc) tnl[c] = V.rng_nloc[b][lo + c];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int r = lane % R, cs = lane / R;
+    const bool rok = r0 + r < dimup;
+    for (int jb = wave * CPW; jb < nc; jb += nwv * CPW) {
+      const int jl = jb + cs;
+      const bool cok = jl < nc && rok;
+      const int jc = jl < nc ? jl : jb;
+      const size_t iel = (size_t)(lo + jc) * ld + r0 + r;
+      const T t0 = cok ? __builtin_nontemporal_load(&tin[iel]) : (T)0;
+      const T o0 = (MODE == 1 && cok) ? __builtin_nontemporal_load(&out[iel]) : (T)0;
+      const uint32_t* wp = tw + toff[jc];
+      const int nl = jl < nc ? tnl[jc] : 0;
+      const int nt = jl < nc ? toff[jc + 1] - toff[jc] : 0;
+      T acc = (T)0;
+      for (int k = 0; k < nl; ++k) {
+        const uint32_t w = wp[k];
+        acc += lut[w >> 24] * S[(int)(w & 0xFFFFFFu) * R + r];
+      }
+      for (int k0 = nl; k0 < nt; k0 += KCH) {
+        uint32_t w[KCH];
+        T x[KCH];
+#pragma unroll
+        for (int j = 0; j < KCH; ++j) w[j] = (k0 + j < nt) ? wp[k0 + j] : ((uint32_t)(lo + jc) | (255u << 24));
+#pragma unroll
+        for (int j = 0; j < KCH; ++j)
+          x[j] = (rok && k0 + j < nt) ? q[(size_t)(w[j] & 0xFFFFFFu) * ld + r0 + r] : (T)0;
+#pragma unroll
+        for (int j = 0; j < KCH; ++j) acc += lut[w[j] >> 24] * x[j];
+      }
+      if (cok) {
+        if (MODE == 0) {
+          __builtin_nontemporal_store(t0 + acc, &out[iel]);
+        } else {
+          const T tv = (t0 + acc) * ts - bo * o0;
+          __builtin_nontemporal_store(tv, &out[iel]);
+          part += (double)tv * (double)S[jc * R + r];
+        }
+      }
+    }
+  }
+  if (MODE == 1) {
+    const double v = block_sum(part * inv_s, sh);
+    if (threadIdx.x == 0) A.part_dw[blk] = v;
+  }
+}
+
 // ------------------------------------------------------ chain reductions
 // One block each, between the passes: the kernel boundary makes the block
 // partials visible to it (a fence inside the passes would write back and
@@ -801,9 +929,70 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
 }
 
 void build_up_color_tables(Sector& s, hipStream_t st) {
-  if (s.nonlocal || !s.sp[0].packed_ok || s.sp[0].dim < 64) return;
+  if (!s.sp[0].packed_ok || s.sp[0].dim < 64) return;
   build_color_table(s.sp[0], 1, st);
   build_color_table(s.sp[0], 3, st);
+}
+
+// Column ranges of b top sites for every b whose ranges average at least 16
+// columns (RangeTable).
+void build_dw_range_tables(Sector& s, hipStream_t st) {
+  SpinTable& t = s.sp[1];
+  if (!t.packed_ok || t.dim < 256) return;
+  const int dim = t.dim;
+  std::vector<int32_t> map(dim), off(dim + 1);
+  EDL_HIP(hipMemcpyAsync(map.data(), t.map, sizeof(int32_t) * dim, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(off.data(), t.off, sizeof(int32_t) * (dim + 1), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  std::vector<uint32_t> csr((size_t)off[dim]);
+  if (!csr.empty())
+    EDL_HIP(hipMemcpyAsync(csr.data(), t.csr, sizeof(uint32_t) * csr.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  int Ns = 0;
+  for (int j = 0; j < dim; ++j)
+    while (Ns < 31 && (map[j] >> Ns) != 0) ++Ns;
+  for (int b = 1; b <= kMaxRangeBits && b < Ns; ++b) {
+    RangeTable& rt = t.rng[b];
+    if (rt.lo) continue;
+    std::vector<int32_t> lo{0}, rid(dim);
+    for (int j = 1; j < dim; ++j)
+      if ((map[j] >> (Ns - b)) != (map[j - 1] >> (Ns - b))) lo.push_back(j);
+    lo.push_back(dim);
+    const int n = (int)lo.size() - 1;
+    if (dim / n < 16) break;  // more top sites only make narrower ranges
+    int maxc = 0, maxw = 0;
+    for (int g = 0; g < n; ++g) {
+      for (int j = lo[g]; j < lo[g + 1]; ++j) rid[j] = g;
+      maxc = std::max(maxc, lo[g + 1] - lo[g]);
+      maxw = std::max(maxw, off[lo[g + 1]] - off[lo[g]]);
+    }
+    std::vector<uint32_t> words(csr.size());
+    std::vector<int32_t> nloc(dim);
+    for (int j = 0; j < dim; ++j) {
+      int a = off[j], z = off[j + 1];
+      for (int k = off[j]; k < off[j + 1]; ++k) {
+        const uint32_t w = csr[k];
+        const int nb = (int)(w & 0xFFFFFFu);
+        if (rid[nb] == rid[j])
+          words[a++] = (uint32_t)(nb - lo[rid[j]]) | (w & 0xFF000000u);
+        else
+          words[--z] = w;
+      }
+      nloc[j] = a - off[j];
+    }
+    EDL_HIP(hipMalloc(&rt.lo, sizeof(int32_t) * (n + 1)));
+    EDL_HIP(hipMalloc(&rt.words, sizeof(uint32_t) * std::max<size_t>(words.size(), 1)));
+    EDL_HIP(hipMalloc(&rt.off, sizeof(int32_t) * (dim + 1)));
+    EDL_HIP(hipMalloc(&rt.nloc, sizeof(int32_t) * dim));
+    EDL_HIP(hipMemcpyAsync(rt.lo, lo.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(rt.words, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(rt.off, off.data(), sizeof(int32_t) * (dim + 1), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(rt.nloc, nloc.data(), sizeof(int32_t) * dim, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
+    rt.n = n;
+    rt.maxcols = maxc;
+    rt.maxwords = maxw;
+  }
 }
 
 static size_t up_lds_bytes(int dimup, int C, size_t sz) {
@@ -812,7 +1001,7 @@ static size_t up_lds_bytes(int dimup, int C, size_t sz) {
 }
 
 bool two_pass_ok(const Sector& s, size_t elem) {
-  return !s.nonlocal && s.sp[0].packed_ok && s.sp[1].packed_ok &&
+  return s.sp[0].packed_ok && s.sp[1].packed_ok &&
          up_lds_bytes(s.sp[0].dim, 1, elem) <= (size_t)lds_limit_bytes() - 256;
 }
 
@@ -841,6 +1030,16 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
     v.pkc_up[W] = s.sp[0].pkc[W].words;
     v.pkc_off[W] = s.sp[0].pkc[W].off;
     v.pkc_ns[W] = s.sp[0].pkc[W].ns;
+  }
+  for (int b = 0; b <= kMaxRangeBits; ++b) {
+    const RangeTable& rt = s.sp[1].rng[b];
+    v.rng_lo[b] = rt.lo;
+    v.rng_words[b] = rt.words;
+    v.rng_off[b] = rt.off;
+    v.rng_nloc[b] = rt.nloc;
+    v.rng_n[b] = rt.n;
+    v.rng_maxcols[b] = rt.maxcols;
+    v.rng_maxwords[b] = rt.maxwords;
   }
   return v;
 }
@@ -911,6 +1110,21 @@ void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, T*
   }
 }
 
+template <typename T>
+void launch_nonlocal(const TwoPassView& V, const InterParams& I, T* qa, T* qb, T* tout, const LzChain* L,
+                     hipStream_t st) {
+  if (I.Jx == 0.0 && I.Jp == 0.0) return;
+  const int64_t n = (int64_t)V.dimup * V.dimdw;
+  const int64_t g = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_nonlocal_add<T>, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0, st, V, I, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0);
+  EDL_HIP(hipGetLastError());
+}
+template void launch_nonlocal<double>(const TwoPassView&, const InterParams&, double*, double*, double*,
+                                      const LzChain*, hipStream_t);
+template void launch_nonlocal<float>(const TwoPassView&, const InterParams&, float*, float*, float*, const LzChain*,
+                                     hipStream_t);
+
 void launch_lz_beta(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st) {
   hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(256), 0, st, L, pass_up_grid(V, cfg));
   EDL_HIP(hipGetLastError());
@@ -923,7 +1137,18 @@ void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& c
 
 int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
   const int npan = (V.dimup + cfg.R - 1) / cfg.R;
+  if (cfg.dw_rng > 0) return 8 * V.rng_n[cfg.dw_rng] * ((npan + 7) / 8);
   return 8 * cfg.nchunk * ((npan + 7) / 8);
+}
+
+static size_t pass_dwr_lds(const TwoPassView& V, int R, int b, size_t elem) {
+  return elem * (size_t)V.rng_maxcols[b] * R + sizeof(uint32_t) * (size_t)V.rng_maxwords[b] +
+         sizeof(int) * (2 * (size_t)V.rng_maxcols[b] + 1);
+}
+
+bool pass_dwr_ok(const TwoPassView& V, int R, int b, size_t elem) {
+  return b >= 1 && b <= kMaxRangeBits && V.rng_lo[b] != nullptr && (R == 32 || R == 64) &&
+         pass_dwr_lds(V, R, b, elem) <= (size_t)lds_limit_bytes() - 4096;
 }
 
 template <typename T, int MODE, int NCOL, int KCH>
@@ -945,6 +1170,7 @@ static void launch_dw_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
 // Dynamic LDS of a down-pass block: the chunk's hop words and their
 // offsets, bounded by ncmax * (kdw + 1) words, ncmax = ceil(dimdw / nchunk) + 1.
 size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg) {
+  if (cfg.dw_rng > 0) return (size_t)-1;  // depends on the element size: see pass_dwr_ok
   const size_t ncmax = (size_t)((V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1);
   return sizeof(uint32_t) * ncmax * (size_t)(V.kdw + 1);
 }
@@ -982,10 +1208,26 @@ bool pass_dw_vec_ok(const TwoPassView& V, const TwoPassCfg& cfg, size_t elem, co
   return true;
 }
 
+template <typename T, int R, int MODE>
+static void launch_dwr_k(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
+                         const TwoPassCfg& cfg, hipStream_t st) {
+  const size_t lds = pass_dwr_lds(V, R, cfg.dw_rng, sizeof(T));
+  set_lds_limit((const void*)k_pass_dwr<T, R, MODE>, lds_limit_bytes() - 4096);
+  hipLaunchKernelGGL((k_pass_dwr<T, R, MODE>), dim3(pass_dw_grid(V, cfg)), dim3(cfg.threads_dw), lds, st, V, lut, qa,
+                     qb, tin, out, A, cfg.dw_rng);
+  EDL_HIP(hipGetLastError());
+}
+
 template <typename T, int MODE>
 static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain& A,
                         const TwoPassCfg& cfg_in, hipStream_t st) {
   TwoPassCfg cfg = cfg_in;
+  if (cfg.dw_rng > 0) {
+    EDL_CHECK(pass_dwr_ok(V, cfg.R, cfg.dw_rng, sizeof(T)) && cfg.threads_dw % 64 == 0,
+              "launch_pass_dw: no range tables for this top-site count, or the range exceeds LDS");
+    if (cfg.R == 32) return launch_dwr_k<T, 32, MODE>(V, lut, qa, qb, tin, out, A, cfg, st);
+    return launch_dwr_k<T, 64, MODE>(V, lut, qa, qb, tin, out, A, cfg, st);
+  }
   const void* bufs[4] = {qa, qb, tin, out};
   if (cfg.dw_vec && !pass_dw_vec_ok(V, cfg, sizeof(T), bufs, 4)) {
     // natural layout (ld = DimUp) or unaligned caller buffers: the 4-byte form
@@ -1252,7 +1494,7 @@ int seg_row_cap(size_t elem, int C) {
 }
 
 bool seg_path(const Sector& s, size_t elem) {
-  if (s.nonlocal || !(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1)
+  if (!(s.sp[0].packed_ok && s.sp[1].packed_ok) || s.sp[0].dim < 1 || s.sp[1].dim < 1)
     return false;
   return !two_pass_ok(s, elem) || seg_row_cap(elem, 1) < s.sp[0].dim;
 }

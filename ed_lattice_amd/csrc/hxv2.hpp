@@ -22,6 +22,14 @@ struct TwoPassView {
   const uint32_t* pkc_up[4];
   const int32_t* pkc_off[4];
   const int32_t* pkc_ns[4];
+  // down-spin column ranges by top-site count b (SpinTable::rng), or null
+  const int32_t* rng_lo[kMaxRangeBits + 1];
+  const uint32_t* rng_words[kMaxRangeBits + 1];
+  const int32_t* rng_off[kMaxRangeBits + 1];
+  const int32_t* rng_nloc[kMaxRangeBits + 1];
+  int rng_n[kMaxRangeBits + 1];
+  int rng_maxcols[kMaxRangeBits + 1];
+  int rng_maxwords[kMaxRangeBits + 1];
 };
 
 // One Lanczos chain driven by the two passes.  A step is four launches:
@@ -62,9 +70,11 @@ struct TwoPassCfg {
   int kch = 16;         // hop chunk of the down pass (8, 16, 32 with 1 column; 8, 16 with 2)
   int dw_vec = 0;       // 1: vector-gather down pass (16-byte lanes; R 16|32|64|128, 256 in fp32;
                         //    (ncol, kch) in (1, 2|4), (2, 1|2|4))
+  int dw_rng = 0;       // b > 0: range-blocked down pass over the column ranges of b top sites
+                        //    (R 32|64, threads_dw; ncol, kch, nchunk, dw_vec unused)
   // Down-pass shape of the Lanczos chains (the pass with the Lanczos terms
   // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
-  int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0;
+  int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0, ch_rng = 0;
   // Up-pass shape of the chains (phase B fused), likewise; ch_C = 0: the operator's.
   int ch_C = 0, ch_up_threads = 0, ch_grid_up = 0, ch_up_rpt = 0, ch_up_kch = 0, ch_up_col = 0;
 };
@@ -80,6 +90,7 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
     r.ncol = c.ch_ncol;
     r.kch = c.ch_kch;
     r.dw_vec = c.ch_vec;
+    r.dw_rng = c.ch_rng;
   }
   if (c.ch_C > 0) {
     r.C = c.ch_C;
@@ -97,6 +108,11 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
 bool up_col_ok(const TwoPassView& V, int C, size_t elem);
 // Builds the bank-scheduled up-spin tables of a two-pass sector (once).
 void build_up_color_tables(Sector& s, hipStream_t st);
+// Builds the down-spin column-range tables of a two-pass sector (once).
+void build_dw_range_tables(Sector& s, hipStream_t st);
+// Whether the range tables for b top sites exist and a block of R rows of
+// the largest range fits in LDS (slab, hop words, per-column offsets).
+bool pass_dwr_ok(const TwoPassView& V, int R, int b, size_t elem);
 
 // Blocks of a down-pass launch (8 * nchunk * ceil(panels / 8)), the size of
 // LzChain::part_dw; blocks of an up-pass launch, the size of part_up.
@@ -130,6 +146,12 @@ void launch_pass_up(const TwoPassView& V, const InterParams& I, const T* lut, T*
 template <typename T>
 void launch_pass_dw(const TwoPassView& V, const T* lut, T* qa, T* qb, const T* tin, T* out, const LzChain* A,
                     const TwoPassCfg& cfg, hipStream_t st);
+// T += N q, the non-local Kanamori block (Jx, Jp; nothing when both are 0):
+// operator mode (L = null) with q = qa, chain mode with q = q_k of the chain
+// (after its beta launch).
+template <typename T>
+void launch_nonlocal(const TwoPassView& V, const InterParams& I, T* qa, T* qb, T* tout, const LzChain* L,
+                     hipStream_t st);
 void launch_lz_beta(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
 void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& cfg, hipStream_t st);
 

@@ -306,6 +306,12 @@ void free_sector(Sector& s) {
       if (ct.off) (void)hipFree(ct.off);
       if (ct.ns) (void)hipFree(ct.ns);
     }
+    for (auto& rt : t.rng) {
+      if (rt.lo) (void)hipFree(rt.lo);
+      if (rt.words) (void)hipFree(rt.words);
+      if (rt.off) (void)hipFree(rt.off);
+      if (rt.nloc) (void)hipFree(rt.nloc);
+    }
     t = SpinTable{};
   }
 }

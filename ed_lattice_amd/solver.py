@@ -220,12 +220,12 @@ class EDLattice:
     def bench_passes(self, isector, prec="fp32", reps=10):
         """Mean ms of each kernel launch of the operator and its schedule."""
         ms = np.zeros(2)
-        cfg = np.zeros(15, np.int32)
+        cfg = np.zeros(16, np.int32)
         self._ck(self.lib.edl_bench_passes(self.h, isector, PREC[prec], reps, ms.ctypes.data_as(_lib._dp),
                                            cfg.ctypes.data_as(_lib._ip)))
         keys = ("two_pass", "ld", "up_cols", "up_threads", "up_grid", "dw_rows", "dw_threads", "dw_chunks",
                 "dw_cols_per_lane", "dw_hop_chunk", "up_rows_per_thread", "up_hop_chunk", "dw_vector_gather",
-                "up_coef_registers", "up_bank_scheduled")
+                "up_coef_registers", "up_bank_scheduled", "dw_range_sites")
         return float(ms[0]), float(ms[1]), dict(zip(keys, (int(x) for x in cfg)))
 
     def last_timing(self):

@@ -183,7 +183,7 @@ int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_
 /* Per-kernel split of the same operator: ms_pass[0] and ms_pass[1] are the
  * mean durations of the up-pass and down-pass launches (HIP events around
  * each launch on the launch stream; a single-pass sector reports its one
- * kernel in ms_pass[0] and 0 in ms_pass[1]).  cfg[15] receives the tuned
+ * kernel in ms_pass[0] and 0 in ms_pass[1]).  cfg[16] receives the tuned
  * schedule: {two_pass (1 = LDS column panels, 2 = segmented up pass for
  * columns larger than LDS, 0 = single pass), column stride, up columns per
  * block, up threads, up grid, down panel rows, down threads, down column
@@ -191,7 +191,8 @@ int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_
  * chunk, down-pass gathers of 16 bytes per lane (1) or one element per lane
  * (0), up-pass hop coefficients from registers (1: fp32, one distinct
  * hopping value) or from the LDS table (0), up-pass hops in the bank-scheduled
- * order (1) or in table order (0)}. */
+ * order (1) or in table order (0), down pass over column ranges of that many
+ * top sites (b > 0, range-blocked form) or over column chunks (0)}. */
 int edl_bench_passes(edl_ctx* ctx, int isector, int prec, int reps, double* ms_pass, int32_t* cfg);
 
 #ifdef __cplusplus

@@ -206,8 +206,15 @@ def test_driver_finite_temperature_ring6(tmp_path, orc):
         docc += b / Z * o["docc"].sum()
     egs = float((tmp_path / "egs.ed").read_text().split()[0])
     assert abs(egs - emin) < 1e-10 * abs(emin)
-    lst = np.loadtxt(tmp_path / "state_list.ed", ndmin=2)
-    np.testing.assert_allclose(lst[:, 1], [k[0] for k in kept], rtol=0, atol=1e-9)
+    # the state list as printed before the cut (print_state_list: i, E_i,
+    # exp(-(E-E0)/T), sector, dim, nup, ndw), then trimmed to the cutoff
+    rows = [ln.split() for ln in r.stdout.splitlines()]
+    table = [float(t[1]) for t in rows if len(t) == 7 and t[0].isdigit() and "E" in t[2]]
+    ncut = int(r.stdout.split("Adjusting lanc_nstates_total to:")[1].split()[0])
+    assert ncut == len(kept)
+    np.testing.assert_allclose(sorted(table[:ncut]), [k[0] for k in kept], rtol=0, atol=1e-9)
+    lst = np.loadtxt(tmp_path / "state_list.ed", ndmin=2)   # istate, isector, nup, ndw
+    assert lst.shape[1] == 4 and lst.shape[0] == len(table)
     dens = np.array((tmp_path / "dens.ed").read_text().split(), float)
     dc = np.array((tmp_path / "docc.ed").read_text().split(), float)
     assert abs(dens.sum() - ntot) < 1e-8

@@ -118,6 +118,67 @@ def _three_orb():
     return H
 
 
+def _two_orb_ring4():
+    # 2 orbitals x 4 sites: rings of 4 per orbital (orbital 2 is sites 5-8),
+    # on-site hybridisation, crystal field
+    Ns = 8
+    H = np.zeros((1, Ns, Ns))
+    for a in range(4):
+        b = (a + 1) % 4
+        H[0, a, b] = H[0, b, a] = -1.0
+        H[0, 4 + a, 4 + b] = H[0, 4 + b, 4 + a] = -0.7
+        H[0, a, 4 + a] = H[0, 4 + a, a] = -0.2
+    for k in range(4):
+        H[0, k, k], H[0, 4 + k, 4 + k] = 0.15, -0.15
+    return H
+
+
+def test_kanamori_on_two_pass_chains_and_batched(ed, orc, monkeypatch):
+    """The non-local Kanamori block (Jx, Jp; HxV_non_local.f90) as the third
+    term of the two-pass operator (T += N q between the passes), in the
+    ground-state chains and in both batched seed forms: 2 orbitals x 4
+    sites, sector (4,4), dim 4900, against the oracle and dense eigh."""
+    H = _two_orb_ring4()
+    U = [3.0, 2.5]
+    kw = dict(Nsites=[4, 4], Ust=1.5, Jh=0.5, Jx=0.5, Jp=0.4, xmu=0.1)
+    m = _use(ed, orc, H, U, **kw)
+    isec = ed.sector_index(4, 4)
+    dim, _, _ = ed.build_sector(isec)
+    for prec in ("fp64", "fp32"):
+        assert ed.bench_passes(isec, prec, 1)[2]["two_pass"] == 1
+    for _ in range(2):
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, 4, 4, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+    Hd = orc.hamiltonian_csr(m, 4, 4).toarray().real
+    w, V = np.linalg.eigh(Hd)
+    assert w[1] - w[0] > 1e-6
+    r = ed.lanczos_eigh(isec, "mixed", niter=300, want_vector=True)
+    assert abs(r["e0"] - w[0]) < 1e-10 * abs(w[0])
+    assert 1 - abs(r["gs"] @ V[:, 0]) < 1e-9
+    seeds = [("cdg", 1, 1, 0), ("c", 1, 5, 0), ("c", 1, 2, 6), ("sz", 1, 3, 0)]
+    targets = [(5, 4), (3, 4), (3, 4), (4, 4)]
+    refs = []
+    for (op, ispin, io, jo), tg in zip(seeds, targets):
+        if op == "sz":
+            s = orc.seed_sz(8, io, (4, 4), r["gs"]).real
+        else:
+            opc = 2 if op == "cdg" else 1
+            s = orc.seed_c_cdg(8, opc, io, 1, (4, 4), tg, r["gs"]).real
+            if jo:
+                s = s + orc.seed_c_cdg(8, opc, jo, 1, (4, 4), tg, r["gs"]).real
+        a, b, _ = orc.lanczos_tridiag(orc.hamiltonian_csr(m, *tg), s, 100)
+        refs.append((s @ s, a, b))
+    for form in ("tiles", "persist"):
+        monkeypatch.setenv("EDL_BATCHED_KERNELS", form)
+        rb = ed.tridiag_batched(seeds, 100, "fp64")
+        print(f"kanamori seeds, {form}: {rb['ms']:.2f} ms", flush=True)
+        for k, (n2, a, b) in enumerate(refs):
+            assert abs(rb["norm2"][k] - n2) < 1e-11
+            _assert_leading(rb["alpha"][k], rb["beta"][k], a, b, 10, 1e-9)
+
+
 def _two_orb():
     H = np.zeros((1, 4, 4))
     for a, b, t in [(0, 1, -1.0), (2, 3, -0.6), (0, 2, -0.3), (1, 3, -0.3)]:
@@ -211,6 +272,34 @@ def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, 
         assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
         assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
     m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    Hc = orc.hamiltonian_csr(m, 6, 6)
+    e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=200, refine_tol=1e-13)
+    assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
+
+
+@pytest.mark.parametrize("R,b,th", [(32, 2, 256), (64, 3, 1024), (32, 4, 512)])
+def test_hxv_range_blocked_down_pass_vs_oracle(ed, orc, R, b, th, monkeypatch):
+    """The range-blocked down pass (row slabs of the column ranges that share
+    the occupation of the top b sites staged in LDS, hops inside a range
+    from LDS, the others from global memory), pinned: H x v on sectors of
+    the 3x4 cluster against the oracle in fp64 and fp32 (natural and padded
+    layouts), and a mixed ground state whose chains run the same form with
+    the Lanczos terms fused (MODE 1)."""
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"3,1024,256,{R},{th},32,2,8,1,8,0,1,{b}")
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", f"1,512,512,{R},{th},32,2,8,2,8,0,0,{b}")
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    for sec in ((7, 6), (6, 6), (5, 6), (6, 4)):
+        isec = ed.sector_index(*sec)
+        dim, du, dd = ed.build_sector(isec)
+        print(f"range-blocked R {R} b {b}: sector {sec} DimUp {du} DimDw {dd}", flush=True)
+        for prec in ("fp32", "fp64"):
+            assert ed.bench_passes(isec, prec, 1)[2]["dw_range_sites"] == b
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
     isec = ed.sector_index(6, 6)
     Hc = orc.hamiltonian_csr(m, 6, 6)
     e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
@@ -448,13 +537,17 @@ def _assert_converged_poles_match(gpu, ora, tol, window=1e-6, conv_tol=1e-10):
             assert abs((p[sg] * w[sg]).sum() / wg - (po[so] * wo[so]).sum() / wo_) < tol * max(1.0, 1e-6 / wg)
 
 
-@pytest.fixture(params=["two_pass", "single", "chain", "chain_segmented"])
+@pytest.fixture(params=["two_pass", "tiles", "persist", "single", "chain", "chain_segmented"])
 def batched_kernels(request, monkeypatch):
     """Kernels of the batched chains: the batched two-pass LDS kernels
-    (default where the sectors fit), the single-pass gather kernels, or one
-    chain at a time through the two-pass step (the route of sectors large
-    enough to fill the GPU alone), also with the segmented up pass."""
-    if request.param == "single":
+    (default where the sectors fit; the faster of the tile launches and the
+    persistent launch, or either one forced), the single-pass gather
+    kernels, or one chain at a time through the two-pass step (the route of
+    sectors large enough to fill the GPU alone), also with the segmented up
+    pass."""
+    if request.param in ("tiles", "persist"):
+        monkeypatch.setenv("EDL_BATCHED_KERNELS", request.param)
+    elif request.param == "single":
         monkeypatch.setenv("EDL_BATCHED_KERNELS", "single")
     elif request.param.startswith("chain"):
         monkeypatch.setenv("EDL_BATCHED_KERNELS", "chain")
@@ -562,9 +655,12 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, w
                                           conv_tol=conv_tol)
 
 
-def test_gf_batched_equals_sequential_rect3x4(ed, orc):
+@pytest.mark.parametrize("form", ["tiles", "persist"])
+def test_gf_batched_equals_sequential_rect3x4(ed, orc, form, monkeypatch):
     """Config rect3x4: 2 x 78 seeds (i <= j pairs, particle + hole) of spin up;
-    one batched launch vs one seed at a time, and a few seeds vs the oracle."""
+    one batched launch (tile launches per step, or one persistent launch) vs
+    one seed at a time, and a few seeds vs the oracle."""
+    monkeypatch.setenv("EDL_BATCHED_KERNELS", form)
     m = _use(ed, orc, lattice.square(3, 4), [6.0])
     isec = ed.sector_index(6, 6)
     r = ed.lanczos_eigh(isec, "fp64", niter=300, want_vector=True)
