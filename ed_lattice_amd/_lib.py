@@ -38,6 +38,7 @@ SIGNATURES = {
     "edl_sector_hops": (c_int, [c_void_p, c_int, c_int, _i32p, _i32p, _dp]),
     "edl_sector_rank": (c_int, [c_void_p, c_int, c_int, c_int64, _i32p, _i32p]),
     "edl_hxv": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "edl_hxv_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "edl_hxv_host": (c_int, [c_void_p, c_int, c_int, _dp, _dp]),
     "edl_hxv_host_complex": (c_int, [c_void_p, c_int, c_int, _dp, _dp]),
     "edl_vec_random": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_uint64]),

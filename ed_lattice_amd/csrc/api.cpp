@@ -1182,6 +1182,7 @@ extern "C" int edl_ctx_create(edl_ctx** out, int device) {
     EDL_HIP(hipGetDeviceProperties(&prop, device));
     ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     init_binomials();
+    noalloc_scratch_setup(device);
     *out = ctx;
     return 0;
   } catch (const std::exception& e) {
@@ -1377,6 +1378,46 @@ extern "C" int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v, void*
   API_END
 }
 
+// edl_hxv_noalloc: the two passes with the intermediate T in the file-scope
+// device scratch, the schedule the tuner cached for this sector shape (a
+// fixed one when none is cached: tuning allocates), no allocation, no
+// events.
+template <typename T>
+static void hxv_noalloc_impl(Ctx& c, Sector& s, const T* v, T* hv) {
+  EDL_CHECK(!seg_path(s, sizeof(T)) && two_pass_ok(s, sizeof(T)),
+            "edl_hxv_noalloc: the sector needs the two-pass kernels (columns within LDS, packed hop tables)");
+  const int ld = s.sp[0].dim;
+  const size_t bytes = sizeof(T) * (size_t)ld * s.sp[1].dim;
+  EDL_CHECK(bytes <= noalloc_scratch_bytes(),
+            "edl_hxv_noalloc: the sector's intermediate (" + std::to_string(bytes) +
+                " bytes) exceeds the 256 MiB scratch; use edl_hxv");
+  const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T));
+  TwoPassCfg cfg;
+  auto it = tp_cache().find(key);
+  if (it != tp_cache().end()) {
+    cfg = it->second;
+  } else {
+    cfg.C = 1;
+    cfg.up_rpt = 2;
+    cfg.up_kch = 8;
+    cfg.threads_up = 1024;
+    cfg.grid_up = c.num_cu;
+    cfg.R = 64;
+    cfg.threads_dw = 256;
+    cfg.nchunk = std::min(32, s.sp[1].dim);
+    cfg.ncol = 1;
+    cfg.kch = 16;
+    cfg.dw_vec = 0;
+  }
+  const TwoPassView V = two_pass_view(s, ld);
+  T* tmp = static_cast<T*>(noalloc_scratch(c.device));
+  T* vin = const_cast<T*>(v);  // operator mode only reads it
+  launch_pass_up<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, cfg, c.stream);
+  launch_nonlocal<T>(V, c.model.inter, vin, vin, tmp, nullptr, c.stream);
+  launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, hv, nullptr, cfg, c.stream);
+  EDL_HIP(hipStreamSynchronize(c.stream));
+}
+
 template <typename T>
 static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_t stride) {
   const int64_t D = s.dim;
@@ -1395,6 +1436,21 @@ static void hxv_host_impl(Ctx& c, Sector& s, const double* v, double* hv, int64_
   EDL_HIP(hipMemcpyAsync(tmp.data(), d64.p, sizeof(double) * D, hipMemcpyDeviceToHost, st));
   EDL_HIP(hipStreamSynchronize(st));
   for (int64_t i = 0; i < D; ++i) hv[i * stride] = tmp[i];
+}
+
+extern "C" int edl_hxv_noalloc(edl_ctx* ctx, int isector, int prec, const void* v, void* hv) {
+  API_BEGIN
+  EDL_CHECK(v && hv, "edl_hxv_noalloc: null vector");
+  EDL_CHECK(prec == EDL_FP64 || prec == EDL_FP32, "edl_hxv_noalloc: prec must be EDL_FP64 or EDL_FP32");
+  Ctx& c = ctx->c;
+  EDL_HIP(hipSetDevice(c.device));
+  auto it = c.sectors.find(isector);
+  EDL_CHECK(it != c.sectors.end(), "edl_hxv_noalloc: sector not built (call edl_sector_build first)");
+  if (prec == EDL_FP64)
+    hxv_noalloc_impl<double>(c, *it->second, static_cast<const double*>(v), static_cast<double*>(hv));
+  else
+    hxv_noalloc_impl<float>(c, *it->second, static_cast<const float*>(v), static_cast<float*>(hv));
+  API_END
 }
 
 extern "C" int edl_hxv_host(edl_ctx* ctx, int isector, int prec, const double* v, double* hv) {

@@ -192,4 +192,9 @@ struct Ctx {
 
 Sector& get_sector(Ctx& c, int isector);
 
+// noalloc.hip: the file-scope device scratch of edl_hxv_noalloc
+void noalloc_scratch_setup(int device);  // with `device` current, once per device
+void* noalloc_scratch(int device);       // that device's address of the array
+size_t noalloc_scratch_bytes();
+
 }  // namespace edl

@@ -5,3 +5,4 @@
 #include "seeds.hip"
 #include "hxv2.hip"
 #include "batched2.hip"
+#include "noalloc.hip"

@@ -75,6 +75,18 @@ int edl_sector_rank(edl_ctx* ctx, int isector, int ispin, int64_t n, const int32
  * operator of VARS_GLOBAL.f90:102-108: hv = H v.  Device pointers to
  * double (EDL_FP64) or float (EDL_FP32) arrays of Dim elements. */
 int edl_hxv(edl_ctx* ctx, int isector, int prec, const void* v_dev, void* hv_dev);
+/* edl_hxv without any allocation -- no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The two-pass
+ * intermediate lives in a file-scope device array of 256 MiB that every GPU
+ * the library runs on holds until the process ends; sectors whose
+ * intermediate (DimUp * DimDw elements of `prec`) exceeds it, sectors not
+ * built yet (edl_sector_build first) and sectors outside the two-pass
+ * kernels are refused with an error.  The schedule is the one the tuner
+ * cached for the sector shape (after an edl_hxv of that shape), else a fixed
+ * one.  Returns when hv is written.  Calls that overlap on one GPU share
+ * the array and race. */
+int edl_hxv_noalloc(edl_ctx* ctx, int isector, int prec, const void* v_dev, void* hv_dev);
 /* Same with host FP64 arrays (converted to `prec` on the device). */
 int edl_hxv_host(edl_ctx* ctx, int isector, int prec, const double* v, double* hv);
 /* Same with host complex(8) arrays, interleaved (re, im). */
