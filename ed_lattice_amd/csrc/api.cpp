@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <tuple>
 
@@ -204,9 +205,9 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
                               &p.up_col, &p.dw_rng);
     EDL_CHECK(p.dw_rng == 0 || pass_dwr_ok(V, p.R, p.dw_rng, sizeof(T)),
               "EDL_TWO_PASS_CFG_F32/_F64: dw_rng needs range tables for that many top sites and R 32|64 within LDS");
-    EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T)),
-              "EDL_TWO_PASS_CFG_F32/_F64: up_col needs fp32 panels of 1 or 3 columns on a sector with the "
-              "bank-scheduled table");
+    EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T), p.up_col),
+              "EDL_TWO_PASS_CFG_F32/_F64: up_col 1 needs fp32 panels of 1 or 3 columns on a sector with the "
+              "bank-scheduled table, up_col 2 also one hopping magnitude and DimUp + 32 < 2^15");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
                       "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
@@ -268,15 +269,15 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     best.grid_up = c.num_cu;
     best.nchunk = std::min(32, V.dimdw);
     double best_up = 1e300, best_dw = 1e300;
-    for (int col : {0, 1})
+    for (int col : {0, 1, 2})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
       for (int th : {512, 1024})
         for (int gm : {1, 2}) {
-          const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
+          const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col == 2 ? 32 : 0)) * C * sizeof(T);
           if (need > (size_t)lds) continue;
           if (gm == 2 && 2 * need > (size_t)lds) continue;
-          if (col && !up_col_ok(V, C, sizeof(T))) continue;
+          if (col && !up_col_ok(V, C, sizeof(T), col)) continue;
           TwoPassCfg w = best;
           w.C = C;
           w.up_col = col;
@@ -465,14 +466,14 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     DevBuf pu(sizeof(double) * (size_t)c.num_cu * 2);
     A.part_up = pu.as<double>();
     double bu2 = 1e300;
-    for (int col : {0, 1})
+    for (int col : {0, 1, 2})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
         for (int th : {512, 1024})
           for (int gm : {1, 2}) {
-            const size_t need = 256 * sizeof(T) + (size_t)V.dimup * C * sizeof(T);
+            const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col == 2 ? 32 : 0)) * C * sizeof(T);
             if (need > (size_t)lds || (gm == 2 && 2 * need > (size_t)lds)) continue;
-            if (col && !up_col_ok(V, C, sizeof(T))) continue;
+            if (col && !up_col_ok(V, C, sizeof(T), col)) continue;
             TwoPassCfg w = best;
             w.C = C;
             w.up_col = col;
@@ -845,47 +846,84 @@ struct TwoPassStep {
   double* defl_coef = nullptr;
 };
 
+// EDL_GRAPHS=0: the steps are launched directly, one launch per kernel and
+// step, instead of replaying a captured graph (same kernels and arguments;
+// for tools that collect per-dispatch counters).
+static bool graphs_enabled() {
+  const char* e = std::getenv("EDL_GRAPHS");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
+// Replays `enqueue` (the launches of one step, all on `st`) through a graph
+// captured once, or launches it directly when graphs are off.
+struct StepRunner {
+  hipStream_t st;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  bool graph;
+  template <typename F>
+  StepRunner(hipStream_t s, F&& enqueue) : st(s), graph(graphs_enabled()) {
+    if (!graph) {
+      direct = [enqueue]() { enqueue(); };
+      return;
+    }
+    EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    EDL_HIP(hipStreamEndCapture(st, &g));
+    EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  }
+  void launch() {
+    if (graph)
+      EDL_HIP(hipGraphLaunch(ge, st));
+    else
+      direct();
+  }
+  ~StepRunner() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  std::function<void()> direct;
+};
+
 template <typename T>
 static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, hipStream_t st,
                      const TwoPassStep<T>* tp = nullptr) {
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  if (tp && (tp->seg || tp->nx > 0)) {
-    // phase B of the previous step (+ deflation), beta, up (segmented, or the
-    // column-panel pass with no phase B pending), down (+ phase A), alpha
-    T* q0 = static_cast<T*>(cs.h[0].q0);
-    T* q1 = static_cast<T*>(cs.h[0].q1);
-    launch_phase_b<T>(tp->V, q0, q1, tp->L, st);
-    if (tp->nx > 0) launch_deflate<T>(tp->V, q0, q1, tp->L, tp->X, tp->nx, tp->defl_part, tp->defl_coef, st);
-    launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
-    if (tp->seg)
-      launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
-    else
+  auto enqueue = [&]() {
+    if (tp && (tp->seg || tp->nx > 0)) {
+      // phase B of the previous step (+ deflation), beta, up (segmented, or the
+      // column-panel pass with no phase B pending), down (+ phase A), alpha
+      T* q0 = static_cast<T*>(cs.h[0].q0);
+      T* q1 = static_cast<T*>(cs.h[0].q1);
+      launch_phase_b<T>(tp->V, q0, q1, tp->L, st);
+      if (tp->nx > 0) launch_deflate<T>(tp->V, q0, q1, tp->L, tp->X, tp->nx, tp->defl_part, tp->defl_coef, st);
+      launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
+      if (tp->seg)
+        launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
+      else
+        launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
+      launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
+      launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
+      launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
+    } else if (tp) {
+      // up (+ phase B of the previous step), beta, down (+ phase A), alpha
+      T* q0 = static_cast<T*>(cs.h[0].q0);
+      T* q1 = static_cast<T*>(cs.h[0].q1);
       launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
-    launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
-    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
-    launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
-  } else if (tp) {
-    // up (+ phase B of the previous step), beta, down (+ phase A), alpha
-    T* q0 = static_cast<T*>(cs.h[0].q0);
-    T* q1 = static_cast<T*>(cs.h[0].q1);
-    launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
-    launch_lz_beta(tp->L, tp->V, tp->cfg, st);
-    launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
-    launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
-    launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
-  } else {
-    lz_phase_a(c, cs, st);
-    lz_phase_b(c, cs, st);
-  }
-  EDL_HIP(hipStreamEndCapture(st, &g));
-  EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      launch_lz_beta(tp->L, tp->V, tp->cfg, st);
+      launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
+      launch_pass_dw<T>(tp->V, tp->lut_dw, q0, q1, tp->tmp, nullptr, &tp->L, tp->cfg, st);
+      launch_lz_alpha(tp->L, tp->V, tp->cfg, st);
+    } else {
+      lz_phase_a(c, cs, st);
+      lz_phase_b(c, cs, st);
+    }
+  };
+  StepRunner run(st, enqueue);
   double e_prev = 0.0;
   bool have_prev = false;
   int done = 0;
   for (int k = 0; k < steps; ++k) {
-    EDL_HIP(hipGraphLaunch(ge, st));
+    run.launch();
     ++done;
     if (check > 0 && tol > 0.0 && (k + 1) % check == 0 && cs.h.size() == 1) {
       LzScalars s{};
@@ -905,8 +943,6 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
     }
   }
   EDL_HIP(hipStreamSynchronize(st));
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   return done;
 }
 
@@ -929,17 +965,12 @@ static constexpr int64_t kChainAloneDim = 50000000;
 // phase B), down pass + phase A; captured once as a graph and replayed.
 template <typename T>
 static int run_steps_bt(Ctx& c, const BtSet& bs, int steps, hipStream_t st) {
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  bt_launch_up<T>(bs, c.model.inter, st);
-  bt_launch_dw<T>(bs, st);
-  EDL_HIP(hipStreamEndCapture(st, &g));
-  EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  for (int k = 0; k < steps; ++k) EDL_HIP(hipGraphLaunch(ge, st));
+  StepRunner run(st, [&]() {
+    bt_launch_up<T>(bs, c.model.inter, st);
+    bt_launch_dw<T>(bs, st);
+  });
+  for (int k = 0; k < steps; ++k) run.launch();
   EDL_HIP(hipStreamSynchronize(st));
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   return steps;
 }
 

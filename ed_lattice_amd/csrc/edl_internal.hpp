@@ -77,6 +77,14 @@ struct ColorTable {
   int32_t* off = nullptr;      // [nblocks]
   int32_t* ns = nullptr;       // [nblocks]
   int64_t slots = 0;           // sum of ns (table size / 64 words)
+  // The same slots two to a word (strings with one hopping magnitude h and
+  // dim + 32 <= 2^15): slot 2p in the low half of pair p, 2p+1 in the high
+  // half, each half the LDS row (15 bits) and the sign of h (bit 15).
+  // Unused slots read one of 32 zero rows staged after the last string (the
+  // one in the bank the slot leaves free).  Null where not built.
+  uint32_t* words16 = nullptr;
+  int32_t* off16 = nullptr;    // [nblocks] first word of block b
+  int32_t* ns16 = nullptr;     // [nblocks] pairs of block b (ceil(ns / 2))
 };
 
 // Column ranges of the down spin for the range-blocked down pass: the

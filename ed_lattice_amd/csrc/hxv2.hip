@@ -79,12 +79,12 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
 
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false>
 static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
                                                 typename Vec<T, C>::type* P, const T* __restrict__ q,
                                                 T* __restrict__ tout, int c0, T ts) {
@@ -113,7 +113,7 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
     }
   }
   __syncthreads();
-  up_rows<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, tout, c0, ts);
+  up_rows<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, tout, c0, ts);
 }
 
 // The row loop of a panel already staged in P: RPT rows per thread, hop
@@ -125,7 +125,10 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
 // scheduled table (ColorTable, width W = C * sizeof(T) / 4): in every slot
 // the 32 lanes of a half-wave gather from distinct LDS banks; every row of a
 // 64-row block runs the block's slot count (unused slots: coefficient 0).
-template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL>
+// H16 (with COL and LREG, fp32): two slots to a table word, a half being the
+// LDS row and the sign of the coefficient; unused slots gather a zero row
+// (rows dimup .. dimup+31 of P hold zeros).
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts) {
@@ -155,7 +158,11 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       row[h] = i0 + h * (int)blockDim.x;
       okr[h] = row[h] < dimup;
       if (!okr[h]) row[h] = i0;
-      if (COL) {
+      if (COL && H16) {
+        const int b = row[h] >> 6;
+        nh[h] = okr[h] ? V.pkh_ns[W][b] : 0;
+        tb[h] = V.pkh_up[W] + V.pkh_off[W][b] + (row[h] & 63);
+      } else if (COL) {
         const int b = row[h] >> 6;
         nh[h] = okr[h] ? V.pkc_ns[W][b] : 0;
         tb[h] = V.pkc_up[W] + V.pkc_off[W][b] + (row[h] & 63);
@@ -171,17 +178,32 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       for (int j = 0; j < C; ++j)
         acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
     }
+    const uint32_t zpad = (uint32_t)dimup | ((uint32_t)dimup << 16);
     for (int k0 = 0; k0 < nmax; k0 += UKCH) {
       uint32_t w[RPT][UKCH];
 #pragma unroll
       for (int h = 0; h < RPT; ++h)
 #pragma unroll
         for (int j = 0; j < UKCH; ++j)
-          w[h][j] = (k0 + j < nh[h]) ? tb[h][(size_t)(k0 + j) * tstride] : (255u << 24);
+          w[h][j] = (k0 + j < nh[h]) ? tb[h][(size_t)(k0 + j) * tstride] : (H16 ? zpad : (255u << 24));
 #pragma unroll
       for (int h = 0; h < RPT; ++h)
 #pragma unroll
         for (int j = 0; j < UKCH; ++j) {
+          if constexpr (H16) {
+            static_assert(sizeof(T) == 4 && COL && LREG, "two hops to a word: fp32 bank-scheduled table only");
+            const uint32_t l0b = __float_as_uint(l0);
+            const uint32_t wl = w[h][j];
+            const T cl = __uint_as_float(l0b ^ ((wl << 16) & 0x80000000u));
+            const T chi = __uint_as_float(l0b ^ (wl & 0x80000000u));
+            const VT xl = P[wl & 0x7FFFu];
+            const VT xh = P[(wl >> 16) & 0x7FFFu];
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[h][c] += cl * Vec<T, C>::get(xl, c);
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[h][c] += chi * Vec<T, C>::get(xh, c);
+            continue;
+          }
           const uint32_t cid = w[h][j] >> 24;
           const T cf = LREG ? (cid == 0u ? l0 : (cid == 1u ? l1 : (T)0)) : lut[cid];
           const VT x = P[w[h][j] & 0xFFFFFFu];
@@ -258,7 +280,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = false>
+template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = false, bool H16 = false>
 __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
@@ -267,10 +289,17 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   VT* P = reinterpret_cast<VT*>(smem + lds_head_bytes(sizeof(T)));
   __shared__ double sh[16];
   const int npan = (V.dimdw + C - 1) / C;
+  if (H16 && threadIdx.x < 32) {
+    // the zero rows that unused slots gather (the staging below never writes them)
+    VT z;
+#pragma unroll
+    for (int j = 0; j < C; ++j) Vec<T, C>::set(z, j, (T)0);
+    P[V.dimup + threadIdx.x] = z;
+  }
   if (!chain) {
     for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
-      up_panel<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, qa, tout, pan * C, (T)1);
+      up_panel<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
   }
   const LzScalars s = *L.sc;
@@ -292,7 +321,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     __syncthreads();
     nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx, bst);
     __syncthreads();
-    if (!last) up_rows<T, C, RPT, UKCH, LREG, COL>(V, I, lut, P, tout, pan * C, (T)1);
+    if (!last) up_rows<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, tout, pan * C, (T)1);
   }
   if (pend && !last) {
     const double v = block_sum(nrm, sh);
@@ -919,12 +948,50 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
     }
   }
   ct.slots = (int64_t)words.size() / 64;
+  // Two slots to a word where the spin has one hopping magnitude (cids 0
+  // and 1 only: lut[1] = -lut[0]) and the rows plus 32 zero rows fit 15
+  // bits.  A slot's padding (cid 255) becomes the zero row in the bank of
+  // its old padding address, so the bank schedule is kept.
+  std::vector<uint32_t> w16;
+  std::vector<int32_t> off16(nb), ns16(nb);
+  const bool h16 = t.nvals == 1 && dim + 32 <= (1 << 15);
+  if (h16) {
+    auto half = [&](uint32_t w) -> uint32_t {
+      const uint32_t cid = w >> 24, r = w & 0xFFFFFFu;
+      if (cid == 255u) {
+        const int bank = (int)(((int64_t)W * r) % 32);
+        for (int j = 0; j < 32; ++j)
+          if ((int)(((int64_t)W * (dim + j)) % 32) == bank) return (uint32_t)(dim + j);
+        return (uint32_t)dim;
+      }
+      return r | ((cid & 1u) << 15);
+    };
+    for (int b = 0; b < nb; ++b) {
+      ns16[b] = (ns[b] + 1) / 2;
+      off16[b] = (int32_t)w16.size();
+      for (int p = 0; p < ns16[b]; ++p)
+        for (int l = 0; l < 64; ++l) {
+          const uint32_t* blk = words.data() + off[b];
+          const uint32_t lo = half(blk[(size_t)(2 * p) * 64 + l]);
+          const uint32_t hi = 2 * p + 1 < ns[b] ? half(blk[(size_t)(2 * p + 1) * 64 + l]) : (uint32_t)dim;
+          w16.push_back(lo | (hi << 16));
+        }
+    }
+  }
   EDL_HIP(hipMalloc(&ct.words, sizeof(uint32_t) * std::max<size_t>(words.size(), 1)));
   EDL_HIP(hipMalloc(&ct.off, sizeof(int32_t) * nb));
   EDL_HIP(hipMalloc(&ct.ns, sizeof(int32_t) * nb));
   EDL_HIP(hipMemcpyAsync(ct.words, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice, st));
   EDL_HIP(hipMemcpyAsync(ct.off, off.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
   EDL_HIP(hipMemcpyAsync(ct.ns, ns.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+  if (h16) {
+    EDL_HIP(hipMalloc(&ct.words16, sizeof(uint32_t) * std::max<size_t>(w16.size(), 1)));
+    EDL_HIP(hipMalloc(&ct.off16, sizeof(int32_t) * nb));
+    EDL_HIP(hipMalloc(&ct.ns16, sizeof(int32_t) * nb));
+    EDL_HIP(hipMemcpyAsync(ct.words16, w16.data(), sizeof(uint32_t) * w16.size(), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(ct.off16, off16.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(ct.ns16, ns16.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+  }
   EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
 }
 
@@ -1030,6 +1097,9 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
     v.pkc_up[W] = s.sp[0].pkc[W].words;
     v.pkc_off[W] = s.sp[0].pkc[W].off;
     v.pkc_ns[W] = s.sp[0].pkc[W].ns;
+    v.pkh_up[W] = s.sp[0].pkc[W].words16;
+    v.pkh_off[W] = s.sp[0].pkc[W].off16;
+    v.pkh_ns[W] = s.sp[0].pkc[W].ns16;
   }
   for (int b = 0; b <= kMaxRangeBits; ++b) {
     const RangeTable& rt = s.sp[1].rng[b];
@@ -1044,27 +1114,36 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   return v;
 }
 
-template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL>
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16 = false>
 static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
-  const size_t lds = up_lds_bytes(V.dimup, C, sizeof(T));
-  set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG, COL>, lds_limit_bytes() - 256);
-  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG, COL>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb,
-                     tout, L ? *L : LzChain{}, L ? 1 : 0);
+  const size_t lds = up_lds_bytes(V.dimup + (H16 ? 32 : 0), C, sizeof(T));
+  EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_pass_up: panel exceeds LDS");
+  set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG, COL, H16>, lds_limit_bytes() - 256);
+  hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG, COL, H16>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa,
+                     qb, tout, L ? *L : LzChain{}, L ? 1 : 0);
   EDL_HIP(hipGetLastError());
 }
 
-bool up_col_ok(const TwoPassView& V, int C, size_t elem) {
+bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
   const int W = C * (int)elem / 4;
-  return elem == 4 && (C == 1 || C == 3) && V.pkc_up[W] != nullptr;
+  if (elem != 4 || (C != 1 && C != 3)) return false;
+  if (col == 2)
+    return V.pkh_up[W] != nullptr && V.nv_up == 1 &&
+           up_lds_bytes(V.dimup + 32, C, elem) <= (size_t)lds_limit_bytes() - 256;
+  return col == 1 && V.pkc_up[W] != nullptr;
 }
 
 template <typename T, int C, int RPT, int UKCH>
 static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                        const LzChain* L, int grid, int threads, bool col, hipStream_t st) {
+                        const LzChain* L, int grid, int threads, int col, hipStream_t st) {
   // fp32 only: in fp64 the two coefficient registers cost more than the
   // LDS reads they save (measured slower)
   if constexpr (sizeof(T) == 4 && (C == 1 || C == 3)) {
+    if (col == 2) {
+      EDL_CHECK(up_col_ok(V, C, sizeof(T), 2), "launch_pass_up: no two-hop bank-scheduled table for this sector");
+      return launch_up_l<T, C, RPT, UKCH, true, true, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+    }
     if (col) {
       EDL_CHECK(up_col_ok(V, C, sizeof(T)), "launch_pass_up: no bank-scheduled table for this panel width");
       if (V.nv_up == 1) return launch_up_l<T, C, RPT, UKCH, true, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
@@ -1082,7 +1161,7 @@ template <typename T, int C>
 static void launch_up_c(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, const TwoPassCfg& cfg, hipStream_t st) {
   const int th = cfg.threads_up;
-  const bool col = cfg.up_col != 0;
+  const int col = cfg.up_col;
   switch (cfg.up_rpt * 100 + cfg.up_kch) {
     case 108: return launch_up_k<T, C, 1, 8>(V, I, lut, qa, qb, tout, L, grid, th, col, st);
     case 116: return launch_up_k<T, C, 1, 16>(V, I, lut, qa, qb, tout, L, grid, th, col, st);

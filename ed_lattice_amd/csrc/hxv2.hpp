@@ -22,6 +22,10 @@ struct TwoPassView {
   const uint32_t* pkc_up[4];
   const int32_t* pkc_off[4];
   const int32_t* pkc_ns[4];
+  // the same two slots to a word (ColorTable::words16), or null
+  const uint32_t* pkh_up[4];
+  const int32_t* pkh_off[4];
+  const int32_t* pkh_ns[4];
   // down-spin column ranges by top-site count b (SpinTable::rng), or null
   const int32_t* rng_lo[kMaxRangeBits + 1];
   const uint32_t* rng_words[kMaxRangeBits + 1];
@@ -62,7 +66,8 @@ struct TwoPassCfg {
   int grid_up = 256;
   int up_rpt = 2;       // rows per thread in the up pass (1 or 2)
   int up_kch = 8;       // hop chunk of the up pass ((1, 8|16) or (2, 4|8))
-  int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3)
+  int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3);
+                        // 2: its form with two hops to a word (one hopping magnitude)
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
@@ -104,8 +109,9 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
 }
 
 // Whether the bank-scheduled up-pass table exists for panels of C columns of
-// T (staged rows of an odd number of dwords: fp32 with C = 1 or 3).
-bool up_col_ok(const TwoPassView& V, int C, size_t elem);
+// T (staged rows of an odd number of dwords: fp32 with C = 1 or 3), in the
+// form `col` (1: a hop to a word, 2: two hops to a word).
+bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col = 1);
 // Builds the bank-scheduled up-spin tables of a two-pass sector (once).
 void build_up_color_tables(Sector& s, hipStream_t st);
 // Builds the down-spin column-range tables of a two-pass sector (once).

@@ -305,6 +305,9 @@ void free_sector(Sector& s) {
       if (ct.words) (void)hipFree(ct.words);
       if (ct.off) (void)hipFree(ct.off);
       if (ct.ns) (void)hipFree(ct.ns);
+      if (ct.words16) (void)hipFree(ct.words16);
+      if (ct.off16) (void)hipFree(ct.off16);
+      if (ct.ns16) (void)hipFree(ct.ns16);
     }
     for (auto& rt : t.rng) {
       if (rt.lo) (void)hipFree(rt.lo);

@@ -241,20 +241,22 @@ def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, monkeypatch):
         assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
 
 
-@pytest.mark.parametrize("rpt,kch,R,dn,dk,vec,col", [
-    (1, 8, 64, 2, 8, 0, 0), (2, 4, 32, 1, 16, 0, 0), (2, 4, 64, 2, 4, 1, 0), (1, 8, 64, 1, 2, 1, 0),
-    (1, 16, 32, 2, 1, 1, 0), (1, 8, 16, 2, 2, 1, 0), (2, 8, 128, 1, 4, 1, 0), (2, 4, 256, 2, 2, 1, 0),
-    (1, 8, 128, 2, 2, 1, 1), (2, 4, 64, 1, 4, 1, 1)])
-def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, monkeypatch):
+@pytest.mark.parametrize("rpt,kch,R,dn,dk,vec,col,C", [
+    (1, 8, 64, 2, 8, 0, 0, 3), (2, 4, 32, 1, 16, 0, 0, 3), (2, 4, 64, 2, 4, 1, 0, 3), (1, 8, 64, 1, 2, 1, 0, 3),
+    (1, 16, 32, 2, 1, 1, 0, 3), (1, 8, 16, 2, 2, 1, 0, 3), (2, 8, 128, 1, 4, 1, 0, 3), (2, 4, 256, 2, 2, 1, 0, 3),
+    (1, 8, 128, 2, 2, 1, 1, 3), (2, 4, 64, 1, 4, 1, 1, 3),
+    (1, 8, 128, 2, 2, 1, 2, 3), (2, 4, 64, 1, 4, 1, 2, 3), (1, 16, 64, 2, 8, 0, 2, 1), (2, 8, 32, 1, 16, 0, 2, 1)])
+def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, C, monkeypatch):
     """Schedules pinned through the override, among them the vector-gather
     down pass (16-byte lanes, panels of 16 to 256 rows; the 8-site ring's
     DimUp 70 leaves a quad that straddles DimUp) and the bank-scheduled up
-    pass (col = 1, fp32; its last 64-row block is partial on every sector
-    here): H x v on three sectors and a mixed-precision ground state (the
-    chain-mode passes) against the oracle."""
+    pass (col = 1, fp32, and col = 2, its form with two hops to a table word
+    and zero rows for the unused slots; the last 64-row block is partial on
+    every sector here): H x v on three sectors and a mixed-precision ground
+    state (the chain-mode passes) against the oracle."""
     tail = f"{R},256,32,{dn},{dk},{rpt},{kch},{vec}"
     tail64 = f"{min(R, 128)},256,32,{dn},{dk},{rpt},{kch},{vec}"  # 256-row panels are fp32 only
-    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", "3,1024,256," + tail + f",{col}")
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"{C},1024,256," + tail + f",{col}")
     monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", "1,512,512," + tail64)
     for cfg, sec in [("ring8", (4, 4)), ("rect3x4", (7, 6)), ("rect3x4", (6, 5))]:
         c = lattice.CONFIGS[cfg]
@@ -653,6 +655,28 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, w
             g = _ritz(raw["alpha"][sidx], raw["beta"][sidx], raw["norm2"][sidx], e0, isign)
             _assert_converged_poles_match(g, _ritz(ao, bo, s @ s, e0, isign), tol, window=window,
                                           conv_tol=conv_tol)
+
+
+@pytest.mark.parametrize("route", ["tiles", "single", "chain"])
+def test_direct_launches_equal_graph_replay(ed, orc, route, monkeypatch):
+    """EDL_GRAPHS=0 launches each step's kernels directly instead of
+    replaying the captured graph: the same kernels on the same data give
+    bit-identical alpha, beta and ground-state energy, on every step runner
+    (batched tiles, single-pass chains, one chain through the two-pass
+    step)."""
+    monkeypatch.setenv("EDL_BATCHED_KERNELS", route)
+    _use(ed, orc, lattice.square(3, 4), [6.0])
+    isec = ed.sector_index(6, 6)
+    seeds = [("cdg", 1, 1, 0), ("c", 1, 2, 5), ("cdg", 2, 3, 0), ("c", 2, 4, 0)]
+    res = {}
+    for g in ("1", "0"):
+        monkeypatch.setenv("EDL_GRAPHS", g)
+        r = ed.lanczos_eigh(isec, "fp64", niter=120, want_vector=False)
+        rb = ed.tridiag_batched(seeds, 60, "fp64")
+        res[g] = (r["e0"], rb["alpha"], rb["beta"], rb["norm2"])
+    assert res["0"][0] == res["1"][0]
+    for a, b in zip(res["0"][1:], res["1"][1:]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
 
 
 @pytest.mark.parametrize("form", ["tiles", "persist"])
