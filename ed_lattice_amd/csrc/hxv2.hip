@@ -137,12 +137,18 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   const size_t tstride = COL ? 64 : (size_t)dimup;
-  uint32_t sd[C];
+  uint32_t sd[C], sdm[C];
   double dgd[C];
+  // one orbital and no inter-orbital pairs (the Hubbard lattices): the
+  // up x down interaction is U * popcount(su & sd & mask), with the column's
+  // part of the AND taken once per panel
+  const bool one_orb = I.norb == 1 && I.npairs == 0;
+  const double u0 = I.U[0];
 #pragma unroll
   for (int j = 0; j < C; ++j) {
     const int c = (c0 + j < dimdw) ? c0 + j : c0;
     sd[j] = (uint32_t)V.map_dw[c];
+    sdm[j] = sd[j] & I.orb_mask[0];
     dgd[j] = V.dg_dw[c];
   }
   const T l0 = lut[0], l1 = lut[1];
@@ -159,9 +165,14 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       okr[h] = row[h] < dimup;
       if (!okr[h]) row[h] = i0;
       if (COL && H16) {
-        const int b = row[h] >> 6;
-        nh[h] = okr[h] ? V.pkh_ns[W][b] : 0;
-        tb[h] = V.pkh_up[W] + V.pkh_off[W][b] + (row[h] & 63);
+        // the 64 rows of a wave are one block of the table (blockDim is a
+        // multiple of 64), so its slot count is wave-uniform: the hop loop
+        // bounds stay scalar.  Rows past DimUp inside the last block read
+        // its zero-row padding (their sums are never stored).
+        const int b = __builtin_amdgcn_readfirstlane(i0 + h * (int)blockDim.x) >> 6;
+        const bool inb = b < ((dimup + 63) >> 6);
+        nh[h] = inb ? V.pkh_ns[W][b] : 0;
+        tb[h] = V.pkh_up[W] + (inb ? V.pkh_off[W][b] : 0) + ((i0 + h * (int)blockDim.x) & 63);
       } else if (COL) {
         const int b = row[h] >> 6;
         nh[h] = okr[h] ? V.pkc_ns[W][b] : 0;
@@ -174,9 +185,15 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       const uint32_t su = (uint32_t)V.map_up[row[h]];
       const double dgu = V.dg_up[row[h]];
       const VT self = P[row[h]];
+      if (one_orb) {
 #pragma unroll
-      for (int j = 0; j < C; ++j)
-        acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+        for (int j = 0; j < C; ++j)
+          acc[h][j] = (T)(dgu + dgd[j] + u0 * (double)__popc(su & sdm[j])) * Vec<T, C>::get(self, j);
+      } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j)
+          acc[h][j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+      }
     }
     const uint32_t zpad = (uint32_t)dimup | ((uint32_t)dimup << 16);
     for (int k0 = 0; k0 < nmax; k0 += UKCH) {
@@ -285,8 +302,11 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   typedef typename Vec<T, C>::type VT;
   extern __shared__ __align__(16) unsigned char smem[];
-  T* lut = reinterpret_cast<T*>(smem);
-  VT* P = reinterpret_cast<VT*>(smem + lds_head_bytes(sizeof(T)));
+  // H16 takes its two coefficients from registers: no LDS table, and the
+  // panel starts at the base of the dynamic LDS (small gather offsets fold
+  // into the LDS instructions)
+  T* lut = H16 ? const_cast<T*>(lut_up) : reinterpret_cast<T*>(smem);
+  VT* P = reinterpret_cast<VT*>(smem + (H16 ? 0 : lds_head_bytes(sizeof(T))));
   __shared__ double sh[16];
   const int npan = (V.dimdw + C - 1) / C;
   if (H16 && threadIdx.x < 32) {
@@ -297,7 +317,8 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     P[V.dimup + threadIdx.x] = z;
   }
   if (!chain) {
-    for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+    if (!H16)
+      for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
       up_panel<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
@@ -307,7 +328,8 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
   const bool pend = s.pad != 0;
   const bool last = pend && s.k + 1 >= L.nmax;
   if (last && !L.x) return;
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+  if (!H16)
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
   T* qn = ((pend ? s.k + 1 : s.k) & 1) ? qb : qa;
   const T* qk = (s.k & 1) ? qb : qa;
   const T am = (T)(s.alpha_k / s.s_cur);
@@ -1117,7 +1139,7 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
 template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16 = false>
 static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, hipStream_t st) {
-  const size_t lds = up_lds_bytes(V.dimup + (H16 ? 32 : 0), C, sizeof(T));
+  const size_t lds = H16 ? (size_t)(V.dimup + 32) * C * sizeof(T) : up_lds_bytes(V.dimup, C, sizeof(T));
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_pass_up: panel exceeds LDS");
   set_lds_limit((const void*)k_pass_up<T, C, RPT, UKCH, LREG, COL, H16>, lds_limit_bytes() - 256);
   hipLaunchKernelGGL((k_pass_up<T, C, RPT, UKCH, LREG, COL, H16>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa,
@@ -1396,12 +1418,15 @@ __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, c
     }
   }
   __syncthreads();
-  uint32_t sd[C];
+  uint32_t sd[C], sdm[C];
   double dgd[C];
+  const bool one_orb = I.norb == 1 && I.npairs == 0;  // U * popcount(su & sd & mask) only
+  const double u0 = I.U[0];
 #pragma unroll
   for (int j = 0; j < C; ++j) {
     const int cc = (c0 + j < dimdw) ? c0 + j : c0;
     sd[j] = (uint32_t)V.map_dw[cc];
+    sdm[j] = sd[j] & I.orb_mask[0];
     dgd[j] = V.dg_dw[cc];
   }
   for (int row = lo + threadIdx.x; row < hi; row += blockDim.x) {
@@ -1410,8 +1435,14 @@ __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, c
     const double dgu = V.dg_up[row];
     const VT self = P[row - lo];
     T acc[C];
+    if (one_orb) {
 #pragma unroll
-    for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+      for (int j = 0; j < C; ++j)
+        acc[j] = (T)(dgu + dgd[j] + u0 * (double)__popc(su & sdm[j])) * Vec<T, C>::get(self, j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+    }
     for (int k0 = 0; k0 < nh; k0 += UKCH) {
       uint32_t w[UKCH];
 #pragma unroll
