@@ -375,6 +375,10 @@ def main():
             "max_abs_ddocc_mixed_vs_fp64": float(abs(omx["docc"] - o64["docc"]).max()),
             "max_abs_ddens_mixed_vs_fp64": float(abs(omx["dens"] - o64["dens"]).max()),
         }
+    # the pinned schedules are the 4x4 sector's: the other workloads' sectors
+    # (other shapes, down to the 8-ring's 70 columns) tune their own
+    for env in ("EDL_TWO_PASS_CFG_F32", "EDL_TWO_PASS_CFG_F64"):
+        os.environ.pop(env, None)
     if not args.no_seeds:
         out["gf_seeds"] = seeds_benchmark(ed)
     if not args.no_large:
