@@ -41,7 +41,9 @@ __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chai
   const int pan = (int)(t - tile0[c]);
   const int c0 = pan * C;
   const T* lsrc = reinterpret_cast<const T*>(B.lut_up);
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  // the coefficient table is requested before the panel and written to LDS
+  // after it (its load completes under the panel's)
+  const T lv = threadIdx.x < 256 ? lsrc[threadIdx.x] : (T)0;
   // Phase B of step k pending: r = t - alpha_k q_k / s_k, stored over t (the
   // buffer that becomes q_{k+1}); this panel of r is then the up-pass input.
   // Otherwise (first launch) the input is q_0.
@@ -49,8 +51,9 @@ __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chai
   T* __restrict__ qn = reinterpret_cast<T*>(((pend ? s.k + 1 : s.k) & 1) ? B.q1 : B.q0);
   const T* __restrict__ qk = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
   const T am = (T)(s.alpha_k / s.s_cur);
-  __syncthreads();
   const double nrm = stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
+  if (threadIdx.x < 256) lut[threadIdx.x] = lv;
+  for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
   __syncthreads();
   up_rows<T, C, RPT, UKCH>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
   if (!pend) return;
@@ -138,18 +141,40 @@ __global__ __launch_bounds__(256) void k_bt_alpha(const BtChain* __restrict__ ch
 // / s_k - beta_k q_{k-1} / s_{k-1} over q_{k-1}; returns this thread's share
 // of <t, q_k> (times s_k).  Both the tile kernel and the persistent kernel
 // run it.
-template <typename T, int RS, int NC>
+// Stage the slab of rows r0 .. r0+RS-1 of every column of q in S[dimdw][RS],
+// SB elements per thread at a time with every load of a batch issued before
+// its LDS writes (one memory round trip per batch).
+template <typename T, int RS, int SB = 8>
+static __device__ __forceinline__ void bt_stage_slab(const TwoPassView& V, T* S, const T* __restrict__ q, int r0) {
+  const int n = V.dimdw * RS;
+  const size_t ld = (size_t)V.ld;
+  for (int base = 0; base < n; base += SB * (int)blockDim.x) {
+    T a[SB];
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int i = base + h * (int)blockDim.x + (int)threadIdx.x;
+      const int j = i / RS, r = i % RS;
+      a[h] = (i < n && r0 + r < V.dimup) ? q[(size_t)j * ld + r0 + r] : (T)0;
+    }
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int i = base + h * (int)blockDim.x + (int)threadIdx.x;
+      if (i < n) S[i] = a[h];
+    }
+  }
+}
+
+// STAGE = false: the caller staged S (bt_stage_slab) and synchronised.
+template <typename T, int RS, int NC, bool STAGE = true>
 static __device__ __forceinline__ double bt_dw_slab(const TwoPassView& V, const T* lut, T* S, const int* cntS,
                                                     const T* __restrict__ q, T* __restrict__ qo,
                                                     const T* __restrict__ tb, int r0, T ts, T bo) {
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < dimdw * RS; i += blockDim.x) {
-    const int j = i / RS, r = i % RS;
-    S[i] = (r0 + r < dimup) ? q[(size_t)j * ld + r0 + r] : (T)0;
+  if (STAGE) {
+    bt_stage_slab<T, RS>(V, S, q, r0);
+    __syncthreads();
   }
-  __syncthreads();
   // lane = row of the slab (fastest) x column: the global accesses of a wave
   // are 64 / RS whole slab segments
   const int r = threadIdx.x % RS, jl = threadIdx.x / RS, jstep = blockDim.x / RS;
@@ -221,12 +246,20 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
   const T* __restrict__ tb = reinterpret_cast<const T*>(B.tb);
   const T* lsrc = reinterpret_cast<const T*>(B.lut_dw);
   const int sl = (int)(t - tile0[c]);
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
   int* cntS = reinterpret_cast<int*>(S + (size_t)V.dimdw * RS);  // [dimdw] hop counts
-  for (int j = threadIdx.x; j < V.dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
+  // the coefficient table, the hop counts and the slab are requested
+  // together and written to LDS after (one memory round trip, not three)
+  const T lv = threadIdx.x < 256 ? lsrc[threadIdx.x] : (T)0;
+  const int cv = (int)threadIdx.x < V.dimdw ? V.cnt_dw[threadIdx.x] : 0;
+  bt_stage_slab<T, RS>(V, S, q, sl * RS);
+  if (threadIdx.x < 256) lut[threadIdx.x] = lv;
+  for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  if ((int)threadIdx.x < V.dimdw) cntS[threadIdx.x] = cv;
+  for (int j = threadIdx.x + blockDim.x; j < V.dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
+  __syncthreads();
   const double inv_s = 1.0 / s.s_cur;
   const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
-  const double part = bt_dw_slab<T, RS, NC>(V, lut, S, cntS, q, qo, tb, sl * RS, ts, bo);
+  const double part = bt_dw_slab<T, RS, NC, false>(V, lut, S, cntS, q, qo, tb, sl * RS, ts, bo);
   const double v = block_sum(part * inv_s, sh);
   if (threadIdx.x == 0) B.partD[sl] = v;
 }
