@@ -868,9 +868,20 @@ struct StepRunner {
       return;
     }
     EDL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueue();
+    try {
+      enqueue();
+    } catch (...) {
+      // leave the stream out of capture mode before the error propagates
+      hipGraph_t part = nullptr;
+      (void)hipStreamEndCapture(st, &part);
+      if (part) (void)hipGraphDestroy(part);
+      throw;
+    }
     EDL_HIP(hipStreamEndCapture(st, &g));
-    EDL_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      throw Error("hipGraphInstantiate failed for the Lanczos step");
+    }
   }
   void launch() {
     if (graph)
@@ -1206,14 +1217,18 @@ extern "C" int edl_ctx_create(edl_ctx** out, int device) {
     EDL_CHECK(n > 0, "edl_ctx_create: no HIP device visible");
     EDL_CHECK(device >= 0 && device < n, "edl_ctx_create: device index out of range");
     EDL_HIP(hipSetDevice(device));
-    auto* ctx = new edl_ctx();
-    ctx->c.device = device;
-    EDL_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     hipDeviceProp_t prop{};
     EDL_HIP(hipGetDeviceProperties(&prop, device));
-    ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     init_binomials();
     noalloc_scratch_setup(device);
+    // the stream is the last resource taken: nothing after it can throw
+    auto* ctx = new edl_ctx();
+    ctx->c.device = device;
+    ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      throw Error("edl_ctx_create: hipStreamCreateWithFlags failed");
+    }
     *out = ctx;
     return 0;
   } catch (const std::exception& e) {
