@@ -985,6 +985,63 @@ static int run_steps_bt(Ctx& c, const BtSet& bs, int steps, hipStream_t st) {
   return steps;
 }
 
+// Chain groups of at least this many chains run as two halves on two
+// streams (the halves' steps overlap: one half's down tiles share the GPU
+// with the other half's up tiles); EDL_BATCHED_STREAMS=1|2 forces one form.
+static constexpr size_t kBtTwoStreamChains = 64;
+static int bt_streams(size_t nchains) {
+  if (const char* e = std::getenv("EDL_BATCHED_STREAMS")) {
+    const int v = std::atoi(e);
+    EDL_CHECK(v == 1 || v == 2, "EDL_BATCHED_STREAMS must be 1 or 2");
+    return nchains >= 2 ? v : 1;
+  }
+  return nchains >= kBtTwoStreamChains ? 2 : 1;
+}
+
+// A stream of the default priority (the library's other stream included)
+// and an event, released on scope exit.
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t e = nullptr;
+  AuxStream() {
+    EDL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamDestroy(s);
+      throw Error("hipEventCreateWithFlags failed");
+    }
+  }
+  ~AuxStream() {
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(s);
+  }
+};
+
+// `steps` steps of two independent chain sets, each on its own stream and
+// replayed from its own graph, launched alternately; `st` waits for both.
+template <typename T>
+static int run_steps_bt2(Ctx& c, const BtSet& a, const BtSet& b, int steps, hipStream_t st) {
+  AuxStream aux;
+  EDL_HIP(hipEventRecord(aux.e, st));  // the chains' set-up on st precedes both halves
+  EDL_HIP(hipStreamWaitEvent(aux.s, aux.e, 0));
+  StepRunner ra(st, [&]() {
+    bt_launch_up<T>(a, c.model.inter, st);
+    bt_launch_dw<T>(a, st);
+  });
+  StepRunner rb(aux.s, [&]() {
+    bt_launch_up<T>(b, c.model.inter, aux.s);
+    bt_launch_dw<T>(b, aux.s);
+  });
+  for (int k = 0; k < steps; ++k) {
+    ra.launch();
+    rb.launch();
+  }
+  EDL_HIP(hipEventRecord(aux.e, aux.s));
+  EDL_HIP(hipStreamWaitEvent(st, aux.e, 0));
+  EDL_HIP(hipStreamSynchronize(st));
+  return steps;
+}
+
 template <typename T>
 static ChainOut read_chain(const ChainDev& cd, int nmax, hipStream_t st) {
   ChainOut o;
@@ -2080,21 +2137,35 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   ChainSet<T> cs;
   cs.h = chains;
   BtShape shp{};
-  BtSet bs;
+  BtSet bs, half[2];
   bool persist = false;
+  int nstreams = 1;
   if (bt) {
     shp = bt_tune<T>(c, btchains, max_up, max_dw, st);
     BtShape pshp{};
     persist = bt_pick_persist<T>(c, btchains, shp, max_up, max_dw, st, &pshp);
     if (persist) shp = pshp;
-    bs.shape = shp;
-    bs.h = bt_resized(btchains, shp);
-    bt_finalize<T>(bs);
+    nstreams = persist ? 1 : bt_streams(btchains.size());
+    if (nstreams == 1) {
+      bs.shape = shp;
+      bs.h = bt_resized(btchains, shp);
+      bt_finalize<T>(bs);
+    } else {
+      const std::vector<BtChain> all = bt_resized(btchains, shp);
+      const size_t m = all.size() / 2;
+      for (int h = 0; h < 2; ++h) {
+        half[h].shape = shp;
+        half[h].h.assign(all.begin() + (h ? m : 0), h ? all.end() : all.begin() + m);
+        bt_finalize<T>(half[h]);
+      }
+    }
   }
   EventTimer run_tm(st);
   if (bt && persist) {
     bt_launch_persist<T>(bs, c.model.inter, st);
     EDL_HIP(hipStreamSynchronize(st));
+  } else if (bt && nstreams == 2) {
+    run_steps_bt2<T>(c, half[0], half[1], steps, st);
   } else if (bt) {
     run_steps_bt<T>(c, bs, steps, st);
   } else if (alone) {
@@ -2135,13 +2206,26 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   }
   if (verbose()) {
     const double ms = run_tm.stop();
-    std::fprintf(stderr, "[edl] batched group: %zu chains, %s kernels (C %d, RS %d), %d steps in %.3f ms (%.1f us/step)\n",
+    std::fprintf(stderr, "[edl] batched group: %zu chains, %s kernels (C %d, RS %d, %d stream%s), %d steps in %.3f ms (%.1f us/step)\n",
                  chains.size(), bt ? (persist ? "persistent two-pass" : "two-pass tile") : (alone ? "one-chain two-pass" : "single-pass"), bt ? shp.C : 0,
-                 bt ? shp.RS : 0, steps, ms, 1e3 * ms / steps);
+                 bt ? shp.RS : 0, nstreams, nstreams > 1 ? "s" : "", steps, ms, 1e3 * ms / steps);
   }
+  // every chain's scalars and coefficients requested, then one wait
+  std::vector<ChainOut> outs(chains.size());
+  std::vector<LzScalars> scs(chains.size());
+  for (size_t k = 0; k < chains.size(); ++k) {
+    outs[k].alpha.assign(nmaxs[k], 0.0);
+    outs[k].beta.assign(nmaxs[k], 0.0);
+    EDL_HIP(hipMemcpyAsync(&scs[k], cs.h[k].sc, sizeof(LzScalars), hipMemcpyDeviceToHost, st));
+    EDL_HIP(hipMemcpyAsync(outs[k].alpha.data(), cs.h[k].alpha, sizeof(double) * nmaxs[k], hipMemcpyDeviceToHost, st));
+    EDL_HIP(hipMemcpyAsync(outs[k].beta.data(), cs.h[k].beta, sizeof(double) * nmaxs[k], hipMemcpyDeviceToHost, st));
+  }
+  EDL_HIP(hipStreamSynchronize(st));
   int64_t hxv = 0;
   for (size_t k = 0; k < chains.size(); ++k) {
-    ChainOut o = read_chain<T>(cs.h[k], nmaxs[k], st);
+    ChainOut& o = outs[k];
+    o.nsteps = scs[k].nsteps;
+    o.norm2 = scs[k].norm2;
     const int s = which[k];
     for (int j = 0; j < nlanc; ++j) {
       alpha[(size_t)s * nlanc + j] = j < nmaxs[k] ? o.alpha[j] : 0.0;

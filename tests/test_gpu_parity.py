@@ -657,6 +657,26 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, w
                                           conv_tol=conv_tol)
 
 
+def test_two_stream_halves_equal_one_stream(ed, orc, monkeypatch):
+    """The batched tile path splits a chain group over two streams (each
+    half replayed from its own graph, the halves' launches interleaved):
+    bit-identical alpha, beta and norm2 to one stream, on both spins of the
+    3x4 seeds (two sector shapes in each half)."""
+    monkeypatch.setenv("EDL_BATCHED_KERNELS", "tiles")
+    _use(ed, orc, lattice.square(3, 4), [6.0])
+    ed.lanczos_eigh(ed.sector_index(6, 6), "fp64", niter=200, want_vector=False)
+    seeds = [(op, sp, i, j) for sp in (1, 2) for i, j in ((1, 0), (2, 7), (5, 0), (3, 12)) for op in ("cdg", "c")]
+    res = {}
+    for n in ("1", "2"):
+        monkeypatch.setenv("EDL_BATCHED_STREAMS", n)
+        for prec in ("fp64", "fp32"):
+            rb = ed.tridiag_batched(seeds, 40, prec)
+            res[(n, prec)] = (rb["alpha"], rb["beta"], rb["norm2"])
+    for prec in ("fp64", "fp32"):
+        for a, b in zip(res[("1", prec)], res[("2", prec)]):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
 @pytest.mark.parametrize("route", ["tiles", "single", "chain"])
 def test_direct_launches_equal_graph_replay(ed, orc, route, monkeypatch):
     """EDL_GRAPHS=0 launches each step's kernels directly instead of
