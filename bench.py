@@ -257,8 +257,8 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
         parts = (("pass_up", up_ms, 2 * dim * elem,
                   f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}, "
                   f"{'true' if sched['up_coef_registers'] else 'false'}, "
-                  f"{'true' if sched['up_bank_scheduled'] else 'false'}, "
-                  f"{'true' if sched['up_bank_scheduled'] == 2 else 'false'}>"),
+                  f"{'true' if sched['up_bank_scheduled'] in (1, 2) else 'false'}, "
+                  f"{'true' if sched['up_bank_scheduled'] in (2, 3) else 'false'}>"),
                  ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
     else:
         parts = (("single_pass", up_ms, 2 * dim * elem, f"k_hxv<{t}>"),)

@@ -207,7 +207,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               "EDL_TWO_PASS_CFG_F32/_F64: dw_rng needs range tables for that many top sites and R 32|64 within LDS");
     EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T), p.up_col),
               "EDL_TWO_PASS_CFG_F32/_F64: up_col 1 needs fp32 panels of 1 or 3 columns on a sector with the "
-              "bank-scheduled table, up_col 2 also one hopping magnitude and DimUp + 32 < 2^15");
+              "bank-scheduled table, up_col 2 also one hopping magnitude and DimUp + 32 < 2^15, up_col 3 "
+              "one hopping magnitude and DimUp + 32 < 2^15");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
                       "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
@@ -269,12 +270,12 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     best.grid_up = c.num_cu;
     best.nchunk = std::min(32, V.dimdw);
     double best_up = 1e300, best_dw = 1e300;
-    for (int col : {0, 1, 2})
+    for (int col : {0, 1, 2, 3})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
       for (int th : {512, 1024})
         for (int gm : {1, 2}) {
-          const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col == 2 ? 32 : 0)) * C * sizeof(T);
+          const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col >= 2 ? 32 : 0)) * C * sizeof(T);
           if (need > (size_t)lds) continue;
           if (gm == 2 && 2 * need > (size_t)lds) continue;
           if (col && !up_col_ok(V, C, sizeof(T), col)) continue;
@@ -466,12 +467,12 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     DevBuf pu(sizeof(double) * (size_t)c.num_cu * 2);
     A.part_up = pu.as<double>();
     double bu2 = 1e300;
-    for (int col : {0, 1, 2})
+    for (int col : {0, 1, 2, 3})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
         for (int th : {512, 1024})
           for (int gm : {1, 2}) {
-            const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col == 2 ? 32 : 0)) * C * sizeof(T);
+            const size_t need = 256 * sizeof(T) + (size_t)(V.dimup + (col >= 2 ? 32 : 0)) * C * sizeof(T);
             if (need > (size_t)lds || (gm == 2 && 2 * need > (size_t)lds)) continue;
             if (col && !up_col_ok(V, C, sizeof(T), col)) continue;
             TwoPassCfg w = best;
@@ -2468,8 +2469,9 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   for (auto& e : ev) (void)hipEventDestroy(e);
   ms_pass[0] = up / reps;
   ms_pass[1] = (two || seg) ? dw / reps : 0.0;
-  // the up pass takes its coefficients from registers in fp32 on one-value lattices
-  const int32_t lreg = (two && sizeof(T) == 4 && s.sp[0].nvals == 1) ? 1 : 0;
+  // the up pass takes its coefficients from registers in fp32 on one-value
+  // lattices, and with a two-hop table (up_col 2 or 3) in either precision
+  const int32_t lreg = (two && s.sp[0].nvals == 1 && (sizeof(T) == 4 || cf.up_col >= 2)) ? 1 : 0;
   const int32_t out[16] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
                            cf.dw_vec,                lreg,      two ? cf.up_col : 0,          cf.dw_rng};

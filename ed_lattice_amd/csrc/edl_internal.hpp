@@ -133,6 +133,11 @@ struct SpinTable {
   // Bank-scheduled copies of the packed table for the up pass, one per LDS
   // element width W (dwords per staged row: 1 or 3), built with the sector.
   ColorTable pkc[4];
+  // The packed table two hops to a word in table order, [pair][dim]: hops
+  // 2p and 2p+1 of a string in the low / high half (LDS row in 15 bits, the
+  // sign of h in bit 15; a missing second hop reads zero row dim).  Built
+  // for spins with one hopping magnitude and dim + 32 <= 2^15, else null.
+  uint32_t* pk16 = nullptr;
   // Column ranges by top-site count b (the down spin's, for the
   // range-blocked down pass), built with the sector for b = 1 .. 8 when
   // the ranges are at least a few columns wide.

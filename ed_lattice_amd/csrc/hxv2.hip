@@ -75,6 +75,12 @@ static __device__ __forceinline__ double inter2(const InterParams& I, uint32_t s
   return e;
 }
 
+// h with its sign bit XOR-ed with bit 31 of `sbit` (the other bits are 0).
+static __device__ __forceinline__ float flip_sign(float h, uint32_t sbit) { return __uint_as_float(__float_as_uint(h) ^ sbit); }
+static __device__ __forceinline__ double flip_sign(double h, uint32_t sbit) {
+  return __longlong_as_double(__double_as_longlong(h) ^ ((long long)sbit << 32));
+}
+
 // ------------------------------------------------------------------ pass U
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
@@ -177,6 +183,10 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
         const int b = row[h] >> 6;
         nh[h] = okr[h] ? V.pkc_ns[W][b] : 0;
         tb[h] = V.pkc_up[W] + V.pkc_off[W][b] + (row[h] & 63);
+      } else if (H16) {
+        // the row's hops two to a word, table order ([pair][row])
+        nh[h] = okr[h] ? (V.cnt_up[row[h]] + 1) >> 1 : 0;
+        tb[h] = V.pk16_up + row[h];
       } else {
         nh[h] = okr[h] ? V.cnt_up[row[h]] : 0;
         tb[h] = V.pk_up + row[h];
@@ -208,11 +218,10 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
 #pragma unroll
         for (int j = 0; j < UKCH; ++j) {
           if constexpr (H16) {
-            static_assert(sizeof(T) == 4 && COL && LREG, "two hops to a word: fp32 bank-scheduled table only");
-            const uint32_t l0b = __float_as_uint(l0);
+            static_assert(LREG && (!COL || sizeof(T) == 4 || C == 1), "two hops to a word: register coefficients");
             const uint32_t wl = w[h][j];
-            const T cl = __uint_as_float(l0b ^ ((wl << 16) & 0x80000000u));
-            const T chi = __uint_as_float(l0b ^ (wl & 0x80000000u));
+            const T cl = flip_sign(l0, (wl << 16) & 0x80000000u);
+            const T chi = flip_sign(l0, wl & 0x80000000u);
             const VT xl = P[wl & 0x7FFFu];
             const VT xh = P[(wl >> 16) & 0x7FFFu];
 #pragma unroll
@@ -904,6 +913,12 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
   ColorTable& ct = t.pkc[W];
   if (ct.words || !t.packed_ok || t.dim < 1) return;
   const int dim = t.dim, kmax = t.kmax;
+  // Lanes served together and the bank classes they must spread over: a
+  // row of W dwords, W odd: 32 lanes over the 32 single-dword banks (class
+  // W * row mod 32); W = 2: 16 lanes over 16 bank pairs (class row mod 16),
+  // a 64-bit read serving 16 lanes of 8 bytes at a time.
+  const int L = (W % 2) ? 32 : 16, G = 64 / L;
+  auto cls = [&](int64_t r) { return (int)((W % 2) ? (W * r) % 32 : r % 16); };
   std::vector<uint32_t> pk((size_t)std::max(kmax, 1) * dim);
   std::vector<int32_t> cnt(dim);
   EDL_HIP(hipMemcpyAsync(pk.data(), t.packed, sizeof(uint32_t) * pk.size(), hipMemcpyDeviceToHost, st));
@@ -915,57 +930,56 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
   std::vector<int> el, ek, color;
   std::vector<uint32_t> ew;
   for (int b = 0; b < nb; ++b) {
-    int nsh[2] = {0, 0};
-    std::vector<int> col[2];
-    std::vector<uint32_t> wds[2];
-    std::vector<int> lanes[2];
-    for (int h = 0; h < 2; ++h) {
+    std::vector<int> nsh(G, 0);
+    std::vector<std::vector<int>> col(G), lanes(G);
+    std::vector<std::vector<uint32_t>> wds(G);
+    for (int h = 0; h < G; ++h) {
       el.clear();
       ek.clear();
       ew.clear();
-      for (int l = 0; l < 32; ++l) {
-        const int r = b * 64 + h * 32 + l;
+      for (int l = 0; l < L; ++l) {
+        const int r = b * 64 + h * L + l;
         if (r >= dim) break;
         for (int k = 0; k < cnt[r]; ++k) {
           const uint32_t w = pk[(size_t)k * dim + r];
           el.push_back(l);
-          ek.push_back((int)(((int64_t)W * (w & 0xFFFFFFu)) % 32));
+          ek.push_back(cls(w & 0xFFFFFFu));
           ew.push_back(w);
         }
       }
-      nsh[h] = color_bipartite(el, ek, 32, 32, color);
+      nsh[h] = color_bipartite(el, ek, L, L, color);
       col[h] = color;
       wds[h] = ew;
       lanes[h] = el;
     }
-    const int nsb = std::max(nsh[0], nsh[1]);
+    const int nsb = *std::max_element(nsh.begin(), nsh.end());
     off[b] = (int32_t)words.size();
     ns[b] = nsb;
     EDL_CHECK(words.size() + (size_t)nsb * 64 < ((size_t)1 << 31), "build_color_table: table exceeds 2^31 words");
     words.resize(words.size() + (size_t)nsb * 64, 0u);
     uint32_t* blk = words.data() + off[b];
-    for (int h = 0; h < 2; ++h) {
-      std::vector<char> used((size_t)nsb * 32, 0);
-      std::vector<char> filled((size_t)nsb * 32, 0);
+    for (int h = 0; h < G; ++h) {
+      std::vector<char> used((size_t)nsb * L, 0);
+      std::vector<char> filled((size_t)nsb * L, 0);
       for (size_t e = 0; e < wds[h].size(); ++e) {
         const int s = col[h][e], l = lanes[h][e];
-        blk[(size_t)s * 64 + h * 32 + l] = wds[h][e];
-        filled[(size_t)s * 32 + l] = 1;
-        used[(size_t)s * 32 + (int)(((int64_t)W * (wds[h][e] & 0xFFFFFFu)) % 32)] = 1;
+        blk[(size_t)s * 64 + h * L + l] = wds[h][e];
+        filled[(size_t)s * L + l] = 1;
+        used[(size_t)s * L + cls(wds[h][e] & 0xFFFFFFu)] = 1;
       }
       for (int s = 0; s < nsb; ++s) {
-        // padding: cid 255 at one address of a bank this slot leaves free
-        int freebank = 0;
-        while (freebank < 32 && used[(size_t)s * 32 + freebank]) ++freebank;
+        // padding: cid 255 at one address of a class this slot leaves free
+        int freecls = 0;
+        while (freecls < L && used[(size_t)s * L + freecls]) ++freecls;
         uint32_t padrow = 0;
-        if (freebank < 32)
+        if (freecls < L)
           for (int r = 0; r < std::min(dim, 32 * W + 32); ++r)
-            if ((W * r) % 32 == freebank) {
+            if (cls(r) == freecls) {
               padrow = (uint32_t)r;
               break;
             }
-        for (int l = 0; l < 32; ++l)
-          if (!filled[(size_t)s * 32 + l]) blk[(size_t)s * 64 + h * 32 + l] = padrow | (255u << 24);
+        for (int l = 0; l < L; ++l)
+          if (!filled[(size_t)s * L + l]) blk[(size_t)s * 64 + h * L + l] = padrow | (255u << 24);
       }
     }
   }
@@ -981,9 +995,9 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
     auto half = [&](uint32_t w) -> uint32_t {
       const uint32_t cid = w >> 24, r = w & 0xFFFFFFu;
       if (cid == 255u) {
-        const int bank = (int)(((int64_t)W * r) % 32);
+        const int c = cls(r);
         for (int j = 0; j < 32; ++j)
-          if ((int)(((int64_t)W * (dim + j)) % 32) == bank) return (uint32_t)(dim + j);
+          if (cls(dim + j) == c) return (uint32_t)(dim + j);
         return (uint32_t)dim;
       }
       return r | ((cid & 1u) << 15);
@@ -1017,9 +1031,35 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
   EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
 }
 
+// The packed table two hops to a word in table order (SpinTable::pk16).
+static void build_pair_table(SpinTable& t, hipStream_t st) {
+  if (t.pk16 || !t.packed_ok || t.nvals != 1 || t.dim < 1 || t.dim + 32 > (1 << 15)) return;
+  const int dim = t.dim, kmax = std::max(t.kmax, 1), kp = (kmax + 1) / 2;
+  std::vector<uint32_t> pk((size_t)kmax * dim);
+  std::vector<int32_t> cnt(dim);
+  EDL_HIP(hipMemcpyAsync(pk.data(), t.packed, sizeof(uint32_t) * pk.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipMemcpyAsync(cnt.data(), t.cnt, sizeof(int32_t) * dim, hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  const uint32_t zero = (uint32_t)dim;  // the first zero row after the staged strings
+  std::vector<uint32_t> w((size_t)kp * dim, zero | (zero << 16));
+  for (int a = 0; a < dim; ++a)
+    for (int k = 0; k < cnt[a]; ++k) {
+      const uint32_t x = pk[(size_t)k * dim + a];
+      const uint32_t half = (x & 0xFFFFFFu) | (((x >> 24) & 1u) << 15);
+      uint32_t& o = w[(size_t)(k / 2) * dim + a];
+      o = (k & 1) ? ((o & 0xFFFFu) | (half << 16)) : ((o & 0xFFFF0000u) | half);
+    }
+  EDL_HIP(hipMalloc(&t.pk16, sizeof(uint32_t) * w.size()));
+  EDL_HIP(hipMemcpyAsync(t.pk16, w.data(), sizeof(uint32_t) * w.size(), hipMemcpyHostToDevice, st));
+  EDL_HIP(hipStreamSynchronize(st));  // w must outlive the copy
+}
+
 void build_up_color_tables(Sector& s, hipStream_t st) {
-  if (!s.sp[0].packed_ok || s.sp[0].dim < 64) return;
+  if (!s.sp[0].packed_ok) return;
+  build_pair_table(s.sp[0], st);
+  if (s.sp[0].dim < 64) return;
   build_color_table(s.sp[0], 1, st);
+  build_color_table(s.sp[0], 2, st);
   build_color_table(s.sp[0], 3, st);
 }
 
@@ -1123,6 +1163,7 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
     v.pkh_off[W] = s.sp[0].pkc[W].off16;
     v.pkh_ns[W] = s.sp[0].pkc[W].ns16;
   }
+  v.pk16_up = s.sp[0].pk16;
   for (int b = 0; b <= kMaxRangeBits; ++b) {
     const RangeTable& rt = s.sp[1].rng[b];
     v.rng_lo[b] = rt.lo;
@@ -1149,6 +1190,11 @@ static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut
 
 bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
   const int W = C * (int)elem / 4;
+  if (col == 3)
+    return V.pk16_up != nullptr && (size_t)(V.dimup + 32) * C * elem <= (size_t)lds_limit_bytes() - 256;
+  if (elem == 8 && C == 1 && col == 2)  // 8-byte rows: 16 lanes over 16 bank pairs
+    return V.pkh_up[W] != nullptr && V.nv_up == 1 &&
+           up_lds_bytes(V.dimup + 32, C, elem) <= (size_t)lds_limit_bytes() - 256;
   if (elem != 4 || (C != 1 && C != 3)) return false;
   if (col == 2)
     return V.pkh_up[W] != nullptr && V.nv_up == 1 &&
@@ -1159,6 +1205,16 @@ bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
 template <typename T, int C, int RPT, int UKCH>
 static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, int col, hipStream_t st) {
+  if (col == 3) {
+    EDL_CHECK(up_col_ok(V, C, sizeof(T), 3), "launch_pass_up: no two-hop table for this sector");
+    return launch_up_l<T, C, RPT, UKCH, true, false, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+  }
+  if constexpr (sizeof(T) == 8 && C == 1) {
+    if (col == 2) {
+      EDL_CHECK(up_col_ok(V, C, sizeof(T), 2), "launch_pass_up: no two-hop bank-scheduled table for this sector");
+      return launch_up_l<T, C, RPT, UKCH, true, true, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+    }
+  }
   // fp32 only: in fp64 the two coefficient registers cost more than the
   // LDS reads they save (measured slower)
   if constexpr (sizeof(T) == 4 && (C == 1 || C == 3)) {

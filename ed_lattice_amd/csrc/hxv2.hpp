@@ -26,6 +26,8 @@ struct TwoPassView {
   const uint32_t* pkh_up[4];
   const int32_t* pkh_off[4];
   const int32_t* pkh_ns[4];
+  // the packed up-spin table two hops to a word in table order (SpinTable::pk16), or null
+  const uint32_t* pk16_up;
   // down-spin column ranges by top-site count b (SpinTable::rng), or null
   const int32_t* rng_lo[kMaxRangeBits + 1];
   const uint32_t* rng_words[kMaxRangeBits + 1];
@@ -67,7 +69,8 @@ struct TwoPassCfg {
   int up_rpt = 2;       // rows per thread in the up pass (1 or 2)
   int up_kch = 8;       // hop chunk of the up pass ((1, 8|16) or (2, 4|8))
   int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3);
-                        // 2: its form with two hops to a word (one hopping magnitude)
+                        // 2: its form with two hops to a word (one hopping magnitude);
+                        // 3: the table order two hops to a word (any precision and width)
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)
@@ -110,9 +113,11 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
 
 // Whether the bank-scheduled up-pass table exists for panels of C columns of
 // T (staged rows of an odd number of dwords: fp32 with C = 1 or 3), in the
-// form `col` (1: a hop to a word, 2: two hops to a word).
+// form `col` (1: a hop to a word, 2: two hops to a word); col 3: the
+// two-hop table in table order exists and C columns plus the zero rows fit.
 bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col = 1);
-// Builds the bank-scheduled up-spin tables of a two-pass sector (once).
+// Builds the bank-scheduled up-spin tables of a two-pass sector and the
+// two-hop table in table order (once).
 void build_up_color_tables(Sector& s, hipStream_t st);
 // Builds the down-spin column-range tables of a two-pass sector (once).
 void build_dw_range_tables(Sector& s, hipStream_t st);

@@ -309,6 +309,7 @@ void free_sector(Sector& s) {
       if (ct.off16) (void)hipFree(ct.off16);
       if (ct.ns16) (void)hipFree(ct.ns16);
     }
+    if (t.pk16) (void)hipFree(t.pk16);
     for (auto& rt : t.rng) {
       if (rt.lo) (void)hipFree(rt.lo);
       if (rt.words) (void)hipFree(rt.words);
