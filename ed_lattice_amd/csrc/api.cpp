@@ -1829,8 +1829,8 @@ static int target_sector(const Ctx& c, int isector, const SeedSpec& s) {
   return 1 + u * (Ns + 1) + w;
 }
 
-static std::map<std::tuple<int, int, int, int>, BtShape>& bt_cache() {
-  static std::map<std::tuple<int, int, int, int>, BtShape> m;
+static std::map<std::tuple<int, int, int, int, int>, BtShape>& bt_cache() {
+  static std::map<std::tuple<int, int, int, int, int>, BtShape> m;
   return m;
 }
 
@@ -1848,7 +1848,8 @@ static std::vector<BtChain> bt_resized(std::vector<BtChain> v, const BtShape& s)
 // pass multiplies q_{-1} by beta_0 = 0); the scalars are put back after.
 template <typename T>
 static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, int max_dw, hipStream_t st) {
-  const auto key = std::make_tuple((int)sizeof(T), max_up, max_dw, (int)chains.size());
+  const char* fk = std::getenv("EDL_BATCHED_UP_H16");
+  const auto key = std::make_tuple((int)sizeof(T), max_up, max_dw, (int)chains.size(), fk ? std::atoi(fk) : -1);
   auto it = bt_cache().find(key);
   if (it != bt_cache().end()) return it->second;
   BtShape best = bt_shape(max_up, max_dw, sizeof(T));
@@ -1888,51 +1889,65 @@ static BtShape bt_tune(Ctx& c, const std::vector<BtChain>& chains, int max_up, i
     return 0.0;
   };
   double bu = 1e300, bd = 1e300;
-  for (int C : {1, 2, 3, 4}) {
-    if (head + (size_t)C * max_up * sizeof(T) > budget) continue;
-    for (int rk : {208, 204, 108})
-      for (int th : {256, 512, 1024}) {
-        BtShape w = best;
-        w.C = C;
-        w.up_rpt = rk / 100;
-        w.up_kch = rk % 100;
-        w.threads_up = th;
-        const double ms = time(w, true);
-        if (ms < bu) {
-          bu = ms;
-          best.C = C;
-          best.up_rpt = w.up_rpt;
-          best.up_kch = w.up_kch;
-          best.threads_up = th;
+  // EDL_BATCHED_UP_H16=0|1 keeps the tuner to one table form in both tiles
+  const bool h16_ok = bt_h16_ok(chains, 0), h16_dw_ok = bt_h16_ok(chains, 1);
+  const char* fh = std::getenv("EDL_BATCHED_UP_H16");
+  const int h16_only = fh ? std::atoi(fh) : -1;
+  for (int h16 : {0, 1}) {
+    if ((h16 && !h16_ok) || (h16_only >= 0 && h16 != h16_only)) continue;
+    for (int C : {1, 2, 3, 4}) {
+      if (head + (size_t)C * (max_up + (h16 ? 32 : 0)) * sizeof(T) > budget) continue;
+      for (int rk : {208, 204, 108})
+        for (int th : {256, 512, 1024}) {
+          BtShape w = best;
+          w.C = C;
+          w.up_rpt = rk / 100;
+          w.up_kch = rk % 100;
+          w.threads_up = th;
+          w.up_h16 = h16;
+          const double ms = time(w, true);
+          if (ms < bu) {
+            bu = ms;
+            best.C = C;
+            best.up_rpt = w.up_rpt;
+            best.up_kch = w.up_kch;
+            best.threads_up = th;
+            best.up_h16 = h16;
+          }
         }
-      }
+    }
   }
-  for (int RS : {32, 16, 8, 4, 2, 1}) {
-    if ((size_t)RS * sizeof(T) > 128 || bt_dw_lds_bytes(max_dw, RS, sizeof(T)) > budget) continue;
-    for (int nc : {1, 2})
-      for (int th : {256, 512, 1024}) {
-        BtShape w = best;
-        w.RS = RS;
-        w.dw_nc = nc;
-        w.threads_dw = th;
-        const double ms = time(w, false);
-        if (ms < bd) {
-          bd = ms;
-          best.RS = RS;
-          best.dw_nc = nc;
-          best.threads_dw = th;
+  for (int h16 : {0, 1}) {
+    if ((h16 && !h16_dw_ok) || (h16_only >= 0 && h16 != h16_only)) continue;
+    for (int RS : {32, 16, 8, 4, 2, 1}) {
+      if ((size_t)RS * sizeof(T) > 128 || bt_dw_lds_bytes(max_dw, RS, sizeof(T), h16) > budget) continue;
+      for (int nc : {1, 2})
+        for (int th : {256, 512, 1024}) {
+          BtShape w = best;
+          w.RS = RS;
+          w.dw_nc = nc;
+          w.threads_dw = th;
+          w.dw_h16 = h16;
+          const double ms = time(w, false);
+          if (ms < bd) {
+            bd = ms;
+            best.RS = RS;
+            best.dw_nc = nc;
+            best.threads_dw = th;
+            best.dw_h16 = h16;
+          }
         }
-      }
+    }
   }
   for (size_t k = 0; k < chains.size(); ++k)
     EDL_HIP(hipMemcpyAsync(chains[k].sc, &snap[k], sizeof(LzScalars), hipMemcpyHostToDevice, st));
   EDL_HIP(hipStreamSynchronize(st));
   if (verbose())
     std::fprintf(stderr,
-                 "[edl] batched two-pass shape for %zu chains: C %d rpt %d kch %d x %d threads (%.3f ms), "
-                 "RS %d nc %d x %d (%.3f ms)\n",
-                 chains.size(), best.C, best.up_rpt, best.up_kch, best.threads_up, bu, best.RS, best.dw_nc,
-                 best.threads_dw, bd);
+                 "[edl] batched two-pass shape for %zu chains: C %d rpt %d kch %d%s x %d threads (%.3f ms), "
+                 "RS %d nc %d%s x %d (%.3f ms)\n",
+                 chains.size(), best.C, best.up_rpt, best.up_kch, best.up_h16 ? " two-hop" : "", best.threads_up, bu,
+                 best.RS, best.dw_nc, best.dw_h16 ? " two-hop" : "", best.threads_dw, bd);
   bt_cache()[key] = best;
   return best;
 }

@@ -22,7 +22,9 @@ static __device__ __forceinline__ int64_t xcd_tile(int64_t n8) {
   return (int64_t)(blockIdx.x % 8) * n8 + blockIdx.x / 8;
 }
 
-template <typename T, int C, int RPT, int UKCH>
+// H16: the chains' up-spin hops two to a word (SpinTable::pk16; one hopping
+// magnitude), ±h from a register, zero rows after the staged strings.
+template <typename T, int C, int RPT, int UKCH, bool H16 = false>
 __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
                                                 const int32_t* __restrict__ tchain, InterParams I, int64_t Tall,
                                                 int64_t n8) {
@@ -52,10 +54,19 @@ __global__ __launch_bounds__(1024) void k_bt_up(const BtChain* __restrict__ chai
   const T* __restrict__ qk = reinterpret_cast<const T*>((s.k & 1) ? B.q1 : B.q0);
   const T am = (T)(s.alpha_k / s.s_cur);
   const double nrm = stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, c0, pend, false, am, nullptr, 0.0);
-  if (threadIdx.x < 256) lut[threadIdx.x] = lv;
-  for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  if (H16) {
+    if (threadIdx.x < 32) {
+      VT z;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(z, j, (T)0);
+      P[V.dimup + threadIdx.x] = z;
+    }
+  } else {
+    if (threadIdx.x < 256) lut[threadIdx.x] = lv;
+    for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  }
   __syncthreads();
-  up_rows<T, C, RPT, UKCH>(V, I, lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
+  up_rows<T, C, RPT, UKCH, H16, false, H16>(V, I, H16 ? lsrc : lut, P, reinterpret_cast<T*>(B.tb), c0, (T)1);
   if (!pend) return;
   const double v = block_sum(nrm, sh);
   if (threadIdx.x == 0) B.partU[pan] = v;
@@ -165,7 +176,10 @@ static __device__ __forceinline__ void bt_stage_slab(const TwoPassView& V, T* S,
 }
 
 // STAGE = false: the caller staged S (bt_stage_slab) and synchronised.
-template <typename T, int RS, int NC, bool STAGE = true>
+// H16: the down spin's hops two to a word (SpinTable::pk16), +-h from a
+// register (lut = the global coefficient table), a missing second hop
+// gathering the zero column dimdw of S.
+template <typename T, int RS, int NC, bool STAGE = true, bool H16 = false>
 static __device__ __forceinline__ double bt_dw_slab(const TwoPassView& V, const T* lut, T* S, const int* cntS,
                                                     const T* __restrict__ q, T* __restrict__ qo,
                                                     const T* __restrict__ tb, int r0, T ts, T bo) {
@@ -195,24 +209,46 @@ static __device__ __forceinline__ double bt_dw_slab(const TwoPassView& V, const 
       col[h] = j0 + h * jstep;
       okc[h] = col[h] < dimdw;
       if (!okc[h]) col[h] = j0;
-      nh[h] = okc[h] ? cntS[col[h]] : 0;
+      nh[h] = okc[h] ? (H16 ? (cntS[col[h]] + 1) >> 1 : cntS[col[h]]) : 0;
       nmax = nh[h] > nmax ? nh[h] : nmax;
       ii[h] = (size_t)col[h] * ld + r0 + r;
       tin[h] = (rok && okc[h]) ? tb[ii[h]] : (T)0;
       qold[h] = (rok && okc[h]) ? qo[ii[h]] : (T)0;
       acc[h] = (T)0;
     }
-    for (int k0 = 0; k0 < nmax; k0 += KCH) {
-      uint32_t w[NC][KCH];
+    if constexpr (H16) {
+      constexpr int KW = KCH / 2;  // words (two hops each) per chunk
+      const T l0 = lut[0];
+      const uint32_t zpad = (uint32_t)dimdw | ((uint32_t)dimdw << 16);
+      for (int k0 = 0; k0 < nmax; k0 += KW) {
+        uint32_t w[NC][KW];
 #pragma unroll
-      for (int h = 0; h < NC; ++h)
+        for (int h = 0; h < NC; ++h)
 #pragma unroll
-        for (int jj = 0; jj < KCH; ++jj)
-          w[h][jj] = (k0 + jj < nh[h]) ? V.pk_dw[(size_t)(k0 + jj) * dimdw + col[h]] : (255u << 24);
+          for (int jj = 0; jj < KW; ++jj)
+            w[h][jj] = (k0 + jj < nh[h]) ? V.pk16_dw[(size_t)(k0 + jj) * dimdw + col[h]] : zpad;
 #pragma unroll
-      for (int h = 0; h < NC; ++h)
+        for (int h = 0; h < NC; ++h)
 #pragma unroll
-        for (int jj = 0; jj < KCH; ++jj) acc[h] += lut[w[h][jj] >> 24] * S[(int)(w[h][jj] & 0xFFFFFFu) * RS + r];
+          for (int jj = 0; jj < KW; ++jj) {
+            const uint32_t wl = w[h][jj];
+            acc[h] += flip_sign(l0, (wl << 16) & 0x80000000u) * S[(int)(wl & 0x7FFFu) * RS + r];
+            acc[h] += flip_sign(l0, wl & 0x80000000u) * S[(int)((wl >> 16) & 0x7FFFu) * RS + r];
+          }
+      }
+    } else {
+      for (int k0 = 0; k0 < nmax; k0 += KCH) {
+        uint32_t w[NC][KCH];
+#pragma unroll
+        for (int h = 0; h < NC; ++h)
+#pragma unroll
+          for (int jj = 0; jj < KCH; ++jj)
+            w[h][jj] = (k0 + jj < nh[h]) ? V.pk_dw[(size_t)(k0 + jj) * dimdw + col[h]] : (255u << 24);
+#pragma unroll
+        for (int h = 0; h < NC; ++h)
+#pragma unroll
+          for (int jj = 0; jj < KCH; ++jj) acc[h] += lut[w[h][jj] >> 24] * S[(int)(w[h][jj] & 0xFFFFFFu) * RS + r];
+      }
     }
     if (rok) {
 #pragma unroll
@@ -227,7 +263,7 @@ static __device__ __forceinline__ double bt_dw_slab(const TwoPassView& V, const 
   return part;
 }
 
-template <typename T, int RS, int NC>
+template <typename T, int RS, int NC, bool H16 = false>
 __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chains, const int64_t* __restrict__ tile0,
                                                 const int32_t* __restrict__ tchain, int64_t Tall, int64_t n8) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -246,20 +282,25 @@ __global__ __launch_bounds__(1024) void k_bt_dw(const BtChain* __restrict__ chai
   const T* __restrict__ tb = reinterpret_cast<const T*>(B.tb);
   const T* lsrc = reinterpret_cast<const T*>(B.lut_dw);
   const int sl = (int)(t - tile0[c]);
-  int* cntS = reinterpret_cast<int*>(S + (size_t)V.dimdw * RS);  // [dimdw] hop counts
+  // [dimdw] hop counts, after the slab (and, with H16, after its zero column)
+  int* cntS = reinterpret_cast<int*>(S + (size_t)(V.dimdw + (H16 ? 1 : 0)) * RS);
   // the coefficient table, the hop counts and the slab are requested
   // together and written to LDS after (one memory round trip, not three)
-  const T lv = threadIdx.x < 256 ? lsrc[threadIdx.x] : (T)0;
+  const T lv = (!H16 && threadIdx.x < 256) ? lsrc[threadIdx.x] : (T)0;
   const int cv = (int)threadIdx.x < V.dimdw ? V.cnt_dw[threadIdx.x] : 0;
   bt_stage_slab<T, RS>(V, S, q, sl * RS);
-  if (threadIdx.x < 256) lut[threadIdx.x] = lv;
-  for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  if (H16) {
+    if (threadIdx.x < RS) S[(size_t)V.dimdw * RS + threadIdx.x] = (T)0;
+  } else {
+    if (threadIdx.x < 256) lut[threadIdx.x] = lv;
+    for (int k = threadIdx.x + blockDim.x; k < 256; k += blockDim.x) lut[k] = lsrc[k];
+  }
   if ((int)threadIdx.x < V.dimdw) cntS[threadIdx.x] = cv;
   for (int j = threadIdx.x + blockDim.x; j < V.dimdw; j += blockDim.x) cntS[j] = V.cnt_dw[j];
   __syncthreads();
   const double inv_s = 1.0 / s.s_cur;
   const T ts = (T)inv_s, bo = (T)(s.beta_k / s.s_old);
-  const double part = bt_dw_slab<T, RS, NC, false>(V, lut, S, cntS, q, qo, tb, sl * RS, ts, bo);
+  const double part = bt_dw_slab<T, RS, NC, false, H16>(V, H16 ? lsrc : lut, S, cntS, q, qo, tb, sl * RS, ts, bo);
   const double v = block_sum(part * inv_s, sh);
   if (threadIdx.x == 0) B.partD[sl] = v;
 }
@@ -372,8 +413,8 @@ __global__ __launch_bounds__(1024) void k_bt_persist(const BtChain* __restrict__
 static constexpr size_t kBtUpBudget = 64 * 1024;   // two or more up tiles per CU
 static constexpr size_t kBtDwBudget = 150 * 1024;  // one slab per CU at most
 
-size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem) {
-  return lds_head_bytes(elem) + (size_t)RS * dimdw * elem + sizeof(int) * (size_t)dimdw;
+size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem, bool h16) {
+  return lds_head_bytes(elem) + (size_t)RS * (dimdw + (h16 ? 1 : 0)) * elem + sizeof(int) * (size_t)dimdw;
 }
 
 BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem) {
@@ -420,10 +461,22 @@ static void bt_attributes() {
                  (const void*)k_bt_up<T, 1, 2, 8>, (const void*)k_bt_up<T, 2, 2, 8>, (const void*)k_bt_up<T, 4, 2, 8>,
                  (const void*)k_bt_up<T, 1, 2, 4>, (const void*)k_bt_up<T, 2, 2, 4>, (const void*)k_bt_up<T, 4, 2, 4>,
                  (const void*)k_bt_up<T, 1, 1, 8>, (const void*)k_bt_up<T, 2, 1, 8>, (const void*)k_bt_up<T, 4, 1, 8>,
+                 (const void*)k_bt_up<T, 3, 2, 8, true>, (const void*)k_bt_up<T, 3, 2, 4, true>,
+                 (const void*)k_bt_up<T, 3, 1, 8, true>, (const void*)k_bt_up<T, 1, 2, 8, true>,
+                 (const void*)k_bt_up<T, 2, 2, 8, true>, (const void*)k_bt_up<T, 4, 2, 8, true>,
+                 (const void*)k_bt_up<T, 1, 2, 4, true>, (const void*)k_bt_up<T, 2, 2, 4, true>,
+                 (const void*)k_bt_up<T, 4, 2, 4, true>, (const void*)k_bt_up<T, 1, 1, 8, true>,
+                 (const void*)k_bt_up<T, 2, 1, 8, true>, (const void*)k_bt_up<T, 4, 1, 8, true>,
                  (const void*)k_bt_dw<T, 1, 1>, (const void*)k_bt_dw<T, 2, 1>, (const void*)k_bt_dw<T, 4, 1>,
                  (const void*)k_bt_dw<T, 8, 1>, (const void*)k_bt_dw<T, 16, 1>, (const void*)k_bt_dw<T, 32, 1>,
                  (const void*)k_bt_dw<T, 1, 2>, (const void*)k_bt_dw<T, 2, 2>, (const void*)k_bt_dw<T, 4, 2>,
-                 (const void*)k_bt_dw<T, 8, 2>, (const void*)k_bt_dw<T, 16, 2>, (const void*)k_bt_dw<T, 32, 2>})
+                 (const void*)k_bt_dw<T, 8, 2>, (const void*)k_bt_dw<T, 16, 2>, (const void*)k_bt_dw<T, 32, 2>,
+                 (const void*)k_bt_dw<T, 1, 1, true>, (const void*)k_bt_dw<T, 2, 1, true>,
+                 (const void*)k_bt_dw<T, 4, 1, true>, (const void*)k_bt_dw<T, 8, 1, true>,
+                 (const void*)k_bt_dw<T, 16, 1, true>, (const void*)k_bt_dw<T, 32, 1, true>,
+                 (const void*)k_bt_dw<T, 1, 2, true>, (const void*)k_bt_dw<T, 2, 2, true>,
+                 (const void*)k_bt_dw<T, 4, 2, true>, (const void*)k_bt_dw<T, 8, 2, true>,
+                 (const void*)k_bt_dw<T, 16, 2, true>, (const void*)k_bt_dw<T, 32, 2, true>})
     set_lds_limit(f, (int)kBtDwBudget);
 }
 
@@ -457,6 +510,12 @@ void bt_finalize(BtSet& bs) {
   EDL_HIP(hipMemcpy(bs.d_chain_dw, cdw.data(), sizeof(int32_t) * cdw.size(), hipMemcpyHostToDevice));
 }
 
+bool bt_h16_ok(const std::vector<BtChain>& chains, int spin) {
+  for (const BtChain& b : chains)
+    if (!(spin == 0 ? b.V.pk16_up : b.V.pk16_dw)) return false;
+  return !chains.empty();
+}
+
 static int max_dim(const BtSet& bs, int which) {
   int m = 1;
   for (const BtChain& b : bs.h) m = std::max(m, which == 0 ? b.V.dimup : b.V.dimdw);
@@ -465,11 +524,17 @@ static int max_dim(const BtSet& bs, int which) {
 
 template <typename T, int C, int RPT, int UKCH>
 static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st, bool scalars) {
-  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)C * max_dim(bs, 0) * sizeof(T);
+  const bool h16 = bs.shape.up_h16 != 0;
+  const size_t lds = lds_head_bytes(sizeof(T)) + (size_t)C * (max_dim(bs, 0) + (h16 ? 32 : 0)) * sizeof(T);
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_up: panel exceeds LDS");
+  EDL_CHECK(!h16 || bt_h16_ok(bs.h, 0), "bt_launch_up: a chain's sector has no two-hop table");
   const int64_t n8 = (bs.tiles_up + 7) / 8;
-  hipLaunchKernelGGL((k_bt_up<T, C, RPT, UKCH>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st, bs.d,
-                     bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
+  if (h16)
+    hipLaunchKernelGGL((k_bt_up<T, C, RPT, UKCH, true>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st,
+                       bs.d, bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
+  else
+    hipLaunchKernelGGL((k_bt_up<T, C, RPT, UKCH>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_up), lds, st,
+                       bs.d, bs.d_tile_up, bs.d_chain_up, I, bs.tiles_up, n8);
   EDL_HIP(hipGetLastError());
   if (!scalars) return;
   hipLaunchKernelGGL(k_bt_beta, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);
@@ -485,12 +550,18 @@ static void bt_up_k(const BtSet& bs, const InterParams& I, hipStream_t st, bool 
 
 template <typename T, int RS, int NC>
 static void bt_dw_r(const BtSet& bs, hipStream_t st, bool scalars) {
-  const size_t lds = bt_dw_lds_bytes(max_dim(bs, 1), RS, sizeof(T));
+  const bool h16 = bs.shape.dw_h16 != 0;
+  const size_t lds = bt_dw_lds_bytes(max_dim(bs, 1), RS, sizeof(T), h16);
   EDL_CHECK(lds <= kBtDwBudget, "bt_launch_dw: slab exceeds LDS");
   EDL_CHECK(bs.shape.threads_dw % RS == 0, "bt_launch_dw: block size not a multiple of the slab height");
+  EDL_CHECK(!h16 || bt_h16_ok(bs.h, 1), "bt_launch_dw: a chain's sector has no two-hop table");
   const int64_t n8 = (bs.tiles_dw + 7) / 8;
-  hipLaunchKernelGGL((k_bt_dw<T, RS, NC>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
-                     bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
+  if (h16)
+    hipLaunchKernelGGL((k_bt_dw<T, RS, NC, true>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st,
+                       bs.d, bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
+  else
+    hipLaunchKernelGGL((k_bt_dw<T, RS, NC>), dim3((unsigned)(8 * n8)), dim3(bs.shape.threads_dw), lds, st, bs.d,
+                       bs.d_tile_dw, bs.d_chain_dw, bs.tiles_dw, n8);
   EDL_HIP(hipGetLastError());
   if (!scalars) return;
   hipLaunchKernelGGL(k_bt_alpha, dim3((unsigned)bs.h.size()), dim3(256), 0, st, bs.d);

@@ -44,14 +44,17 @@ struct BtShape {
   int up_kch = 8;
   int dw_nc = 2;         // columns per thread in the down tile
   int threads_dw = 256;
+  int up_h16 = 0;        // 1: the up tiles read the hops two to a word (bt_h16_ok, spin 0)
+  int dw_h16 = 0;        // 1: the down tiles likewise (spin 1)
 };
 
 // Largest up panel width and down slab height that fit the sectors, or
 // RS = 0 when some sector does not fit (the caller then takes the
 // single-pass kernels).
 BtShape bt_shape(int max_dimup, int max_dimdw, size_t elem);
-// LDS bytes of a down tile: coefficient table, RS-row slab, hop counts.
-size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem);
+// LDS bytes of a down tile: coefficient table, RS-row slab (plus a zero
+// column with the two-hop table), hop counts.
+size_t bt_dw_lds_bytes(int dimdw, int RS, size_t elem, bool h16 = false);
 
 struct BtSet {
   std::vector<BtChain> h;
@@ -64,6 +67,10 @@ struct BtSet {
   BtShape shape;
   ~BtSet();
 };
+
+// Whether every chain's table of spin `spin` (0 up, 1 down) also exists two
+// hops to a word (one hopping magnitude, dim + 32 <= 2^15).
+bool bt_h16_ok(const std::vector<BtChain>& chains, int spin);
 
 template <typename T>
 void bt_finalize(BtSet& bs);

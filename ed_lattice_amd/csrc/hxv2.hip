@@ -1055,6 +1055,7 @@ static void build_pair_table(SpinTable& t, hipStream_t st) {
 }
 
 void build_up_color_tables(Sector& s, hipStream_t st) {
+  if (s.sp[1].packed_ok) build_pair_table(s.sp[1], st);  // the batched down tiles read it
   if (!s.sp[0].packed_ok) return;
   build_pair_table(s.sp[0], st);
   if (s.sp[0].dim < 64) return;
@@ -1164,6 +1165,7 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
     v.pkh_ns[W] = s.sp[0].pkc[W].ns16;
   }
   v.pk16_up = s.sp[0].pk16;
+  v.pk16_dw = s.sp[1].pk16;
   for (int b = 0; b <= kMaxRangeBits; ++b) {
     const RangeTable& rt = s.sp[1].rng[b];
     v.rng_lo[b] = rt.lo;

@@ -28,6 +28,7 @@ struct TwoPassView {
   const int32_t* pkh_ns[4];
   // the packed up-spin table two hops to a word in table order (SpinTable::pk16), or null
   const uint32_t* pk16_up;
+  const uint32_t* pk16_dw;  // the down spin's, likewise (read by the batched down tiles)
   // down-spin column ranges by top-site count b (SpinTable::rng), or null
   const int32_t* rng_lo[kMaxRangeBits + 1];
   const uint32_t* rng_words[kMaxRangeBits + 1];
@@ -117,7 +118,7 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
 // two-hop table in table order exists and C columns plus the zero rows fit.
 bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col = 1);
 // Builds the bank-scheduled up-spin tables of a two-pass sector and the
-// two-hop table in table order (once).
+// two-hop tables in table order of both spins (once).
 void build_up_color_tables(Sector& s, hipStream_t st);
 // Builds the down-spin column-range tables of a two-pass sector (once).
 void build_dw_range_tables(Sector& s, hipStream_t st);

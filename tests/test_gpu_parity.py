@@ -660,6 +660,31 @@ def test_gf_batched_ring8_vs_oracle(ed, orc, prec, tol, conv_tol, lead, nlead, w
                                           conv_tol=conv_tol)
 
 
+@pytest.mark.parametrize("h16", ["0", "1"])
+def test_batched_up_tile_table_forms_vs_oracle(ed, orc, h16, monkeypatch):
+    """The batched up tiles with the up-spin hops one to a word (LDS
+    coefficients) and two to a word (+-h from a register, zero rows for the
+    odd last hop): both spins of the 3x4 seeds against the oracle's chains."""
+    monkeypatch.setenv("EDL_BATCHED_KERNELS", "tiles")
+    monkeypatch.setenv("EDL_BATCHED_UP_H16", h16)
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    r = ed.lanczos_eigh(ed.sector_index(6, 6), "fp64", niter=300, want_vector=True)
+    seeds = [("cdg", 1, 1, 0), ("c", 1, 2, 7), ("cdg", 2, 3, 0), ("c", 2, 4, 9)]
+    # fp32 chains: the leading coefficients to single-precision rounding
+    # amplified over a few steps
+    for prec, nk, tol in (("fp64", 8, 1e-9), ("fp32", 5, 1e-3)):
+        rb = ed.tridiag_batched(seeds, 30, prec)
+        for k, (op, sp, io, jo) in enumerate(seeds):
+            opc = 2 if op == "cdg" else 1
+            d = 1 if opc == 2 else -1
+            nJ = (6 + d, 6) if sp == 1 else (6, 6 + d)
+            s = orc.seed_c_cdg(12, opc, io, sp, (6, 6), nJ, r["gs"]).real
+            if jo:
+                s = s + orc.seed_c_cdg(12, opc, jo, sp, (6, 6), nJ, r["gs"]).real
+            ao, bo, _ = orc.lanczos_tridiag(orc.hamiltonian_csr(m, *nJ), s, 30)
+            _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, nk, tol)
+
+
 def test_two_stream_halves_equal_one_stream(ed, orc, monkeypatch):
     """The batched tile path splits a chain group over two streams (each
     half replayed from its own graph, the halves' launches interleaved):
