@@ -251,7 +251,11 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
     t = "float" if elem == 4 else "double"
     out = {}
     if sched["two_pass"] == 2:  # segmented up pass (columns larger than LDS)
-        parts = (("pass_up", up_ms, 2 * dim * elem, f"k_up_seg<{t}, {sched['up_cols']}, 8>"),
+        # the schedule's up_bank_scheduled slot holds the segmented pass's
+        # hop-target form there: 1 = rank arithmetic, 0 = hop table
+        up = (f"k_up_segr<{t}, {sched['up_cols']}>" if sched["up_bank_scheduled"] == 1
+              else f"k_up_seg<{t}, {sched['up_cols']}, 8>")
+        parts = (("pass_up", up_ms, 2 * dim * elem, up),
                  ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
     elif sched["two_pass"]:
         parts = (("pass_up", up_ms, 2 * dim * elem,

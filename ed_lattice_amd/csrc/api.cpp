@@ -517,12 +517,35 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
 // narrower ones keep more hops inside a segment) and the down-pass shape,
 // each timed on two launches of the sector itself -- a few dozen launches in
 // all, against the hundreds of the two-pass search, since one launch here
-// moves gigabytes.  Cached per sector shape, precision and row cap.
+// moves gigabytes.  The up pass is timed with the hop table and, where the
+// sector has rank tables, with hop targets by rank arithmetic
+// (EDL_UP_SEG_RANK = 0 or 1 pins one of the two; 1 where the sector has rank
+// tables).  Cached per sector shape, precision, row cap and pin.
+static int seg_rank_pin() {
+  const char* e = std::getenv("EDL_UP_SEG_RANK");
+  if (!e || !*e) return -1;
+  const int v = std::atoi(e);
+  EDL_CHECK(v == 0 || v == 1, "EDL_UP_SEG_RANK must be 0 or 1");
+  return v;
+}
+// EDL_UP_SEG_COLS pins the up-pass panel width C of segmented sectors (1 to
+// 4, or 8, which runs with fp32 rank arithmetic and as 4 otherwise); 0 when
+// unset.
+static int seg_cols_pin() {
+  const char* e = std::getenv("EDL_UP_SEG_COLS");
+  if (!e || !*e) return 0;
+  const int v = std::atoi(e);
+  EDL_CHECK((v >= 1 && v <= 4) || v == 8, "EDL_UP_SEG_COLS must be 1, 2, 3, 4 or 8");
+  return v;
+}
+
 template <typename T>
 static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
-  static std::map<std::tuple<int, int, int, int, int, int>, TwoPassCfg> cache;
+  static std::map<std::tuple<int, int, int, int, int, int, int, int>, TwoPassCfg> cache;
+  const int pin = (seg_rank_pin() == 1 && !seg_rank_ok(s)) ? 0 : seg_rank_pin();
+  const int cpin = seg_cols_pin();
   const auto key = std::make_tuple(s.sp[0].dim, s.sp[1].dim, s.sp[0].kmax, s.sp[1].kmax, (int)sizeof(T),
-                                   seg_row_cap(sizeof(T), 1));
+                                   seg_row_cap(sizeof(T), 1), pin, cpin);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   hipStream_t st = c.stream;
@@ -533,19 +556,29 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
   launch_zero_bytes(v.p, bytes, st);
   TwoPassCfg best = seg_dw_cfg(s.sp[1].dim);
   best.C = 1;
+  best.seg_rank = pin == 1 ? 1 : 0;
   double bu = 1e300;
-  for (int C = 1; C <= 4; ++C) {
-    seg_bounds(s, sizeof(T), C);
-    double ms = 0.0;
-    for (int r = 0; r < 2; ++r) {
-      EventTimer tm(st);
-      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), t.as<T>(), nullptr, s, C, st);
-      ms = tm.stop();
-    }
-    if (verbose()) std::fprintf(stderr, "[edl] segmented up pass C %d: %zu segments, %.3f ms\n", C, s.seg_h.size() - 1, ms);
-    if (ms < bu) {
-      bu = ms;
-      best.C = C;
+  for (int rank = 0; rank <= 1; ++rank) {
+    if ((pin >= 0 && rank != pin) || (rank == 1 && !seg_rank_ok(s))) continue;
+    for (int C : {1, 2, 3, 4, 8}) {
+      if (C == 8 && (rank == 0 || sizeof(T) != 4)) continue;  // fp32 rank arithmetic only
+      if (cpin && C != ((cpin == 8 && (rank == 0 || sizeof(T) != 4)) ? 4 : cpin)) continue;  // 8: else 4
+      seg_bounds(s, sizeof(T), C);
+      double ms = 0.0;
+      for (int r = 0; r < 2; ++r) {
+        EventTimer tm(st);
+        launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), t.as<T>(), nullptr, s, C, rank,
+                         st);
+        ms = tm.stop();
+      }
+      if (verbose())
+        std::fprintf(stderr, "[edl] segmented up pass C %d rank %d: %zu segments, %.3f ms\n", C, rank,
+                     s.seg_h.size() - 1, ms);
+      if (ms < bu) {
+        bu = ms;
+        best.C = C;
+        best.seg_rank = rank;
+      }
     }
   }
   // {R, threads, chunks, vector gathers}: 4-byte gathers with 2 columns per
@@ -656,7 +689,7 @@ static void apply_h(Ctx& c, Sector& s, const T* v, T* out, int ld, hipStream_t s
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)ld * s.sp[1].dim));
     const TwoPassView V = two_pass_view(s, ld);
     T* vin = const_cast<T*>(v);  // operator mode only reads it
-    launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, cfg.C, st);
+    launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), vin, vin, tmp, nullptr, s, cfg.C, cfg.seg_rank, st);
     launch_nonlocal<T>(V, c.model.inter, vin, vin, tmp, nullptr, st);
     launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), vin, vin, tmp, out, nullptr, cfg, st);
   } else if (two_pass_ok(s, sizeof(T))) {
@@ -910,7 +943,8 @@ static int run_steps(Ctx& c, ChainSet<T>& cs, int steps, int check, double tol, 
       if (tp->nx > 0) launch_deflate<T>(tp->V, q0, q1, tp->L, tp->X, tp->nx, tp->defl_part, tp->defl_coef, st);
       launch_lz_beta_n(tp->L, phase_b_grid(tp->V), st);
       if (tp->seg)
-        launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C, st);
+        launch_up_seg<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, *tp->seg, tp->cfg.C,
+                         tp->cfg.seg_rank, st);
       else
         launch_pass_up<T>(tp->V, c.model.inter, tp->lut_up, q0, q1, tp->tmp, &tp->L, tp->cfg, st);
       launch_nonlocal<T>(tp->V, c.model.inter, q0, q1, tp->tmp, &tp->L, st);
@@ -2444,7 +2478,8 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
     T* tmp = static_cast<T*>(scratch(c, sizeof(T) * (size_t)Dp));
     for (int r = 0; r < reps; ++r) {
       EDL_HIP(hipEventRecord(ev[3 * r], st));
-      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, cf.C, st);
+      launch_up_seg<T>(V, c.model.inter, lut_dev<T>(s.sp[0]), v.as<T>(), v.as<T>(), tmp, nullptr, s, cf.C, cf.seg_rank,
+                       st);
       launch_nonlocal<T>(V, c.model.inter, v.as<T>(), v.as<T>(), tmp, nullptr, st);
       EDL_HIP(hipEventRecord(ev[3 * r + 1], st));
       launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), v.as<T>(), v.as<T>(), tmp, h.as<T>(), nullptr, cf, st);
@@ -2489,7 +2524,7 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   const int32_t lreg = (two && s.sp[0].nvals == 1 && (sizeof(T) == 4 || cf.up_col >= 2)) ? 1 : 0;
   const int32_t out[16] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
-                           cf.dw_vec,                lreg,      two ? cf.up_col : 0,          cf.dw_rng};
+                           cf.dw_vec,                lreg,      two ? cf.up_col : cf.seg_rank, cf.dw_rng};
   for (int k = 0; k < 16; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 

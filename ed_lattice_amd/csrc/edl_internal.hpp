@@ -104,6 +104,26 @@ struct RangeTable {
 };
 constexpr int kMaxRangeBits = 8;
 
+// Hops by rank arithmetic.  For strings of popcount N the rank splits at
+// bit B:  rank(s) = rlo[s & (2^B - 1)] + rhi[s >> B],  with
+//   rlo[L] = sum over the j-th set bit p of L   of C(p, j),
+//   rhi[H] = sum over the j-th set bit p of H   of C(B + p, N - popcount(H) + j),
+// so the target of a hop across bond {p, q} is two table reads from the
+// string itself, with no hop table.  Built for spins of at most 20 sites
+// (tables of at most 1024 words each), at most kMaxBonds bonds and a
+// symmetric hopping matrix (the same h gathered in both directions).
+constexpr int kMaxBonds = 48;
+constexpr int kRankHalfWords = 1024;
+struct RankHops {
+  bool ok = false;
+  int B = 0, nlo = 0, nhi = 0;   // split bit, table lengths 2^B and 2^(Ns-B)
+  int nb = 0;                    // bonds p < q with h = Hij(p,q) = Hij(q,p) != 0
+  uint8_t p[kMaxBonds] = {}, q[kMaxBonds] = {};
+  double h[kMaxBonds] = {};
+  uint32_t* rlo = nullptr;       // device tables
+  uint32_t* rhi = nullptr;
+};
+
 // One spin species of one sector: sorted strings, hopping table, diagonal.
 struct SpinTable {
   int N = 0;           // electrons of this spin
@@ -142,6 +162,7 @@ struct SpinTable {
   // range-blocked down pass), built with the sector for b = 1 .. 8 when
   // the ranges are at least a few columns wide.
   RangeTable rng[kMaxRangeBits + 1];
+  RankHops rk;
 };
 
 struct Sector {
@@ -154,6 +175,7 @@ struct Sector {
   std::vector<int> seg_h;
   int* d_seg = nullptr;
   int seg_cap = 0;
+  int seg_sh = 0;  // the segments are unions of runs of equal map >> seg_sh
   // The model has the non-local Kanamori block: every operator and chain of
   // this sector runs on the single-pass kernels, which carry that term.
   bool nonlocal = false;

@@ -53,6 +53,12 @@ struct Vec<T, 4> {
   static __device__ __forceinline__ T get(const type& v, int j) { return v[j]; }
   static __device__ __forceinline__ void set(type& v, int j, T x) { v[j] = x; }
 };
+template <typename T>
+struct Vec<T, 8> {  // fp32 only (32-byte rows): the widest panel of the rank-arithmetic segmented up pass
+  typedef T type __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ T get(const type& v, int j) { return v[j]; }
+  static __device__ __forceinline__ void set(type& v, int j, T x) { v[j] = x; }
+};
 
 // Rows per thread of a panel-staging batch: as many as about `regs` 32-bit
 // registers of loads hold (`per` loads per element), 2 to `cap`.  All loads
@@ -1527,6 +1533,204 @@ __global__ __launch_bounds__(1024) void k_up_seg(TwoPassView V, InterParams I, c
   }
 }
 
+// Segmented up pass with hop targets by rank arithmetic: the same tiles as
+// k_up_seg, but a string's neighbours come from its own bits instead of the
+// hop table, which a panel of C columns would otherwise read in full (on
+// the 3x6 open cluster the table is 5.3 MB, larger than an XCD's L2, and
+// its re-reads were most of the launch's traffic).  For bond {p, q} a row
+// string su hops iff exactly one of p, q is occupied; the other string is
+// su ^ (1 << p | 1 << q) (the hop is its own inverse, and h is the same in
+// both directions), its rank is rlo[..] + rhi[..] (RankHops) read from LDS,
+// and the sign is (-1)^popcount(su & sites strictly between p and q) as in
+// k_hop_fill.  Bonds come in twelve runs: [local | may leave the segment] x
+// [both ends below B | both at or above B | across] x [no site between p
+// and q | some], local ones being those below the segments' top-site split,
+// whose targets stay in the segment.  Local bonds go four at a time (all
+// table reads, then all row reads, each group one LDS round trip); a lane
+// whose string does not hop reads the zero row after the segment.
+struct RankArgs {
+  int n[12];
+  int B;
+  uint32_t lomask;
+  uint32_t m[kMaxBonds];     // 1 << p | 1 << q
+  uint32_t mlo4[kMaxBonds];  // (m & lomask) << 2: byte offset change in rlo
+  uint32_t mhi4[kMaxBonds];  // (m >> B) << 2: likewise in rhi
+  uint32_t btw[kMaxBonds];   // sites strictly between p and q
+  double hd[kMaxBonds];
+  float hf[kMaxBonds];
+  const uint32_t* rlo;
+  const uint32_t* rhi;
+  int nlo, nhi;
+};
+constexpr size_t kRankHead = 8 * kRankHalfWords;  // both tables, static LDS of k_up_segr
+
+static __device__ __forceinline__ float rank_h(const RankArgs& R, int b, float) { return R.hf[b]; }
+static __device__ __forceinline__ double rank_h(const RankArgs& R, int b, double) { return R.hd[b]; }
+
+// Rank of the string su ^ m (CLS 0: m below B, 1: m at or above B, 2: across).
+template <int CLS>
+static __device__ __forceinline__ uint32_t rank_target(const uint32_t* rt, const RankArgs& R, int b, uint32_t L4,
+                                                       uint32_t H4, uint32_t rl, uint32_t rh) {
+  const unsigned char* sm = reinterpret_cast<const unsigned char*>(rt);
+  const unsigned char* sh = sm + 4 * kRankHalfWords;
+  if (CLS == 0) return *reinterpret_cast<const uint32_t*>(sm + (L4 ^ R.mlo4[b])) + rh;
+  if (CLS == 1) return rl + *reinterpret_cast<const uint32_t*>(sh + (H4 ^ R.mhi4[b]));
+  return *reinterpret_cast<const uint32_t*>(sm + (L4 ^ R.mlo4[b])) +
+         *reinterpret_cast<const uint32_t*>(sh + (H4 ^ R.mhi4[b]));
+}
+
+template <typename T, bool SGN>
+static __device__ __forceinline__ T rank_coef(const RankArgs& R, int b, uint32_t su) {
+  const T h = rank_h(R, b, (T)0);
+  return SGN ? flip_sign(h, (uint32_t)__popc(su & R.btw[b]) << 31) : h;
+}
+
+template <typename T, int C, int CLS, bool SGN, bool LEAVE>
+static __device__ __forceinline__ void rank_bonds(const RankArgs& R, int b0, int b1, uint32_t su, uint32_t L4,
+                                                  uint32_t H4, uint32_t rl, uint32_t rh, const uint32_t* rt,
+                                                  const typename Vec<T, C>::type* P, uint32_t lo, uint32_t nrows,
+                                                  const T* __restrict__ q, size_t ld, int c0, int dimdw, T (&acc)[C]) {
+  typedef typename Vec<T, C>::type VT;
+  int b = b0;
+  if (!LEAVE) {
+    constexpr int G = 4;
+    for (; b + G <= b1; b += G) {
+      uint32_t sel[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const uint32_t m = R.m[b + u], tm = su & m;
+        const uint32_t idx = rank_target<CLS>(rt, R, b + u, L4, H4, rl, rh);
+        sel[u] = (tm != 0u && tm != m) ? idx - lo : nrows;
+      }
+      VT x[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) x[u] = P[sel[u]];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const T cf = rank_coef<T, SGN>(R, b + u, su);
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] += cf * Vec<T, C>::get(x[u], c);
+      }
+    }
+  }
+  for (; b < b1; ++b) {
+    const uint32_t m = R.m[b], tm = su & m;
+    const bool hop = tm != 0u && tm != m;
+    const uint32_t idx = rank_target<CLS>(rt, R, b, L4, H4, rl, rh);
+    const uint32_t li = idx - lo;
+    VT x = P[(hop && li < nrows) ? li : nrows];
+    if (LEAVE && hop && li >= nrows) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) Vec<T, C>::set(x, c, (c0 + c < dimdw) ? q[(size_t)(c0 + c) * ld + idx] : (T)0);
+    }
+    const T cf = rank_coef<T, SGN>(R, b, su);
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] += cf * Vec<T, C>::get(x, c);
+  }
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_up_segr(TwoPassView V, InterParams I, RankArgs R, T* qa, T* qb,
+                                                  T* __restrict__ tout, LzChain L, int chain,
+                                                  const int* __restrict__ seg, int nseg, int64_t ntiles, int64_t n8) {
+  typedef typename Vec<T, C>::type VT;
+  __shared__ uint32_t rt[2 * kRankHalfWords];
+  extern __shared__ __align__(16) unsigned char smem[];
+  VT* P = reinterpret_cast<VT*>(smem);  // segment rows, then one zero row
+  const int64_t t = seg_xcd_tile(n8);
+  if (t >= ntiles) return;
+  const T* __restrict__ q = qa;
+  if (chain) {
+    const LzScalars s = *L.sc;
+    if (s.stopped) return;
+    q = (s.k & 1) ? qb : qa;
+  }
+  const int pan = (int)(t / nseg), sg = (int)(t % nseg);
+  const int lo = seg[sg], hi = seg[sg + 1];
+  const int dimdw = V.dimdw;
+  const int c0 = pan * C;
+  const size_t ld = (size_t)V.ld;
+  const uint32_t nrows = (uint32_t)(hi - lo);
+  for (int k = threadIdx.x; k < R.nlo; k += blockDim.x) rt[k] = R.rlo[k];
+  for (int k = threadIdx.x; k < R.nhi; k += blockDim.x) rt[kRankHalfWords + k] = R.rhi[k];
+  if (threadIdx.x == 0) {
+    VT z;
+#pragma unroll
+    for (int j = 0; j < C; ++j) Vec<T, C>::set(z, j, (T)0);
+    P[nrows] = z;
+  }
+  constexpr int SB = stage_rows<T, C>(32, 1, 16);
+  for (int r0 = lo + threadIdx.x; r0 < hi; r0 += SB * blockDim.x) {
+    T a[SB][C];
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) a[h][j] = (r < hi && c0 + j < dimdw) ? q[(size_t)(c0 + j) * ld + r] : (T)0;
+    }
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+      if (r >= hi) break;
+      VT x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, a[h][j]);
+      P[r - lo] = x;
+    }
+  }
+  __syncthreads();
+  uint32_t sd[C], sdm[C];
+  double dgd[C];
+  const bool one_orb = I.norb == 1 && I.npairs == 0;
+  const double u0 = I.U[0];
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    const int cc = (c0 + j < dimdw) ? c0 + j : c0;
+    sd[j] = (uint32_t)V.map_dw[cc];
+    sdm[j] = sd[j] & I.orb_mask[0];
+    dgd[j] = V.dg_dw[cc];
+  }
+  int e[13];
+  e[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) e[k + 1] = e[k] + R.n[k];
+  for (int row = lo + threadIdx.x; row < hi; row += blockDim.x) {
+    const uint32_t su = (uint32_t)V.map_up[row];
+    const double dgu = V.dg_up[row];
+    const VT self = P[row - lo];
+    T acc[C];
+    if (one_orb) {
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        acc[j] = (T)(dgu + dgd[j] + u0 * (double)__popc(su & sdm[j])) * Vec<T, C>::get(self, j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < C; ++j) acc[j] = (T)(dgu + dgd[j] + inter2(I, su, sd[j])) * Vec<T, C>::get(self, j);
+    }
+    const uint32_t L4 = (su & R.lomask) << 2, H4 = (su >> R.B) << 2;
+    const uint32_t rl = rt[L4 >> 2], rh = rt[kRankHalfWords + (H4 >> 2)];
+#define EDL_RANK_RUN(k, CLS, SGN, LEAVE)                                                                         \
+  rank_bonds<T, C, CLS, SGN, LEAVE>(R, e[k], e[k + 1], su, L4, H4, rl, rh, rt, P, (uint32_t)lo, nrows, q, ld, c0, \
+                                    dimdw, acc)
+    EDL_RANK_RUN(0, 0, false, false);
+    EDL_RANK_RUN(1, 0, true, false);
+    EDL_RANK_RUN(2, 1, false, false);
+    EDL_RANK_RUN(3, 1, true, false);
+    EDL_RANK_RUN(4, 2, false, false);
+    EDL_RANK_RUN(5, 2, true, false);
+    EDL_RANK_RUN(6, 0, false, true);
+    EDL_RANK_RUN(7, 0, true, true);
+    EDL_RANK_RUN(8, 1, false, true);
+    EDL_RANK_RUN(9, 1, true, true);
+    EDL_RANK_RUN(10, 2, false, true);
+    EDL_RANK_RUN(11, 2, true, true);
+#undef EDL_RANK_RUN
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+      if (c0 + j < dimdw) tout[(size_t)(c0 + j) * ld + row] = acc[j];
+  }
+}
+
 // Phase B of a chain over whole columns (block = columns b, b + grid, ...).
 template <typename T>
 __global__ __launch_bounds__(256) void k_phase_b(TwoPassView V, T* qa, T* qb, LzChain L) {
@@ -1651,8 +1855,10 @@ __global__ __launch_bounds__(256) void k_lz_beta_n(LzChain L, int nparts) {
 }
 
 int seg_row_cap(size_t elem, int C) {
-  const size_t head = lds_head_bytes(elem);
-  int cap = (int)(((size_t)lds_limit_bytes() - 256 - head) / (elem * (size_t)C));
+  // room for the larger head of the two kernels (coefficient lut, rank
+  // tables) and k_up_segr's zero row
+  const size_t head = std::max(lds_head_bytes(elem), kRankHead);
+  int cap = (int)(((size_t)lds_limit_bytes() - 256 - head) / (elem * (size_t)C)) - 1;
   if (const char* e = std::getenv("EDL_UP_SEGMENT_ROWS")) {
     const int v = std::atoi(e);
     EDL_CHECK(v >= 1, "EDL_UP_SEGMENT_ROWS must be a positive row count");
@@ -1677,7 +1883,8 @@ const std::vector<int>& seg_bounds(Sector& s, size_t elem, int C) {
   while (bits < 31 && (map.back() >> bits) != 0) ++bits;
   // the coarsest split by top-site occupation whose runs all fit
   std::vector<int> runs;
-  for (int sh = bits; sh >= 0; --sh) {
+  int sh = bits;
+  for (; sh >= 0; --sh) {
     runs.assign(1, 0);
     for (int i = 1; i < t.dim; ++i)
       if ((map[i] >> sh) != (map[i - 1] >> sh)) runs.push_back(i);
@@ -1686,6 +1893,7 @@ const std::vector<int>& seg_bounds(Sector& s, size_t elem, int C) {
     for (size_t k = 0; k + 1 < runs.size(); ++k) longest = std::max(longest, runs[k + 1] - runs[k]);
     if (longest <= cap) break;
   }
+  EDL_CHECK(sh >= 0, "seg_bounds: no split fits");
   // adjacent runs merged while they fit (fewer segments, more local hops)
   std::vector<int> b(1, 0);
   for (size_t k = 1; k < runs.size(); ++k) {
@@ -1698,6 +1906,7 @@ const std::vector<int>& seg_bounds(Sector& s, size_t elem, int C) {
   EDL_HIP(hipMemcpy(s.d_seg, b.data(), sizeof(int) * b.size(), hipMemcpyHostToDevice));
   s.seg_h = b;
   s.seg_cap = cap;
+  s.seg_sh = sh;
   return s.seg_h;
 }
 
@@ -1728,11 +1937,81 @@ static void launch_up_seg_c(const TwoPassView& V, const InterParams& I, const T*
   EDL_HIP(hipGetLastError());
 }
 
+bool seg_rank_ok(const Sector& s) { return s.sp[0].rk.ok; }
+
+// Kernel arguments of k_up_segr: the bonds of the up spin sorted into the
+// six runs (local / leaving the segment x below B / at or above B / across).
+static RankArgs rank_args(const Sector& s) {
+  const RankHops& rk = s.sp[0].rk;
+  RankArgs R{};
+  R.B = rk.B;
+  R.lomask = (uint32_t)rk.nlo - 1u;
+  R.rlo = rk.rlo;
+  R.rhi = rk.rhi;
+  R.nlo = rk.nlo;
+  R.nhi = rk.nhi;
+  int k = 0;
+  for (int run = 0; run < 12; ++run) {
+    const bool leave = run >= 6, sgn = run & 1;
+    const int cls = (run % 6) / 2;
+    for (int b = 0; b < rk.nb; ++b) {
+      const int p = rk.p[b], q = rk.q[b];  // p < q
+      if ((q >= s.seg_sh) != leave) continue;
+      const int c = (q < rk.B) ? 0 : (p >= rk.B ? 1 : 2);
+      const uint32_t btw = ((1u << q) - 1u) & ~((2u << p) - 1u);
+      if (c != cls || (btw != 0u) != sgn) continue;
+      const uint32_t m = (1u << p) | (1u << q);
+      R.m[k] = m;
+      R.mlo4[k] = (m & R.lomask) << 2;
+      R.mhi4[k] = (m >> rk.B) << 2;
+      R.btw[k] = btw;
+      R.hd[k] = rk.h[b];
+      R.hf[k] = (float)rk.h[b];
+      ++R.n[run];
+      ++k;
+    }
+  }
+  EDL_CHECK(k == rk.nb, "rank_args: bond runs do not cover the bonds");
+  return R;
+}
+
+template <typename T, int C>
+static void launch_up_segr_c(const TwoPassView& V, const InterParams& I, T* qa, T* qb, T* tout, const LzChain* L,
+                             const Sector& s, hipStream_t st) {
+  const int nseg = (int)s.seg_h.size() - 1;
+  int rows = 0;
+  for (int k = 0; k < nseg; ++k) rows = std::max(rows, s.seg_h[k + 1] - s.seg_h[k]);
+  const size_t lds = (size_t)(rows + 1) * C * sizeof(T);  // dynamic: segment and zero row
+  EDL_CHECK(kRankHead + lds <= (size_t)lds_limit_bytes() - 256, "launch_up_seg: segment exceeds LDS");
+  const RankArgs R = rank_args(s);
+  EDL_CHECK(R.nlo <= kRankHalfWords && R.nhi <= kRankHalfWords, "launch_up_seg: rank tables exceed LDS");
+  set_lds_limit((const void*)k_up_segr<T, C>, lds_limit_bytes() - 256 - (int)kRankHead);
+  const int64_t ntiles = (int64_t)((V.dimdw + C - 1) / C) * nseg;
+  const int64_t n8 = (ntiles + 7) / 8;
+  EDL_CHECK(8 * n8 < ((int64_t)1 << 31), "launch_up_seg: too many tiles");
+  hipLaunchKernelGGL((k_up_segr<T, C>), dim3((unsigned)(8 * n8)), dim3(1024), lds, st, V, I, R, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0, s.d_seg, nseg, ntiles, n8);
+  EDL_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                   const LzChain* L, const Sector& s, int C, hipStream_t st) {
+                   const LzChain* L, const Sector& s, int C, int rank, hipStream_t st) {
   EDL_CHECK(s.d_seg && s.seg_h.size() >= 2 && s.seg_cap == seg_row_cap(sizeof(T), C),
             "launch_up_seg: segment bounds not built for this panel width");
+  if (rank) {
+    EDL_CHECK(seg_rank_ok(s), "launch_up_seg: no rank tables for this sector");
+    switch (C) {
+      case 1: return launch_up_segr_c<T, 1>(V, I, qa, qb, tout, L, s, st);
+      case 2: return launch_up_segr_c<T, 2>(V, I, qa, qb, tout, L, s, st);
+      case 3: return launch_up_segr_c<T, 3>(V, I, qa, qb, tout, L, s, st);
+      case 4: return launch_up_segr_c<T, 4>(V, I, qa, qb, tout, L, s, st);
+      case 8:
+        if constexpr (sizeof(T) == 4) return launch_up_segr_c<T, 8>(V, I, qa, qb, tout, L, s, st);
+        throw Error("launch_up_seg: 8-column panels are fp32 only");
+      default: throw Error("launch_up_seg: unsupported panel width");
+    }
+  }
   switch (C) {
     case 1: return launch_up_seg_c<T, 1>(V, I, lut, qa, qb, tout, L, s, st);
     case 2: return launch_up_seg_c<T, 2>(V, I, lut, qa, qb, tout, L, s, st);
@@ -1756,9 +2035,9 @@ void launch_lz_beta_n(const LzChain& L, int nparts, hipStream_t st) {
 }
 
 template void launch_up_seg<double>(const TwoPassView&, const InterParams&, const double*, double*, double*, double*,
-                                    const LzChain*, const Sector&, int, hipStream_t);
+                                    const LzChain*, const Sector&, int, int, hipStream_t);
 template void launch_up_seg<float>(const TwoPassView&, const InterParams&, const float*, float*, float*, float*,
-                                   const LzChain*, const Sector&, int, hipStream_t);
+                                   const LzChain*, const Sector&, int, int, hipStream_t);
 template <typename T>
 void launch_deflate(const TwoPassView& V, T* qa, T* qb, const LzChain& L, const double* X, int nx, double* part,
                     double* coef, hipStream_t st) {

@@ -81,6 +81,8 @@ struct TwoPassCfg {
                         //    (ncol, kch) in (1, 2|4), (2, 1|2|4))
   int dw_rng = 0;       // b > 0: range-blocked down pass over the column ranges of b top sites
                         //    (R 32|64, threads_dw; ncol, kch, nchunk, dw_vec unused)
+  int seg_rank = 0;     // segmented sectors, 1: the up pass finds hop targets by rank arithmetic
+                        //    (RankHops) instead of reading the hop table
   // Down-pass shape of the Lanczos chains (the pass with the Lanczos terms
   // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
   int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0, ch_rng = 0;
@@ -193,9 +195,12 @@ TwoPassCfg seg_dw_cfg(int dimdw);
 // H_up] q_k unscaled for the current k (phase B already done).  Tiles are
 // (panel of C columns, segment); seg_bounds(s, sizeof(T), C) must be the
 // sector's current plan.
+// rank = 1: hop targets by rank arithmetic (needs s.sp[0].rk.ok).
 template <typename T>
 void launch_up_seg(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
-                   const LzChain* L, const Sector& s, int C, hipStream_t st);
+                   const LzChain* L, const Sector& s, int C, int rank, hipStream_t st);
+// Whether the segmented up pass of `s` can find hop targets by rank arithmetic.
+bool seg_rank_ok(const Sector& s);
 // Phase B of a chain as a streaming pass (r = t - alpha_k q_k / s_k over t,
 // block partials of |r|^2 into L.part_up, replay term x += c_k q_k / s_k),
 // for the segmented path; `grid` blocks = the partial count for

@@ -153,6 +153,70 @@ __global__ void k_rank(const int32_t* states, int64_t n, int32_t* out, int Ns, i
 
 static int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// Split rank tables and bond list of one spin (RankHops), where its hopping
+// matrix is symmetric with at most kMaxBonds bonds on at most 20 sites.  The
+// tables are checked against the ascending enumeration of all C(Ns, N)
+// strings (Gosper's successor), which is the order of the sector's map.
+static void build_rank_hops(SpinTable& t, int Ns, const std::vector<HopPair>& pairs, hipStream_t st) {
+  RankHops& r = t.rk;
+  r = RankHops{};
+  if (Ns > 20 || t.N < 1 || t.N >= Ns) return;
+  for (const HopPair& a : pairs) {
+    bool twin = false;
+    for (const HopPair& b : pairs) twin = twin || (b.p == a.q && b.q == a.p && b.h == a.h);
+    if (!twin) return;
+    if (a.p < a.q) {
+      if (r.nb == kMaxBonds) return;
+      r.p[r.nb] = a.p;
+      r.q[r.nb] = a.q;
+      r.h[r.nb] = a.h;
+      ++r.nb;
+    }
+  }
+  r.B = Ns / 2;
+  r.nlo = 1 << r.B;
+  r.nhi = 1 << (Ns - r.B);
+  uint64_t C[33][33] = {};
+  for (int n = 0; n <= 32; ++n) {
+    C[n][0] = 1;
+    for (int k = 1; k <= n; ++k) C[n][k] = C[n - 1][k - 1] + (k <= n - 1 ? C[n - 1][k] : 0);
+  }
+  std::vector<uint32_t> lo(r.nlo), hi(r.nhi);
+  for (int L = 0; L < r.nlo; ++L) {
+    uint64_t v = 0;
+    int j = 0;
+    for (int p = 0; p < r.B; ++p)
+      if ((L >> p) & 1) v += C[p][++j];
+    lo[L] = (uint32_t)v;
+  }
+  for (int H = 0; H < r.nhi; ++H) {
+    const int nl = t.N - __builtin_popcount((unsigned)H);
+    uint64_t v = 0;
+    if (nl >= 0) {
+      int j = 0;
+      for (int p = 0; p < Ns - r.B; ++p)
+        if ((H >> p) & 1) {
+          ++j;
+          v += C[r.B + p][nl + j];
+        }
+    }
+    hi[H] = (uint32_t)v;
+  }
+  const uint32_t lomask = (uint32_t)r.nlo - 1u;
+  uint32_t s = (1u << t.N) - 1u;
+  for (int a = 0; a < t.dim; ++a) {
+    EDL_CHECK(lo[s & lomask] + hi[s >> r.B] == (uint32_t)a, "build_rank_hops: split rank tables disagree");
+    const uint32_t c = s & (0u - s), rr = s + c;  // next string of the same popcount
+    s = (((rr ^ s) >> 2) / c) | rr;
+  }
+  EDL_HIP(hipMalloc(&r.rlo, sizeof(uint32_t) * r.nlo));
+  EDL_HIP(hipMalloc(&r.rhi, sizeof(uint32_t) * r.nhi));
+  EDL_HIP(hipMemcpyAsync(r.rlo, lo.data(), sizeof(uint32_t) * r.nlo, hipMemcpyHostToDevice, st));
+  EDL_HIP(hipMemcpyAsync(r.rhi, hi.data(), sizeof(uint32_t) * r.nhi, hipMemcpyHostToDevice, st));
+  EDL_HIP(hipStreamSynchronize(st));  // lo, hi (host) must outlive the copies
+  r.ok = true;
+}
+
 static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 dw*/) {
   const int Ns = M.Ns;
   hipStream_t st = c.stream;
@@ -180,6 +244,7 @@ static void build_spin(Ctx& c, const Model& M, SpinTable& t, int spin /*0 up, 1 
       pairs.push_back(HopPair{(uint8_t)p, (uint8_t)q, (uint8_t)(v < 255 ? v : 255), h});
     }
   t.nvals = (int)vals.size();
+  if (spin == 0) build_rank_hops(t, Ns, pairs, st);  // read by the segmented up pass only
   HopPair* d_pairs = nullptr;
   int32_t* d_cnt = nullptr;
   const int np = (int)pairs.size();
@@ -310,6 +375,8 @@ void free_sector(Sector& s) {
       if (ct.ns16) (void)hipFree(ct.ns16);
     }
     if (t.pk16) (void)hipFree(t.pk16);
+    if (t.rk.rlo) (void)hipFree(t.rk.rlo);
+    if (t.rk.rhi) (void)hipFree(t.rk.rhi);
     for (auto& rt : t.rng) {
       if (rt.lo) (void)hipFree(rt.lo);
       if (rt.words) (void)hipFree(rt.words);

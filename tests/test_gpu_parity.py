@@ -221,19 +221,30 @@ def test_hxv_device_buffers_natural_layout(ed, orc, prec, dt, tol):
     assert _relerr(got, ref) < tol
 
 
-@pytest.mark.parametrize("cfg,sec,rows", [("ring8", (4, 4), 16), ("rect3x4", (7, 6), 100), ("rect3x4", (6, 6), 300)])
-def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, monkeypatch):
+@pytest.mark.parametrize("rank,cols", [(0, 0), (1, 0), (1, 8)])
+@pytest.mark.parametrize("cfg,sec,rows", [("ring8", (4, 4), 16), ("rect3x4", (7, 6), 100), ("rect3x4", (6, 6), 300),
+                                          ("rect3x4", (3, 6), 40), ("rect3x4", (11, 6), 4)])
+def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, rank, cols, monkeypatch):
     """The segmented up pass (columns cut into row ranges that share their
     top sites' occupation; cross-segment neighbours from global memory),
-    forced on small sectors by capping the rows per segment."""
+    forced on small sectors by capping the rows per segment, with hop
+    targets from the hop table (rank 0) and by rank arithmetic (rank 1:
+    split rank tables, bonds below / above / across the split, local and
+    leaving the segment, with and without sites between their ends), the
+    panel width tuned or pinned to 8 columns (fp32; 4 in fp64)."""
     monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", str(rows))
+    monkeypatch.setenv("EDL_UP_SEG_RANK", str(rank))
+    monkeypatch.setenv("EDL_UP_SEG_COLS", str(cols) if cols else "")
     c = lattice.CONFIGS[cfg]
     m = _use(ed, orc, c["H"](), [c["U"]])
     isec = ed.sector_index(*sec)
     dim, du, _ = ed.build_sector(isec)
-    print(f"segmented up pass {cfg} {sec} DimUp {du} rows/segment {rows}", flush=True)
+    print(f"segmented up pass {cfg} {sec} DimUp {du} rows/segment {rows} rank {rank}", flush=True)
     up, dw, sched = ed.bench_passes(isec, "fp64", 1)
     assert sched["two_pass"] == 2
+    assert sched["up_bank_scheduled"] == rank
+    if cols:
+        assert sched["up_cols"] == 4 and ed.bench_passes(isec, "fp32", 1)[2]["up_cols"] == cols
     for _ in range(2):
         v = RNG.standard_normal(dim)
         ref = orc.matvec(m, *sec, v, nthreads=8).real
@@ -342,6 +353,7 @@ def test_lanczos_segmented_rect3x4_vs_oracle(ed, orc, monkeypatch):
     ref_run = ed.lanczos_eigh(isec, "fp64", niter=200)
     obs_ref = ed.observables()
     monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "300")
+    monkeypatch.setenv("EDL_UP_SEG_RANK", "1")  # the other form: the chain-mode test of the batched routes
     r = ed.lanczos_eigh(isec, "fp64", niter=200)
     obs = ed.observables()
     Hc = orc.hamiltonian_csr(m, 6, 6)
@@ -542,22 +554,23 @@ def _assert_converged_poles_match(gpu, ora, tol, window=1e-6, conv_tol=1e-10):
             assert abs((p[sg] * w[sg]).sum() / wg - (po[so] * wo[so]).sum() / wo_) < tol * max(1.0, 1e-6 / wg)
 
 
-@pytest.fixture(params=["two_pass", "tiles", "persist", "single", "chain", "chain_segmented"])
+@pytest.fixture(params=["two_pass", "tiles", "persist", "single", "chain", "chain_segmented", "chain_segmented_rank"])
 def batched_kernels(request, monkeypatch):
     """Kernels of the batched chains: the batched two-pass LDS kernels
     (default where the sectors fit; the faster of the tile launches and the
     persistent launch, or either one forced), the single-pass gather
     kernels, or one chain at a time through the two-pass step (the route of
     sectors large enough to fill the GPU alone), also with the segmented up
-    pass."""
+    pass (hop table, or hop targets by rank arithmetic)."""
     if request.param in ("tiles", "persist"):
         monkeypatch.setenv("EDL_BATCHED_KERNELS", request.param)
     elif request.param == "single":
         monkeypatch.setenv("EDL_BATCHED_KERNELS", "single")
     elif request.param.startswith("chain"):
         monkeypatch.setenv("EDL_BATCHED_KERNELS", "chain")
-        if request.param == "chain_segmented":
+        if request.param.startswith("chain_segmented"):
             monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "24")
+            monkeypatch.setenv("EDL_UP_SEG_RANK", "1" if request.param.endswith("rank") else "0")
     else:
         monkeypatch.delenv("EDL_BATCHED_KERNELS", raising=False)
     print(f"batched kernels: {request.param}", flush=True)
@@ -887,6 +900,7 @@ def test_lanczos_states_ring8_vs_dense(ed, orc, route, monkeypatch):
     residuals |Hx - Ex| and orthonormality of the vectors."""
     if route == "segmented":
         monkeypatch.setenv("EDL_UP_SEGMENT_ROWS", "24")
+        monkeypatch.setenv("EDL_UP_SEG_RANK", "1")
     m = _use(ed, orc, lattice.ring(8), [4.0])
     isec = ed.sector_index(4, 4)
     Hd = orc.hamiltonian_csr(m, 4, 4).toarray().real
