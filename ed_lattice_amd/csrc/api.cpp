@@ -183,6 +183,21 @@ static void with_chain_shapes(TwoPassCfg& dst, const TwoPassCfg& src) {
   dst.ch_rng = src.ch_rng;
 }
 
+// Column-chunk counts the down-pass tuner tries: fixed ones, and for the
+// vector-gather form (slots = columns a block takes per round, NCOL x waves)
+// the counts that give every chunk one or two full rounds, so that no round
+// runs with idle waves (at most kMaxFitChunks).
+constexpr int kMaxFitChunks = 1024;
+static std::vector<int> dw_chunk_counts(int dimdw, int slots, bool with32 = true) {
+  std::vector<int> v = with32 ? std::vector<int>{32, 64, 128, 256} : std::vector<int>{64, 128, 256};
+  if (slots > 0)
+    for (int k : {1, 2}) {
+      const int n = (dimdw + k * slots - 1) / (k * slots);
+      if (n <= kMaxFitChunks && std::find(v.begin(), v.end(), n) == v.end()) v.push_back(n);
+    }
+  return v;
+}
+
 // Pick panel widths, block sizes and grids of the two passes by timing the
 // operator on the sector itself (3 launches per candidate after a warm-up).
 template <typename T>
@@ -309,7 +324,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     for (int R : {16, 32, 64, 128, 256})
       for (int nk : {108, 116, 132, 208, 216, 102, 104, 201, 202, 204})
         for (int th : {256, 512, 1024})
-          for (int nch : {32, 64, 128, 256}) {
+          for (int nch : dw_chunk_counts(V.dimdw, vec ? (nk / 100) * (th / 64) : 0)) {
           if (nch > V.dimdw) continue;
           const int nc = nk / 100, kc = nk % 100;
           const bool vk = kc <= 4;
@@ -379,7 +394,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     launch_zero_bytes(ab.p, sizeof(double) * 8, st);
     size_t maxpart = 1;
     for (int R : {64, 128})
-      for (int nch : {64, 128, 256}) {
+      for (int nch : {64, 128, 256, kMaxFitChunks}) {
         TwoPassCfg w = best;
         w.R = R;
         w.nchunk = std::min(nch, V.dimdw);
@@ -405,7 +420,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     for (int R : {64, 128})
       for (int nk : {102, 104, 201, 202, 204})
         for (int th : {256, 512, 1024})
-          for (int nch : {64, 128, 256}) {
+          for (int nch : dw_chunk_counts(V.dimdw, (nk / 100) * (th / 64), false)) {
             if (nch > V.dimdw) continue;
             TwoPassCfg w = best;
             w.R = R;

@@ -523,13 +523,17 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
   const int dimup = V.dimup, dimdw = V.dimdw;
   const size_t ld = (size_t)V.ld;
   const int b = blockIdx.x;
-  const int xcd = b % 8, ib = b / 8;
-  const int jb = ib % nchunk;
+  // (panel, chunk) units in panel-major order, each XCD a contiguous run of
+  // them (block b goes to XCD b % 8): a panel's chunks run on one XCD at a
+  // time, and the XCDs' shares differ by at most one unit
+  const int64_t n8 = gridDim.x / 8;
+  const int64_t u = (int64_t)(b % 8) * n8 + b / 8;
+  const int jb = (int)(u % nchunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / LPS, l = lane % LPS;
   const int nwaves = blockDim.x >> 6;
   const int npan = (dimup + R - 1) / R;
-  const int p = 8 * (ib / nchunk) + xcd;
+  const int p = (int)(u / nchunk);
   const int cb = (int)((int64_t)jb * dimdw / nchunk), ce = (int)((int64_t)(jb + 1) * dimdw / nchunk);
   extern __shared__ __align__(16) unsigned char dsm[];
   uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);
@@ -1302,6 +1306,7 @@ void launch_lz_alpha(const LzChain& L, const TwoPassView& V, const TwoPassCfg& c
 
 int pass_dw_grid(const TwoPassView& V, const TwoPassCfg& cfg) {
   const int npan = (V.dimup + cfg.R - 1) / cfg.R;
+  if (cfg.dw_vec == 1 && cfg.dw_rng == 0) return (int)(8 * (((int64_t)npan * cfg.nchunk + 7) / 8));
   if (cfg.dw_rng > 0) return 8 * V.rng_n[cfg.dw_rng] * ((npan + 7) / 8);
   return 8 * cfg.nchunk * ((npan + 7) / 8);
 }
@@ -1396,6 +1401,8 @@ static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
   const void* bufs[4] = {qa, qb, tin, out};
   if (cfg.dw_vec && !pass_dw_vec_ok(V, cfg, sizeof(T), bufs, 4)) {
     // natural layout (ld = DimUp) or unaligned caller buffers: the 4-byte form
+    // (operator mode only: a chain's partial count follows its own form)
+    EDL_CHECK(MODE == 0, "launch_pass_dw: the chain step's vector-gather down pass needs the padded, aligned layout");
     cfg.dw_vec = 0;
     cfg.ncol = 2;
     cfg.kch = 8;
