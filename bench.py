@@ -257,6 +257,10 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
               else f"k_up_seg<{t}, {sched['up_cols']}, 8>")
         parts = (("pass_up", up_ms, 2 * dim * elem, up),
                  ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
+    elif sched["two_pass"] and sched["up_bank_scheduled"] == 4:  # two-hop words split by sign
+        parts = (("pass_up", up_ms, 2 * dim * elem,
+                  f"k_pass_ups<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}>"),
+                 ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
     elif sched["two_pass"]:
         parts = (("pass_up", up_ms, 2 * dim * elem,
                   f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}, "

@@ -222,8 +222,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               "EDL_TWO_PASS_CFG_F32/_F64: dw_rng needs range tables for that many top sites and R 32|64 within LDS");
     EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T), p.up_col),
               "EDL_TWO_PASS_CFG_F32/_F64: up_col 1 needs fp32 panels of 1 or 3 columns on a sector with the "
-              "bank-scheduled table, up_col 2 also one hopping magnitude and DimUp + 32 < 2^15, up_col 3 "
-              "one hopping magnitude and DimUp + 32 < 2^15");
+              "bank-scheduled table, up_col 2 and 4 also one hopping magnitude and DimUp + 32 < 2^15 (fp64: "
+              "one column), up_col 3 one hopping magnitude and DimUp + 32 < 2^15");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
                       "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
@@ -285,7 +285,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     best.grid_up = c.num_cu;
     best.nchunk = std::min(32, V.dimdw);
     double best_up = 1e300, best_dw = 1e300;
-    for (int col : {0, 1, 2, 3})
+    for (int col : {0, 1, 2, 3, 4})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
       for (int th : {512, 1024})
@@ -482,7 +482,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     DevBuf pu(sizeof(double) * (size_t)c.num_cu * 2);
     A.part_up = pu.as<double>();
     double bu2 = 1e300;
-    for (int col : {0, 1, 2, 3})
+    for (int col : {0, 1, 2, 3, 4})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
         for (int th : {512, 1024})

@@ -85,6 +85,14 @@ struct ColorTable {
   uint32_t* words16 = nullptr;
   int32_t* off16 = nullptr;    // [nblocks] first word of block b
   int32_t* ns16 = nullptr;     // [nblocks] pairs of block b (ceil(ns / 2))
+  // Two hops to a word, split by sign (built with words16): the hops of a
+  // block that take +h and those that take -h are bank-scheduled
+  // separately, each set's slots paired into words (halves = LDS rows, no
+  // sign bits), the +h words first, so every lane of a word has its sign.
+  uint32_t* words16s = nullptr;
+  int32_t* off16s = nullptr;   // [nblocks] first word of block b
+  int32_t* ns16s = nullptr;    // [nblocks] words of block b
+  int32_t* np16s = nullptr;    // [nblocks] its leading words that take +h
 };
 
 // Column ranges of the down spin for the range-blocked down pass: the

@@ -91,12 +91,14 @@ static __device__ __forceinline__ double flip_sign(double h, uint32_t sbit) {
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false,
+          bool SGN = false>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
 
-template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false>
+template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false,
+          bool SGN = false>
 static __device__ __forceinline__ void up_panel(const TwoPassView& V, const InterParams& I, const T* lut,
                                                 typename Vec<T, C>::type* P, const T* __restrict__ q,
                                                 T* __restrict__ tout, int c0, T ts) {
@@ -125,7 +127,7 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
     }
   }
   __syncthreads();
-  up_rows<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, tout, c0, ts);
+  up_rows<T, C, RPT, UKCH, LREG, COL, H16, SGN>(V, I, lut, P, tout, c0, ts);
 }
 
 // The row loop of a panel already staged in P: RPT rows per thread, hop
@@ -140,7 +142,7 @@ static __device__ __forceinline__ void up_panel(const TwoPassView& V, const Inte
 // H16 (with COL and LREG, fp32): two slots to a table word, a half being the
 // LDS row and the sign of the coefficient; unused slots gather a zero row
 // (rows dimup .. dimup+31 of P hold zeros).
-template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16>
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16, bool SGN>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts) {
@@ -166,7 +168,7 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
   const T l0 = lut[0], l1 = lut[1];
   const int stride = RPT * blockDim.x;
   for (int i0 = threadIdx.x; i0 < dimup; i0 += stride) {
-    int row[RPT], nh[RPT];
+    int row[RPT], nh[RPT], npw[RPT];
     const uint32_t* tb[RPT];
     bool okr[RPT];
     int nmax = 0;
@@ -176,6 +178,7 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
       row[h] = i0 + h * (int)blockDim.x;
       okr[h] = row[h] < dimup;
       if (!okr[h]) row[h] = i0;
+      npw[h] = 0;
       if (COL && H16) {
         // the 64 rows of a wave are one block of the table (blockDim is a
         // multiple of 64), so its slot count is wave-uniform: the hop loop
@@ -183,8 +186,15 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
         // its zero-row padding (their sums are never stored).
         const int b = __builtin_amdgcn_readfirstlane(i0 + h * (int)blockDim.x) >> 6;
         const bool inb = b < ((dimup + 63) >> 6);
-        nh[h] = inb ? V.pkh_ns[W][b] : 0;
-        tb[h] = V.pkh_up[W] + (inb ? V.pkh_off[W][b] : 0) + ((i0 + h * (int)blockDim.x) & 63);
+        if (SGN) {
+          // split by sign: words [0, npw) take +h, the rest -h
+          nh[h] = inb ? V.pks_ns[W][b] : 0;
+          npw[h] = inb ? V.pks_np[W][b] : 0;
+          tb[h] = V.pks_up[W] + (inb ? V.pks_off[W][b] : 0) + ((i0 + h * (int)blockDim.x) & 63);
+        } else {
+          nh[h] = inb ? V.pkh_ns[W][b] : 0;
+          tb[h] = V.pkh_up[W] + (inb ? V.pkh_off[W][b] : 0) + ((i0 + h * (int)blockDim.x) & 63);
+        }
       } else if (COL) {
         const int b = row[h] >> 6;
         nh[h] = okr[h] ? V.pkc_ns[W][b] : 0;
@@ -226,8 +236,11 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
           if constexpr (H16) {
             static_assert(LREG && (!COL || sizeof(T) == 4 || C == 1), "two hops to a word: register coefficients");
             const uint32_t wl = w[h][j];
-            const T cl = flip_sign(l0, (wl << 16) & 0x80000000u);
-            const T chi = flip_sign(l0, wl & 0x80000000u);
+            // sign-split words carry one sign for the whole wave (set by the
+            // word index); the other forms' halves carry their own (bit 15)
+            const T cw = (k0 + j < npw[h]) ? l0 : l1;
+            const T cl = SGN ? cw : flip_sign(l0, (wl << 16) & 0x80000000u);
+            const T chi = SGN ? cw : flip_sign(l0, wl & 0x80000000u);
             const VT xl = P[wl & 0x7FFFu];
             const VT xh = P[(wl >> 16) & 0x7FFFu];
 #pragma unroll
@@ -312,9 +325,10 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = false, bool H16 = false>
-__global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
-                                                  T* qb, T* __restrict__ tout, LzChain L, int chain) {
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16, bool SGN>
+static __device__ __forceinline__ void pass_up_body(const TwoPassView& V, const InterParams& I,
+                                                    const T* __restrict__ lut_up, T* qa, T* qb,
+                                                    T* __restrict__ tout, const LzChain& L, int chain) {
   typedef typename Vec<T, C>::type VT;
   extern __shared__ __align__(16) unsigned char smem[];
   // H16 takes its two coefficients from registers: no LDS table, and the
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     if (!H16)
       for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
-      up_panel<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, qa, tout, pan * C, (T)1);
+      up_panel<T, C, RPT, UKCH, LREG, COL, H16, SGN>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
   }
   const LzScalars s = *L.sc;
@@ -358,12 +372,27 @@ __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, 
     __syncthreads();
     nrm += stage_panel<T, C, stage_rows<T, C>(32, 2, 8)>(V, P, qn, qk, pan * C, pend, last, am, x, cx, bst);
     __syncthreads();
-    if (!last) up_rows<T, C, RPT, UKCH, LREG, COL, H16>(V, I, lut, P, tout, pan * C, (T)1);
+    if (!last) up_rows<T, C, RPT, UKCH, LREG, COL, H16, SGN>(V, I, lut, P, tout, pan * C, (T)1);
   }
   if (pend && !last) {
     const double v = block_sum(nrm, sh);
     if (threadIdx.x == 0) L.part_up[blockIdx.x] = v;
   }
+}
+
+template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = false, bool H16 = false>
+__global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
+                                                  T* qb, T* __restrict__ tout, LzChain L, int chain) {
+  pass_up_body<T, C, RPT, UKCH, LREG, COL, H16, false>(V, I, lut_up, qa, qb, tout, L, chain);
+}
+
+// The same pass reading the bank-scheduled two-hop words split by sign
+// (ColorTable::words16s): one coefficient per word for the whole wave, so a
+// hop costs no sign decoding.
+template <typename T, int C, int RPT, int UKCH>
+__global__ __launch_bounds__(1024) void k_pass_ups(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
+                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
+  pass_up_body<T, C, RPT, UKCH, true, true, true, true>(V, I, lut_up, qa, qb, tout, L, chain);
 }
 
 // ------------------------------------------------------------------ pass D
@@ -1024,6 +1053,77 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
         }
     }
   }
+  // The sign-split form: the +h hops (cid 0) and the -h hops (cid 1) of
+  // each half-wave group coloured on their own, so that all lanes of a word
+  // share its sign; a slot's unused lanes read the zero row in a bank class
+  // the slot leaves free, and a set's odd last slot pairs with zero row dim.
+  std::vector<uint32_t> w16s;
+  std::vector<int32_t> off16s(nb), ns16s(nb), np16s(nb);
+  if (h16) {
+    auto zero_row = [&](int c) -> uint32_t {
+      for (int j = 0; j < 32; ++j)
+        if (cls(dim + j) == c) return (uint32_t)(dim + j);
+      return (uint32_t)dim;
+    };
+    for (int b = 0; b < nb; ++b) {
+      int nsl[2] = {0, 0};
+      std::vector<std::vector<int>> colS[2], laneS[2];
+      std::vector<std::vector<uint32_t>> rowS[2];
+      for (int sg = 0; sg < 2; ++sg) {
+        colS[sg].resize(G);
+        laneS[sg].resize(G);
+        rowS[sg].resize(G);
+      }
+      for (int h = 0; h < G; ++h) {
+        std::vector<int> el2[2], ek2[2];
+        std::vector<uint32_t> er2[2];
+        for (int l = 0; l < L; ++l) {
+          const int r = b * 64 + h * L + l;
+          if (r >= dim) break;
+          for (int k = 0; k < cnt[r]; ++k) {
+            const uint32_t w = pk[(size_t)k * dim + r];
+            const int sg = (int)((w >> 24) & 1u);
+            el2[sg].push_back(l);
+            ek2[sg].push_back(cls(w & 0xFFFFFFu));
+            er2[sg].push_back(w & 0xFFFFFFu);
+          }
+        }
+        for (int sg = 0; sg < 2; ++sg) {
+          nsl[sg] = std::max(nsl[sg], color_bipartite(el2[sg], ek2[sg], L, L, colS[sg][h]));
+          laneS[sg][h] = el2[sg];
+          rowS[sg][h] = er2[sg];
+        }
+      }
+      off16s[b] = (int32_t)w16s.size();
+      np16s[b] = (nsl[0] + 1) / 2;
+      ns16s[b] = np16s[b] + (nsl[1] + 1) / 2;
+      for (int sg = 0; sg < 2; ++sg) {
+        const int nsg = nsl[sg];
+        std::vector<uint32_t> slot((size_t)nsg * 64, 0xFFFFFFFFu);
+        for (int h = 0; h < G; ++h) {
+          std::vector<char> used((size_t)nsg * L, 0);
+          for (size_t e = 0; e < rowS[sg][h].size(); ++e) {
+            const int sl = colS[sg][h][e], l = laneS[sg][h][e];
+            slot[(size_t)sl * 64 + h * L + l] = rowS[sg][h][e];
+            used[(size_t)sl * L + cls(rowS[sg][h][e])] = 1;
+          }
+          for (int sl = 0; sl < nsg; ++sl) {
+            int freecls = 0;
+            while (freecls < L && used[(size_t)sl * L + freecls]) ++freecls;
+            const uint32_t z = zero_row(freecls < L ? freecls : 0);
+            for (int l = 0; l < L; ++l)
+              if (slot[(size_t)sl * 64 + h * L + l] == 0xFFFFFFFFu) slot[(size_t)sl * 64 + h * L + l] = z;
+          }
+        }
+        for (int p = 0; p < (nsg + 1) / 2; ++p)
+          for (int l = 0; l < 64; ++l) {
+            const uint32_t lo = slot[(size_t)(2 * p) * 64 + l];
+            const uint32_t hi = 2 * p + 1 < nsg ? slot[(size_t)(2 * p + 1) * 64 + l] : (uint32_t)dim;
+            w16s.push_back(lo | (hi << 16));
+          }
+      }
+    }
+  }
   EDL_HIP(hipMalloc(&ct.words, sizeof(uint32_t) * std::max<size_t>(words.size(), 1)));
   EDL_HIP(hipMalloc(&ct.off, sizeof(int32_t) * nb));
   EDL_HIP(hipMalloc(&ct.ns, sizeof(int32_t) * nb));
@@ -1037,6 +1137,14 @@ static void build_color_table(SpinTable& t, int W, hipStream_t st) {
     EDL_HIP(hipMemcpyAsync(ct.words16, w16.data(), sizeof(uint32_t) * w16.size(), hipMemcpyHostToDevice, st));
     EDL_HIP(hipMemcpyAsync(ct.off16, off16.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
     EDL_HIP(hipMemcpyAsync(ct.ns16, ns16.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMalloc(&ct.words16s, sizeof(uint32_t) * std::max<size_t>(w16s.size(), 1)));
+    EDL_HIP(hipMalloc(&ct.off16s, sizeof(int32_t) * nb));
+    EDL_HIP(hipMalloc(&ct.ns16s, sizeof(int32_t) * nb));
+    EDL_HIP(hipMalloc(&ct.np16s, sizeof(int32_t) * nb));
+    EDL_HIP(hipMemcpyAsync(ct.words16s, w16s.data(), sizeof(uint32_t) * w16s.size(), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(ct.off16s, off16s.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(ct.ns16s, ns16s.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(ct.np16s, np16s.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
   }
   EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
 }
@@ -1173,6 +1281,10 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
     v.pkh_up[W] = s.sp[0].pkc[W].words16;
     v.pkh_off[W] = s.sp[0].pkc[W].off16;
     v.pkh_ns[W] = s.sp[0].pkc[W].ns16;
+    v.pks_up[W] = s.sp[0].pkc[W].words16s;
+    v.pks_off[W] = s.sp[0].pkc[W].off16s;
+    v.pks_ns[W] = s.sp[0].pkc[W].ns16s;
+    v.pks_np[W] = s.sp[0].pkc[W].np16s;
   }
   v.pk16_up = s.sp[0].pk16;
   v.pk16_dw = s.sp[1].pk16;
@@ -1202,6 +1314,11 @@ static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut
 
 bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
   const int W = C * (int)elem / 4;
+  if (col == 4) {  // the sign-split words of the two-hop bank-scheduled tables
+    if (!((elem == 8 && C == 1) || (elem == 4 && (C == 1 || C == 3)))) return false;
+    return V.pks_up[W] != nullptr && V.nv_up == 1 &&
+           up_lds_bytes(V.dimup + 32, C, elem) <= (size_t)lds_limit_bytes() - 256;
+  }
   if (col == 3)
     return V.pk16_up != nullptr && (size_t)(V.dimup + 32) * C * elem <= (size_t)lds_limit_bytes() - 256;
   if (elem == 8 && C == 1 && col == 2)  // 8-byte rows: 16 lanes over 16 bank pairs
@@ -1215,8 +1332,24 @@ bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
 }
 
 template <typename T, int C, int RPT, int UKCH>
+static void launch_up_s(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                        const LzChain* L, int grid, int threads, hipStream_t st) {
+  const size_t lds = (size_t)(V.dimup + 32) * C * sizeof(T);
+  EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_pass_up: panel exceeds LDS");
+  set_lds_limit((const void*)k_pass_ups<T, C, RPT, UKCH>, lds_limit_bytes() - 256);
+  hipLaunchKernelGGL((k_pass_ups<T, C, RPT, UKCH>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0);
+  EDL_HIP(hipGetLastError());
+}
+
+template <typename T, int C, int RPT, int UKCH>
 static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, int col, hipStream_t st) {
+  if (col == 4) {
+    EDL_CHECK(up_col_ok(V, C, sizeof(T), 4), "launch_pass_up: no sign-split two-hop table for this sector and width");
+    if constexpr ((sizeof(T) == 8 && C == 1) || (sizeof(T) == 4 && (C == 1 || C == 3)))
+      return launch_up_s<T, C, RPT, UKCH>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+  }
   if (col == 3) {
     EDL_CHECK(up_col_ok(V, C, sizeof(T), 3), "launch_pass_up: no two-hop table for this sector");
     return launch_up_l<T, C, RPT, UKCH, true, false, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);

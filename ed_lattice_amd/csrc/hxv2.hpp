@@ -26,6 +26,11 @@ struct TwoPassView {
   const uint32_t* pkh_up[4];
   const int32_t* pkh_off[4];
   const int32_t* pkh_ns[4];
+  // two hops to a word bank-scheduled by sign (ColorTable::words16s), or null
+  const uint32_t* pks_up[4];
+  const int32_t* pks_off[4];
+  const int32_t* pks_ns[4];
+  const int32_t* pks_np[4];
   // the packed up-spin table two hops to a word in table order (SpinTable::pk16), or null
   const uint32_t* pk16_up;
   const uint32_t* pk16_dw;  // the down spin's, likewise (read by the batched down tiles)
@@ -71,7 +76,8 @@ struct TwoPassCfg {
   int up_kch = 8;       // hop chunk of the up pass ((1, 8|16) or (2, 4|8))
   int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3);
                         // 2: its form with two hops to a word (one hopping magnitude);
-                        // 3: the table order two hops to a word (any precision and width)
+                        // 3: the table order two hops to a word (any precision and width);
+                        // 4: two hops to a word bank-scheduled by sign (as 2, no per-hop sign)
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)

@@ -373,6 +373,10 @@ void free_sector(Sector& s) {
       if (ct.words16) (void)hipFree(ct.words16);
       if (ct.off16) (void)hipFree(ct.off16);
       if (ct.ns16) (void)hipFree(ct.ns16);
+      if (ct.words16s) (void)hipFree(ct.words16s);
+      if (ct.off16s) (void)hipFree(ct.off16s);
+      if (ct.ns16s) (void)hipFree(ct.ns16s);
+      if (ct.np16s) (void)hipFree(ct.np16s);
     }
     if (t.pk16) (void)hipFree(t.pk16);
     if (t.rk.rlo) (void)hipFree(t.rk.rlo);
