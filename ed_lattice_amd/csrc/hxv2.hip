@@ -235,6 +235,10 @@ static __device__ __forceinline__ void up_rows(const TwoPassView& V, const Inter
         for (int j = 0; j < UKCH; ++j) {
           if constexpr (H16) {
             static_assert(LREG && (!COL || sizeof(T) == 4 || C == 1), "two hops to a word: register coefficients");
+            // bank-scheduled words with their own signs: the word count is
+            // wave-uniform, so a chunk's words past it are skipped by a
+            // scalar branch (measured slower on the sign-split words)
+            if (COL && !SGN && k0 + j >= nmax) continue;
             const uint32_t wl = w[h][j];
             // sign-split words carry one sign for the whole wave (set by the
             // word index); the other forms' halves carry their own (bit 15)
