@@ -1,6 +1,7 @@
 // C-ABI of libedlat_hip (include/edlat_gpu.h): context, Hamiltonian and
 // sector set-up, the operator, the Lanczos drivers and the batched seeds.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -599,8 +600,15 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
   // {R, threads, chunks, vector gathers}: 4-byte gathers with 2 columns per
   // lane and hop chunk 8, vector gathers with 2 columns per wave and hop
   // chunk 2 per lane group
-  const int shapes[8][4] = {{64, 256, 256, 0}, {64, 1024, 64, 0}, {64, 512, 128, 0}, {32, 256, 256, 0},
-                            {64, 1024, 128, 1}, {128, 1024, 128, 1}, {128, 1024, 256, 1}, {32, 1024, 128, 1}};
+  // plus vector-gather shapes whose column chunks fill whole rounds of the
+  // block's waves (32 columns for 1024 threads x 2 columns per wave), and the
+  // 256-row fp32 panel
+  const int fit32 = (s.sp[1].dim + 31) / 32, fit16 = (s.sp[1].dim + 15) / 16;
+  std::vector<std::array<int, 4>> shapes = {{64, 256, 256, 0},  {64, 1024, 64, 0},   {64, 512, 128, 0},
+                                            {32, 256, 256, 0},  {64, 1024, 128, 1},  {128, 1024, 128, 1},
+                                            {128, 1024, 256, 1}, {32, 1024, 128, 1}, {128, 1024, fit32, 1},
+                                            {64, 1024, fit32, 1}};
+  if (sizeof(T) == 4) shapes.push_back({256, 512, fit16, 1});
   double bd = 1e300;
   TwoPassCfg dw = best;
   for (const auto& sh : shapes) {
@@ -637,8 +645,9 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
     EDL_HIP(hipMemcpyAsync(scd.p, &s0, sizeof(s0), hipMemcpyHostToDevice, st));
     launch_zero_bytes(ab.p, sizeof(double) * 8, st);
     // {R, columns per wave, hop chunk, threads, chunks}
-    const int cshapes[6][5] = {{128, 1, 4, 1024, 256}, {128, 2, 2, 1024, 256}, {64, 1, 4, 1024, 128},
-                               {64, 2, 2, 1024, 128},  {32, 1, 4, 1024, 128},  {32, 2, 2, 1024, 128}};
+    const std::vector<std::array<int, 5>> cshapes = {
+        {128, 1, 4, 1024, 256}, {128, 2, 2, 1024, 256}, {64, 1, 4, 1024, 128},   {64, 2, 2, 1024, 128},
+        {32, 1, 4, 1024, 128},  {32, 2, 2, 1024, 128},  {128, 2, 2, 1024, fit32}, {128, 1, 4, 1024, fit16}};
     size_t maxpart = 1;
     for (const auto& sh : cshapes) {
       TwoPassCfg w = dw;
