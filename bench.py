@@ -261,6 +261,11 @@ def kernel_table(dim, elem, up_ms, dw_ms, sched):
         parts = (("pass_up", up_ms, 2 * dim * elem,
                   f"k_pass_ups<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}>"),
                  ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
+    elif sched["two_pass"] and sched["up_bank_scheduled"] in (5, 6):  # forms 2 / 4, next panel preloaded
+        parts = (("pass_up", up_ms, 2 * dim * elem,
+                  f"k_pass_upp<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}, "
+                  f"{'true' if sched['up_bank_scheduled'] == 6 else 'false'}>"),
+                 ("pass_dw", dw_ms, 3 * dim * elem, dw_kernel_name(t, sched)))
     elif sched["two_pass"]:
         parts = (("pass_up", up_ms, 2 * dim * elem,
                   f"k_pass_up<{t}, {sched['up_cols']}, {sched['up_rows_per_thread']}, {sched['up_hop_chunk']}, "

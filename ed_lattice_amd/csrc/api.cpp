@@ -286,7 +286,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     best.grid_up = c.num_cu;
     best.nchunk = std::min(32, V.dimdw);
     double best_up = 1e300, best_dw = 1e300;
-    for (int col : {0, 1, 2, 3, 4})
+    for (int col : {0, 1, 2, 3, 4, 5, 6})
     for (int C : {1, 2, 3, 4})
       for (int rk : {208, 204, 108, 116})
       for (int th : {512, 1024})
@@ -295,6 +295,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
           if (need > (size_t)lds) continue;
           if (gm == 2 && 2 * need > (size_t)lds) continue;
           if (col && !up_col_ok(V, C, sizeof(T), col)) continue;
+          if (col >= 5 && V.dimup > 13 * th) continue;  // prefetching forms: 13 rows per thread at most
           TwoPassCfg w = best;
           w.C = C;
           w.up_col = col;

@@ -91,11 +91,48 @@ static __device__ __forceinline__ double flip_sign(double h, uint32_t sbit) {
 // One panel of C columns starting at c0: stage them in LDS (interleaved by
 // row), then T(:, c0+j) = [D + H_up] q(:, c0+j) * ts for every row.  Both
 // the single-sector kernel and the batched chain kernel run it.
+// Registers that hold a block's next panel between its loads and its LDS
+// store (PFR rows per thread and column).
+template <typename T, int C, int PFR>
+struct PanelPrefetch {
+  T v[C][PFR > 0 ? PFR : 1];
+};
+constexpr int kUpPrefetchRows = 13;  // 12 870 strings at 1024 threads (the 4x4 sector)
+
 template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false,
           bool SGN = false>
 static __device__ __forceinline__ void up_rows(const TwoPassView& V, const InterParams& I, const T* lut,
                                                const typename Vec<T, C>::type* P, T* __restrict__ tout, int c0,
                                                T ts);
+
+// Columns c0 .. c0+C-1 of q staged in P, rows interleaved (no barrier).
+template <typename T, int C>
+static __device__ __forceinline__ void stage_plain(const TwoPassView& V, typename Vec<T, C>::type* P,
+                                                   const T* __restrict__ q, int c0) {
+  typedef typename Vec<T, C>::type VT;
+  const int dimup = V.dimup, dimdw = V.dimdw;
+  const size_t ld = (size_t)V.ld;
+  constexpr int SB = stage_rows<T, C>(32, 1, 16);
+  for (int r0 = threadIdx.x; r0 < dimup; r0 += SB * blockDim.x) {
+    T a[SB][C];
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        a[h][j] = (r < dimup && c0 + j < dimdw) ? __builtin_nontemporal_load(&q[(size_t)(c0 + j) * ld + r]) : (T)0;
+    }
+#pragma unroll
+    for (int h = 0; h < SB; ++h) {
+      const int r = r0 + h * (int)blockDim.x;
+      if (r >= dimup) break;
+      VT x;
+#pragma unroll
+      for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, a[h][j]);
+      P[r] = x;
+    }
+  }
+}
 
 template <typename T, int C, int RPT = 2, int UKCH = 8, bool LREG = false, bool COL = false, bool H16 = false,
           bool SGN = false>
@@ -329,7 +366,7 @@ static __device__ __forceinline__ double stage_panel(const TwoPassView& V, typen
 // partial of |r|^2 into L.part_up, replay term x += c_k v_k), then
 // T = [D + H_up] q (unscaled; the down pass divides by s).  When the chain
 // has run its nmax steps only the replay term remains.
-template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16, bool SGN>
+template <typename T, int C, int RPT, int UKCH, bool LREG, bool COL, bool H16, bool SGN, int PFR = 0>
 static __device__ __forceinline__ void pass_up_body(const TwoPassView& V, const InterParams& I,
                                                     const T* __restrict__ lut_up, T* qa, T* qb,
                                                     T* __restrict__ tout, const LzChain& L, int chain) {
@@ -352,6 +389,49 @@ static __device__ __forceinline__ void pass_up_body(const TwoPassView& V, const 
   if (!chain) {
     if (!H16)
       for (int k = threadIdx.x; k < 256; k += blockDim.x) lut[k] = lut_up[k];
+    if constexpr (PFR > 0) {
+      // the block's first panel is staged as usual; each later one is
+      // loaded into registers (PanelPrefetch) before the barrier that ends
+      // the current panel and written to P after it
+      const int dimup = V.dimup;
+      int pan = blockIdx.x;
+      if (pan >= npan) return;
+      stage_plain<T, C>(V, P, qa, pan * C);
+      __syncthreads();
+      for (;;) {
+        const int nxt = pan + (int)gridDim.x;
+        up_rows<T, C, RPT, UKCH, LREG, COL, H16, SGN>(V, I, lut, P, tout, pan * C, (T)1);
+        if (nxt >= npan) break;
+        // the next panel's loads go out as soon as this thread's rows are
+        // done, so their round trip overlaps the other waves' last rows and
+        // the barrier (the waves of a block end an iteration apart)
+        PanelPrefetch<T, C, PFR> pf;
+        const int c1 = nxt * C;
+#pragma unroll
+        for (int j = 0; j < C; ++j)
+#pragma unroll
+          for (int kk = 0; kk < PFR; ++kk) {
+            const int r = (int)threadIdx.x + kk * (int)blockDim.x;
+            pf.v[j][kk] = (r < dimup && c1 + j < V.dimdw)
+                              ? __builtin_nontemporal_load(&qa[(size_t)(c1 + j) * V.ld + r])
+                              : (T)0;
+          }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < PFR; ++kk) {
+          const int r = (int)threadIdx.x + kk * (int)blockDim.x;
+          if (r < dimup) {
+            VT x;
+#pragma unroll
+            for (int j = 0; j < C; ++j) Vec<T, C>::set(x, j, pf.v[j][kk]);
+            P[r] = x;
+          }
+        }
+        __syncthreads();
+        pan = nxt;
+      }
+      return;
+    }
     for (int pan = blockIdx.x; pan < npan; pan += gridDim.x)
       up_panel<T, C, RPT, UKCH, LREG, COL, H16, SGN>(V, I, lut, P, qa, tout, pan * C, (T)1);
     return;
@@ -388,6 +468,16 @@ template <typename T, int C, int RPT, int UKCH, bool LREG = false, bool COL = fa
 __global__ __launch_bounds__(1024) void k_pass_up(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
   pass_up_body<T, C, RPT, UKCH, LREG, COL, H16, false>(V, I, lut_up, qa, qb, tout, L, chain);
+}
+
+// The bank-scheduled two-hop passes (SGN: the words split by sign) with the
+// block's next panel loaded into registers as each thread finishes the
+// current one (operator mode; DimUp <= kUpPrefetchRows * blockDim; a chain
+// step runs the form without it).
+template <typename T, int C, int RPT, int UKCH, bool SGN>
+__global__ __launch_bounds__(1024) void k_pass_upp(TwoPassView V, InterParams I, const T* __restrict__ lut_up, T* qa,
+                                                   T* qb, T* __restrict__ tout, LzChain L, int chain) {
+  pass_up_body<T, C, RPT, UKCH, true, true, true, SGN, kUpPrefetchRows>(V, I, lut_up, qa, qb, tout, L, chain);
 }
 
 // The same pass reading the bank-scheduled two-hop words split by sign
@@ -1318,6 +1408,10 @@ static void launch_up_l(const TwoPassView& V, const InterParams& I, const T* lut
 
 bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col) {
   const int W = C * (int)elem / 4;
+  // 5, 6: the forms 2, 4 with the next panel prefetched (launch checks DimUp
+  // against kUpPrefetchRows * threads)
+  if (col == 5 || col == 6)
+    return (col == 5 ? (elem == 4 ? (C == 1 || C == 3) : C == 1) : true) && up_col_ok(V, C, elem, col == 5 ? 2 : 4);
   if (col == 4) {  // the sign-split words of the two-hop bank-scheduled tables
     if (!((elem == 8 && C == 1) || (elem == 4 && (C == 1 || C == 3)))) return false;
     return V.pks_up[W] != nullptr && V.nv_up == 1 &&
@@ -1346,9 +1440,28 @@ static void launch_up_s(const TwoPassView& V, const InterParams& I, const T* lut
   EDL_HIP(hipGetLastError());
 }
 
+template <typename T, int C, int RPT, int UKCH, bool SGN>
+static void launch_up_p(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
+                        const LzChain* L, int grid, int threads, hipStream_t st) {
+  const size_t lds = (size_t)(V.dimup + 32) * C * sizeof(T);
+  EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 256, "launch_pass_up: panel exceeds LDS");
+  EDL_CHECK(V.dimup <= kUpPrefetchRows * threads, "launch_pass_up: the prefetching form takes DimUp <= kUpPrefetchRows * threads");
+  set_lds_limit((const void*)k_pass_upp<T, C, RPT, UKCH, SGN>, lds_limit_bytes() - 256);
+  hipLaunchKernelGGL((k_pass_upp<T, C, RPT, UKCH, SGN>), dim3(grid), dim3(threads), lds, st, V, I, lut, qa, qb, tout,
+                     L ? *L : LzChain{}, L ? 1 : 0);
+  EDL_HIP(hipGetLastError());
+}
+
 template <typename T, int C, int RPT, int UKCH>
 static void launch_up_k(const TwoPassView& V, const InterParams& I, const T* lut, T* qa, T* qb, T* tout,
                         const LzChain* L, int grid, int threads, int col, hipStream_t st) {
+  if (col == 5 || col == 6) {
+    EDL_CHECK(up_col_ok(V, C, sizeof(T), col), "launch_pass_up: no two-hop bank-scheduled table for this form");
+    if constexpr ((sizeof(T) == 8 && C == 1) || (sizeof(T) == 4 && (C == 1 || C == 3))) {
+      if (col == 5) return launch_up_p<T, C, RPT, UKCH, false>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+      return launch_up_p<T, C, RPT, UKCH, true>(V, I, lut, qa, qb, tout, L, grid, threads, st);
+    }
+  }
   if (col == 4) {
     EDL_CHECK(up_col_ok(V, C, sizeof(T), 4), "launch_pass_up: no sign-split two-hop table for this sector and width");
     if constexpr ((sizeof(T) == 8 && C == 1) || (sizeof(T) == 4 && (C == 1 || C == 3)))

@@ -258,7 +258,8 @@ def test_hxv_segmented_up_pass_vs_oracle(ed, orc, cfg, sec, rows, rank, cols, mo
     (1, 8, 128, 2, 2, 1, 1, 3), (2, 4, 64, 1, 4, 1, 1, 3),
     (1, 8, 128, 2, 2, 1, 2, 3), (2, 4, 64, 1, 4, 1, 2, 3), (1, 16, 64, 2, 8, 0, 2, 1), (2, 8, 32, 1, 16, 0, 2, 1),
     (1, 8, 128, 2, 2, 1, 3, 3), (2, 4, 64, 1, 4, 1, 3, 2), (1, 16, 64, 2, 8, 0, 3, 1),
-    (1, 8, 128, 2, 2, 1, 4, 3), (2, 4, 64, 1, 4, 1, 4, 1), (2, 8, 32, 2, 8, 0, 4, 3)])
+    (1, 8, 128, 2, 2, 1, 4, 3), (2, 4, 64, 1, 4, 1, 4, 1), (2, 8, 32, 2, 8, 0, 4, 3),
+    (1, 16, 128, 2, 2, 1, 5, 3), (2, 4, 64, 1, 4, 1, 5, 1), (1, 16, 64, 2, 2, 1, 6, 3), (1, 8, 32, 1, 4, 1, 6, 1)])
 def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, C, monkeypatch):
     """Schedules pinned through the override, among them the vector-gather
     down pass (16-byte lanes, panels of 16 to 256 rows; the 8-site ring's
@@ -266,15 +267,17 @@ def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, 
     pass (col = 1, fp32, and col = 2, its form with two hops to a table word
     and zero rows for the unused slots; the last 64-row block is partial on
     every sector here), the table-order two-hop table (col = 3, fp32 and
-    fp64), and the bank-scheduled two-hop words split by sign (col = 4, one
-    coefficient per word; fp32 and fp64): H x v on three sectors and a
-    mixed-precision ground state (the chain-mode passes) against the
-    oracle."""
+    fp64), the bank-scheduled two-hop words split by sign (col = 4, one
+    coefficient per word; fp32 and fp64), and forms 2 and 4 with each
+    block's next panel loaded into registers before the barrier that ends
+    the current one (col = 5, 6; several panels per block on these grids):
+    H x v on three sectors and a mixed-precision ground state (the
+    chain-mode passes) against the oracle."""
     tail = f"{R},256,32,{dn},{dk},{rpt},{kch},{vec}"
     tail64 = f"{min(R, 128)},256,32,{dn},{dk},{rpt},{kch},{vec}"  # 256-row panels are fp32 only
     monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"{C},1024,256," + tail + f",{col}")
     # fp64 (one column): the two-hop tables, bank-scheduled over 16 bank pairs (2) or in table order (3)
-    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", "1,512,512," + tail64 + (f",{col}" if col in (2, 3, 4) else ""))
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", "1,512,512," + tail64 + (f",{col}" if col >= 2 else ""))
     for cfg, sec in [("ring8", (4, 4)), ("rect3x4", (7, 6)), ("rect3x4", (6, 5))]:
         c = lattice.CONFIGS[cfg]
         m = _use(ed, orc, c["H"](), [c["U"]])
@@ -285,7 +288,7 @@ def test_hxv_pinned_schedules_vs_oracle(ed, orc, rpt, kch, R, dn, dk, vec, col, 
             sched = ed.bench_passes(isec, prec, 1)[2]
             rows = 128 if (R == 256 and prec == "fp64") else R
             assert (sched["dw_rows"], sched["dw_vector_gather"], sched["up_rows_per_thread"]) == (rows, vec, rpt)
-            assert sched["up_bank_scheduled"] == (col if prec == "fp32" or col in (2, 3, 4) else 0)
+            assert sched["up_bank_scheduled"] == (col if prec == "fp32" or col >= 2 else 0)
         v = RNG.standard_normal(dim)
         ref = orc.matvec(m, *sec, v, nthreads=8).real
         assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
