@@ -77,7 +77,9 @@ struct TwoPassCfg {
   int up_col = 0;       // 1: the up pass reads the bank-scheduled hop table (fp32, C 1 or 3);
                         // 2: its form with two hops to a word (one hopping magnitude);
                         // 3: the table order two hops to a word (any precision and width);
-                        // 4: two hops to a word bank-scheduled by sign (as 2, no per-hop sign)
+                        // 4: two hops to a word bank-scheduled by sign (as 2, no per-hop sign);
+                        // 5, 6: forms 2, 4 with the block's next panel preloaded into registers
+                        //    (operator mode; DimUp <= 13 * threads_up)
   int R = 64;           // rows per down-pass panel (32 or 64)
   int threads_dw = 256;
   int nchunk = 32;      // column chunks per down-pass panel (one block each)

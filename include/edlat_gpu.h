@@ -204,8 +204,10 @@ int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_
  * (0), up-pass hop coefficients from registers (1: fp32, one distinct
  * hopping value) or from the LDS table (0), up-pass hops in the bank-scheduled
  * order (1: a hop to a table word, 2: two hops to a word, 4: two hops to a
- * word scheduled separately for each sign of h) or in table order (0: a hop
- * to a word, 3: two hops to a word) -- on segmented sectors (cfg[0]
+ * word scheduled separately for each sign of h; 5, 6: forms 2, 4 with each
+ * block's next panel loaded into registers before the barrier that ends the
+ * current one) or in table order (0: a hop to a word, 3: two hops to a
+ * word) -- on segmented sectors (cfg[0]
  * = 2) this slot holds the form of the segmented up pass instead: 1 = hop
  * targets by rank arithmetic, 0 = the hop table --, down pass over column
  * ranges of that many top sites (b > 0, range-blocked form) or over column
