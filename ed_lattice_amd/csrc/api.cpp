@@ -1045,15 +1045,16 @@ static int run_steps_bt(Ctx& c, const BtSet& bs, int steps, hipStream_t st) {
   return steps;
 }
 
-// Chain groups of at least this many chains run as two halves on two
-// streams (the halves' steps overlap: one half's down tiles share the GPU
-// with the other half's up tiles); EDL_BATCHED_STREAMS=1|2 forces one form.
+// Chain groups of at least this many chains run as parts on several
+// streams (the parts' steps overlap: one part's down tiles share the GPU
+// with another part's up tiles); EDL_BATCHED_STREAMS=1..4 forces a count.
 static constexpr size_t kBtTwoStreamChains = 64;
+static constexpr int kBtMaxStreams = 4;
 static int bt_streams(size_t nchains) {
   if (const char* e = std::getenv("EDL_BATCHED_STREAMS")) {
     const int v = std::atoi(e);
-    EDL_CHECK(v == 1 || v == 2, "EDL_BATCHED_STREAMS must be 1 or 2");
-    return nchains >= 2 ? v : 1;
+    EDL_CHECK(v >= 1 && v <= kBtMaxStreams, "EDL_BATCHED_STREAMS must be 1 .. 4");
+    return (int)std::min<size_t>((size_t)v, std::max<size_t>(nchains, 1));
   }
   return nchains >= kBtTwoStreamChains ? 2 : 1;
 }
@@ -1077,27 +1078,33 @@ struct AuxStream {
   }
 };
 
-// `steps` steps of two independent chain sets, each on its own stream and
-// replayed from its own graph, launched alternately; `st` waits for both.
+// `steps` steps of independent chain sets, part 0 on `st` and each other
+// part on a stream of its own, each replayed from its own graph, launched
+// in turn; `st` waits for all of them.
 template <typename T>
-static int run_steps_bt2(Ctx& c, const BtSet& a, const BtSet& b, int steps, hipStream_t st) {
-  AuxStream aux;
-  EDL_HIP(hipEventRecord(aux.e, st));  // the chains' set-up on st precedes both halves
-  EDL_HIP(hipStreamWaitEvent(aux.s, aux.e, 0));
-  StepRunner ra(st, [&]() {
-    bt_launch_up<T>(a, c.model.inter, st);
-    bt_launch_dw<T>(a, st);
-  });
-  StepRunner rb(aux.s, [&]() {
-    bt_launch_up<T>(b, c.model.inter, aux.s);
-    bt_launch_dw<T>(b, aux.s);
-  });
-  for (int k = 0; k < steps; ++k) {
-    ra.launch();
-    rb.launch();
+static int run_steps_btn(Ctx& c, const std::vector<BtSet>& parts, int steps, hipStream_t st) {
+  const size_t n = parts.size();
+  EDL_CHECK(n >= 1 && n <= (size_t)kBtMaxStreams, "run_steps_btn: 1 .. 4 chain sets");
+  std::vector<std::unique_ptr<AuxStream>> aux;
+  for (size_t i = 1; i < n; ++i) aux.emplace_back(new AuxStream());
+  hipEvent_t e0 = aux.empty() ? nullptr : aux[0]->e;
+  if (e0) EDL_HIP(hipEventRecord(e0, st));  // the chains' set-up on st precedes every part
+  for (auto& a : aux) EDL_HIP(hipStreamWaitEvent(a->s, e0, 0));
+  std::vector<std::unique_ptr<StepRunner>> run;
+  for (size_t i = 0; i < n; ++i) {
+    const BtSet* bs = &parts[i];
+    hipStream_t s = i ? aux[i - 1]->s : st;
+    run.emplace_back(new StepRunner(s, [&c, bs, s]() {
+      bt_launch_up<T>(*bs, c.model.inter, s);
+      bt_launch_dw<T>(*bs, s);
+    }));
   }
-  EDL_HIP(hipEventRecord(aux.e, aux.s));
-  EDL_HIP(hipStreamWaitEvent(st, aux.e, 0));
+  for (int k = 0; k < steps; ++k)
+    for (auto& r : run) r->launch();
+  for (auto& a : aux) {
+    EDL_HIP(hipEventRecord(a->e, a->s));
+    EDL_HIP(hipStreamWaitEvent(st, a->e, 0));
+  }
   EDL_HIP(hipStreamSynchronize(st));
   return steps;
 }
@@ -2212,7 +2219,8 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   ChainSet<T> cs;
   cs.h = chains;
   BtShape shp{};
-  BtSet bs, half[2];
+  BtSet bs;
+  std::vector<BtSet> parts;
   bool persist = false;
   int nstreams = 1;
   if (bt) {
@@ -2227,11 +2235,11 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
       bt_finalize<T>(bs);
     } else {
       const std::vector<BtChain> all = bt_resized(btchains, shp);
-      const size_t m = all.size() / 2;
-      for (int h = 0; h < 2; ++h) {
-        half[h].shape = shp;
-        half[h].h.assign(all.begin() + (h ? m : 0), h ? all.end() : all.begin() + m);
-        bt_finalize<T>(half[h]);
+      parts.resize(nstreams);
+      for (int h = 0; h < nstreams; ++h) {
+        parts[h].shape = shp;
+        parts[h].h.assign(all.begin() + all.size() * h / nstreams, all.begin() + all.size() * (h + 1) / nstreams);
+        bt_finalize<T>(parts[h]);
       }
     }
   }
@@ -2239,8 +2247,8 @@ static int64_t tridiag_group(Ctx& c, const std::vector<SeedSpec>& seeds, const s
   if (bt && persist) {
     bt_launch_persist<T>(bs, c.model.inter, st);
     EDL_HIP(hipStreamSynchronize(st));
-  } else if (bt && nstreams == 2) {
-    run_steps_bt2<T>(c, half[0], half[1], steps, st);
+  } else if (bt && nstreams > 1) {
+    run_steps_btn<T>(c, parts, steps, st);
   } else if (bt) {
     run_steps_bt<T>(c, bs, steps, st);
   } else if (alone) {

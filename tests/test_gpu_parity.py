@@ -704,24 +704,26 @@ def test_batched_up_tile_table_forms_vs_oracle(ed, orc, h16, monkeypatch):
             _assert_leading(rb["alpha"][k], rb["beta"][k], ao, bo, nk, tol)
 
 
-def test_two_stream_halves_equal_one_stream(ed, orc, monkeypatch):
-    """The batched tile path splits a chain group over two streams (each
-    half replayed from its own graph, the halves' launches interleaved):
-    bit-identical alpha, beta and norm2 to one stream, on both spins of the
-    3x4 seeds (two sector shapes in each half)."""
+def test_stream_parts_equal_one_stream(ed, orc, monkeypatch):
+    """The batched tile path splits a chain group over two to four streams
+    (each part replayed from its own graph, the parts' launches
+    interleaved): bit-identical alpha, beta and norm2 to one stream, on 16
+    chains over both spins of the 3x4 seeds (four target sectors of two
+    shapes; three parts of 5, 5 and 6 chains)."""
     monkeypatch.setenv("EDL_BATCHED_KERNELS", "tiles")
     _use(ed, orc, lattice.square(3, 4), [6.0])
     ed.lanczos_eigh(ed.sector_index(6, 6), "fp64", niter=200, want_vector=False)
     seeds = [(op, sp, i, j) for sp in (1, 2) for i, j in ((1, 0), (2, 7), (5, 0), (3, 12)) for op in ("cdg", "c")]
     res = {}
-    for n in ("1", "2"):
+    for n in ("1", "2", "3", "4"):
         monkeypatch.setenv("EDL_BATCHED_STREAMS", n)
         for prec in ("fp64", "fp32"):
             rb = ed.tridiag_batched(seeds, 40, prec)
             res[(n, prec)] = (rb["alpha"], rb["beta"], rb["norm2"])
-    for prec in ("fp64", "fp32"):
-        for a, b in zip(res[("1", prec)], res[("2", prec)]):
-            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    for n in ("2", "3", "4"):
+        for prec in ("fp64", "fp32"):
+            for a, b in zip(res[("1", prec)], res[(n, prec)]):
+                np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
 
 
 @pytest.mark.parametrize("route", ["tiles", "single", "chain"])
