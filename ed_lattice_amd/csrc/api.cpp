@@ -40,6 +40,7 @@ Sector& get_sector(Ctx& c, int isector) {
   try {
     build_up_color_tables(*s, c.stream);
     build_dw_range_tables(*s, c.stream);
+    build_dw_order(*s, c.stream);
   } catch (...) {
     free_sector(*s);
     throw;
@@ -182,6 +183,7 @@ static void with_chain_shapes(TwoPassCfg& dst, const TwoPassCfg& src) {
   dst.ch_up_kch = src.ch_up_kch;
   dst.ch_up_col = src.ch_up_col;
   dst.ch_rng = src.ch_rng;
+  dst.ch_ord = src.ch_ord;
 }
 
 // Column-chunk counts the down-pass tuner tries: fixed ones, and for the
@@ -210,15 +212,18 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   TwoPassCfg best;
   bool pinned = false;
   // EDL_TWO_PASS_CFG_F32 / _F64 =
-  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]"
+  // "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng[,dw_ord]]]]]]"
   // pins the operator schedule of that precision (reproducible profiles of
   // one configuration), ahead of any tuned schedule; the operator is not
   // tuned then, its chain shapes still are (once per sector shape).
   if (const char* e = std::getenv(sizeof(T) == 4 ? "EDL_TWO_PASS_CFG_F32" : "EDL_TWO_PASS_CFG_F64")) {
     TwoPassCfg p;
-    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up, &p.R,
-                              &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec,
-                              &p.up_col, &p.dw_rng);
+    const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%d", &p.C, &p.threads_up, &p.grid_up,
+                              &p.R, &p.threads_dw, &p.nchunk, &p.ncol, &p.kch, &p.up_rpt, &p.up_kch, &p.dw_vec,
+                              &p.up_col, &p.dw_rng, &p.dw_ord);
+    EDL_CHECK(p.dw_ord == 0 || (p.dw_ord == 1 && p.dw_vec == 1 && p.dw_rng == 0 && V.ord_dw),
+              "EDL_TWO_PASS_CFG_F32/_F64: dw_ord 1 needs the vector-gather down pass and a sector with the column "
+              "order (down spin of at least 256 strings)");
     EDL_CHECK(p.dw_rng == 0 || pass_dwr_ok(V, p.R, p.dw_rng, sizeof(T)),
               "EDL_TWO_PASS_CFG_F32/_F64: dw_rng needs range tables for that many top sites and R 32|64 within LDS");
     EDL_CHECK(p.up_col == 0 || up_col_ok(V, p.C, sizeof(T), p.up_col),
@@ -226,7 +231,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               "bank-scheduled table, up_col 2 and 4 also one hopping magnitude and DimUp + 32 < 2^15 (fp64: "
               "one column), up_col 3 one hopping magnitude and DimUp + 32 < 2^15");
     EDL_CHECK(n >= 6, "EDL_TWO_PASS_CFG_F32/_F64: expected "
-                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng]]]]]");
+                      "C,threads_up,grid_up,R,threads_dw,nchunk[,ncol,kch[,up_rpt,up_kch[,dw_vec[,up_col[,dw_rng[,"
+                      "dw_ord]]]]]]");
     EDL_CHECK((p.up_rpt == 1 && (p.up_kch == 8 || p.up_kch == 16)) || (p.up_rpt == 2 && (p.up_kch == 4 || p.up_kch == 8)),
               "EDL_TWO_PASS_CFG_F32/_F64: (up_rpt, up_kch) must be (1, 8|16) or (2, 4|8)");
     EDL_CHECK(p.dw_rng > 0 || pass_dw_lds_bytes(V, p) <= (size_t)(160 * 1024 - 4096),
@@ -258,17 +264,19 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
   // The chain shapes do not depend on the operator's: a pinned operator
   // schedule still runs its chains on their own tuned shapes.
   auto ct = tp_chain_cache().find(key);
-  // A pinned range-blocked down pass (dw_rng > 0) pins the chain's down
-  // pass to the same form, so that its Lanczos terms are tested pinned too.
+  // A pinned range-blocked down pass (dw_rng > 0) or one in the locality
+  // order of columns (dw_ord = 1) pins the chain's down pass to the same
+  // form, so that its Lanczos terms are tested pinned too.
   auto pin_chain_range = [&](TwoPassCfg& b) {
-    if (!pinned || b.dw_rng == 0) return;
+    if (!pinned || (b.dw_rng == 0 && b.dw_ord == 0)) return;
     b.ch_R = b.R;
     b.ch_threads = b.threads_dw;
     b.ch_nchunk = b.nchunk;
     b.ch_ncol = b.ncol;
     b.ch_kch = b.kch;
-    b.ch_vec = 0;
+    b.ch_vec = b.dw_rng > 0 ? 0 : b.dw_vec;
     b.ch_rng = b.dw_rng;
+    b.ch_ord = b.dw_ord;
   };
   if (pinned && ct != tp_chain_cache().end()) {
     with_chain_shapes(best, ct->second);
@@ -321,23 +329,26 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             best.grid_up = w.grid_up;
           }
         }
-    // 4-byte (8-byte) gathers, then 16-byte vector gathers
-    for (int vec : {0, 1})
+    // 4-byte (8-byte) gathers, then 16-byte vector gathers (columns in the
+    // natural order, then in the locality order)
+    for (int vo : {0, 1, 2})
     for (int R : {16, 32, 64, 128, 256})
       for (int nk : {108, 116, 132, 208, 216, 102, 104, 201, 202, 204})
         for (int th : {256, 512, 1024})
-          for (int nch : dw_chunk_counts(V.dimdw, vec ? (nk / 100) * (th / 64) : 0)) {
+          for (int nch : dw_chunk_counts(V.dimdw, vo ? (nk / 100) * (th / 64) : 0)) {
           if (nch > V.dimdw) continue;
           const int nc = nk / 100, kc = nk % 100;
           const bool vk = kc <= 4;
-          if (vk != (vec == 1) || (!vec && (R == 16 || R > 64)) || (R == 256 && sizeof(T) != 4)) continue;
+          if (vk != (vo >= 1) || (!vo && (R == 16 || R > 64)) || (R == 256 && sizeof(T) != 4)) continue;
+          if (vo == 2 && !V.ord_dw) continue;
           TwoPassCfg w = best;
           w.R = R;
           w.ncol = nc;
           w.kch = kc;
           w.threads_dw = th;
           w.nchunk = nch;
-          w.dw_vec = vec;
+          w.dw_vec = vo >= 1;
+          w.dw_ord = vo == 2;
           if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
           launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
           EventTimer tm(st);
@@ -345,8 +356,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), q.as<T>(), t.as<T>(), o.as<T>(), nullptr, w, st);
           const double ms = tm.stop();
           if (verbose())
-            std::fprintf(stderr, "[edl] tune dw  fp%d vec %d R %3d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
-                         8 * (int)sizeof(T), vec, R, nc, kc, th, nch, ms / 3);
+            std::fprintf(stderr, "[edl] tune dw  fp%d vec %d ord %d R %3d ncol %d kch %2d threads %4d chunks %3d: %.4f ms\n",
+                         8 * (int)sizeof(T), w.dw_vec, w.dw_ord, R, nc, kc, th, nch, ms / 3);
           if (ms < best_dw) {
             best_dw = ms;
             best.R = R;
@@ -354,7 +365,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             best.kch = kc;
             best.threads_dw = th;
             best.nchunk = nch;
-            best.dw_vec = vec;
+            best.dw_vec = w.dw_vec;
+            best.dw_ord = w.dw_ord;
             best.dw_rng = 0;
           }
         }
@@ -380,6 +392,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             best.R = R;
             best.threads_dw = th;
             best.dw_rng = b;
+            best.dw_ord = 0;
           }
         }
   }
@@ -419,11 +432,12 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
     A.part_dw = part.as<double>();
     A.nmax = 4;
     double bc = 1e300;
+    for (int ord : {0, 1})
     for (int R : {64, 128})
       for (int nk : {102, 104, 201, 202, 204})
         for (int th : {256, 512, 1024})
           for (int nch : dw_chunk_counts(V.dimdw, (nk / 100) * (th / 64), false)) {
-            if (nch > V.dimdw) continue;
+            if (nch > V.dimdw || (ord && !V.ord_dw)) continue;
             TwoPassCfg w = best;
             w.R = R;
             w.ncol = nk / 100;
@@ -431,6 +445,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             w.threads_dw = th;
             w.nchunk = nch;
             w.dw_vec = 1;
+            w.dw_rng = 0;
+            w.dw_ord = ord;
             if (pass_dw_lds_bytes(V, w) > (size_t)lds - 4096) continue;
             launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
             EventTimer tm(st);
@@ -438,8 +454,8 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               launch_pass_dw<T>(V, lut_dev<T>(s.sp[1]), q.as<T>(), o.as<T>(), t.as<T>(), nullptr, &A, w, st);
             const double ms = tm.stop();
             if (verbose())
-              std::fprintf(stderr, "[edl] tune dw chain fp%d R %3d ncol %d kch %d threads %4d chunks %3d: %.4f ms\n",
-                           8 * (int)sizeof(T), R, w.ncol, w.kch, th, nch, ms / 3);
+              std::fprintf(stderr, "[edl] tune dw chain fp%d ord %d R %3d ncol %d kch %d threads %4d chunks %3d: %.4f ms\n",
+                           8 * (int)sizeof(T), ord, R, w.ncol, w.kch, th, nch, ms / 3);
             if (ms < bc) {
               bc = ms;
               best.ch_R = R;
@@ -449,6 +465,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
               best.ch_kch = w.kch;
               best.ch_vec = 1;
               best.ch_rng = 0;
+              best.ch_ord = ord;
             }
           }
     for (int b = 1; b <= kMaxRangeBits; ++b)
@@ -474,6 +491,7 @@ static TwoPassCfg two_pass_cfg(Ctx& c, const Sector& s) {
             best.ch_threads = th;
             best.ch_rng = b;
             best.ch_vec = 0;
+            best.ch_ord = 0;
           }
         }
     // The chain step's up pass with phase B pending and alpha = 0 (r = t,
@@ -612,12 +630,15 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
   if (sizeof(T) == 4) shapes.push_back({256, 512, fit16, 1});
   double bd = 1e300;
   TwoPassCfg dw = best;
+  for (int ord : {0, 1})  // vector gathers: columns in the natural, then the locality order
   for (const auto& sh : shapes) {
+    if (ord && (!sh[3] || !V.ord_dw)) continue;
     TwoPassCfg w = best;
     w.R = sh[0];
     w.threads_dw = sh[1];
     w.nchunk = std::max(1, std::min(sh[2], s.sp[1].dim));
     w.dw_vec = sh[3];
+    w.dw_ord = ord;
     w.ncol = 2;
     w.kch = sh[3] ? 2 : 8;
     if (pass_dw_lds_bytes(V, w) > (size_t)(160 * 1024 - 4096)) continue;
@@ -628,8 +649,8 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
       ms = tm.stop();
     }
     if (verbose())
-      std::fprintf(stderr, "[edl] segmented sector down pass vec %d R %d x %d threads, %d chunks: %.3f ms\n",
-                   w.dw_vec, w.R, w.threads_dw, w.nchunk, ms);
+      std::fprintf(stderr, "[edl] segmented sector down pass vec %d ord %d R %d x %d threads, %d chunks: %.3f ms\n",
+                   w.dw_vec, w.dw_ord, w.R, w.threads_dw, w.nchunk, ms);
     if (ms < bd) {
       bd = ms;
       dw = w;
@@ -664,7 +685,9 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
     A.part_dw = part.as<double>();
     A.nmax = 4;
     double bc = 1e300;
+    for (int ord : {0, 1})
     for (const auto& sh : cshapes) {
+      if (ord && !V.ord_dw) continue;
       TwoPassCfg w = dw;
       w.R = sh[0];
       w.ncol = sh[1];
@@ -672,6 +695,7 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
       w.threads_dw = sh[3];
       w.nchunk = std::max(1, std::min(sh[4], s.sp[1].dim));
       w.dw_vec = 1;
+      w.dw_ord = ord;
       if (pass_dw_lds_bytes(V, w) > (size_t)(160 * 1024 - 4096)) continue;
       double ms = 0.0;
       for (int r = 0; r < 2; ++r) {
@@ -680,8 +704,8 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
         ms = tm.stop();
       }
       if (verbose())
-        std::fprintf(stderr, "[edl] segmented sector chain down pass R %d ncol %d kch %d x %d threads, %d chunks: %.3f ms\n",
-                     w.R, w.ncol, w.kch, w.threads_dw, w.nchunk, ms);
+        std::fprintf(stderr, "[edl] segmented sector chain down pass ord %d R %d ncol %d kch %d x %d threads, %d chunks: %.3f ms\n",
+                     ord, w.R, w.ncol, w.kch, w.threads_dw, w.nchunk, ms);
       if (ms < bc) {
         bc = ms;
         dw.ch_R = w.R;
@@ -690,6 +714,7 @@ static TwoPassCfg seg_cfg(Ctx& c, Sector& s) {
         dw.ch_ncol = w.ncol;
         dw.ch_kch = w.kch;
         dw.ch_vec = 1;
+        dw.ch_ord = ord;
       }
     }
   }
@@ -2557,7 +2582,7 @@ static void bench_passes_impl(Ctx& c, Sector& s, int reps, double* ms_pass, int3
   const int32_t lreg = (two && s.sp[0].nvals == 1 && (sizeof(T) == 4 || cf.up_col >= 2)) ? 1 : 0;
   const int32_t out[16] = {two ? 1 : (seg ? 2 : 0), ld,      cf.C,    cf.threads_up, cf.grid_up, cf.R,
                            cf.threads_dw,            cf.nchunk, cf.ncol, cf.kch,        cf.up_rpt,  cf.up_kch,
-                           cf.dw_vec,                lreg,      two ? cf.up_col : cf.seg_rank, cf.dw_rng};
+                           cf.dw_vec ? 1 + cf.dw_ord : 0, lreg, two ? cf.up_col : cf.seg_rank, cf.dw_rng};
   for (int k = 0; k < 16; ++k) cfg[k] = (two || seg) ? out[k] : (k < 2 ? out[k] : 0);
 }
 

@@ -171,6 +171,15 @@ struct SpinTable {
   // the ranges are at least a few columns wide.
   RangeTable rng[kMaxRangeBits + 1];
   RankHops rk;
+  // The columns of the down spin in a locality order (Cuthill-McKee over the
+  // hop graph: breadth first from a string of least degree, neighbours by
+  // increasing degree), so that columns near in the order gather from
+  // columns near in the order: ord[k] is the k-th column, and csro[offo[k] ..
+  // offo[k+1]) its packed hop words (as csr).  Built with the sector for the
+  // down spin (dim >= 256, with the packed table), else null.
+  int32_t* ord = nullptr;
+  uint32_t* csro = nullptr;
+  int32_t* offo = nullptr;
 };
 
 struct Sector {

@@ -16,6 +16,9 @@
 // Hop tables are packed words nbr | cid << 24 with the signed coefficient
 // lut[cid] kept in LDS.  Pass D carries the Lanczos phase-A fusion
 // (- beta v_{k-1}, alpha partial sums) when given a chain.
+#include <algorithm>
+#include <numeric>
+
 #include "chains.hpp"
 #include "hxv2.hpp"
 
@@ -623,7 +626,11 @@ __global__ __launch_bounds__(1024) void k_pass_dw(TwoPassView V, const T* __rest
 // and it admits panels taller than a wave (longer neighbour segments, the
 // shape that ran fastest).  Same block mapping, modes and LDS column table
 // as k_pass_dw.  A quad that straddles DimUp loads rows of the column's
-// padding (inside the column) and stores only rows < DimUp.
+// padding (inside the column) and stores only rows < DimUp.  With V.ord_dw
+// set, chunk positions are taken in the locality order: position k is
+// column ord_dw[k] (staged in LDS with the chunk's words, which come from the
+// reordered table), so the columns a panel's concurrent blocks work on
+// gather from a narrower window of the panel.
 template <typename T, int R, int MODE, int NCOL, int KCH>
 __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __restrict__ lut_dw, T* qa, T* qb,
                                                    const T* __restrict__ tin, T* outp, LzChain A, int nchunk) {
@@ -661,10 +668,15 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
   extern __shared__ __align__(16) unsigned char dsm[];
   uint32_t* tbl = reinterpret_cast<uint32_t*>(dsm);
   const int ncol = ce - cb;
-  const int w0 = V.off_dw[cb], nw = V.off_dw[ce] - w0;
+  const int32_t* __restrict__ ord = V.ord_dw;
+  const uint32_t* __restrict__ csr = ord ? V.csro_dw : V.csr_dw;
+  const int32_t* __restrict__ offs = ord ? V.offo_dw : V.off_dw;
+  const int w0 = offs[cb], nw = offs[ce] - w0;
   int* toff = reinterpret_cast<int*>(tbl + nw);
-  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = __builtin_nontemporal_load(&V.csr_dw[w0 + i]);
-  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = __builtin_nontemporal_load(&V.off_dw[cb + c]) - w0;
+  int* tcol = toff + ncol + 1;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) tbl[i] = __builtin_nontemporal_load(&csr[w0 + i]);
+  for (int c = threadIdx.x; c <= ncol; c += blockDim.x) toff[c] = __builtin_nontemporal_load(&offs[cb + c]) - w0;
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) tcol[c] = ord ? __builtin_nontemporal_load(&ord[cb + c]) : cb + c;
   __syncthreads();
   const double inv_s = (MODE == 1) ? 1.0 / s.s_cur : 1.0;
   const T ts = (T)inv_s;
@@ -677,7 +689,7 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
     const int nval = ok ? min(VW, dimup - iup0) : 0;  // rows of the quad below DimUp
     const bool lead = ok && g == 0;
     for (int c0 = cb + wave; c0 < ce; c0 += NCOL * nwaves) {
-      int col[NCOL], nh[NCOL];
+      int col[NCOL], nh[NCOL], cx[NCOL];
       bool okc[NCOL];
       VT t0[NCOL], o0[NCOL], q0v[NCOL], acc[NCOL];
       int nmax = 0;
@@ -686,9 +698,10 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
         col[h] = c0 + h * nwaves;
         okc[h] = col[h] < ce;
         if (!okc[h]) col[h] = c0;
+        cx[h] = tcol[col[h] - cb];  // the column at this position
         nh[h] = okc[h] ? toff[col[h] - cb + 1] - toff[col[h] - cb] : 0;
         nmax = nh[h] > nmax ? nh[h] : nmax;
-        const size_t i = (size_t)col[h] * ld + iuc;
+        const size_t i = (size_t)cx[h] * ld + iuc;
         const bool live = lead && okc[h];
         t0[h] = live ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(&tin[i])) : (VT)0;
         o0[h] = (MODE == 1 && live) ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(&out[i])) : (VT)0;
@@ -732,7 +745,7 @@ __global__ __launch_bounds__(1024) void k_pass_dwv(TwoPassView V, const T* __res
 #pragma unroll
         for (int h = 0; h < NCOL; ++h) {
           if (!okc[h]) continue;
-          const size_t i = (size_t)col[h] * ld + iup0;
+          const size_t i = (size_t)cx[h] * ld + iup0;
           VT tv;
           if (MODE == 0) {
             tv = t0[h] + acc[h];
@@ -1337,6 +1350,77 @@ void build_dw_range_tables(Sector& s, hipStream_t st) {
   }
 }
 
+void build_dw_order(Sector& s, hipStream_t st) {
+  SpinTable& t = s.sp[1];
+  if (!t.packed_ok || t.dim < 256 || t.ord) return;
+  const int dim = t.dim;
+  std::vector<int32_t> off(dim + 1);
+  EDL_HIP(hipMemcpyAsync(off.data(), t.off, sizeof(int32_t) * (dim + 1), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  std::vector<uint32_t> csr((size_t)off[dim]);
+  if (!csr.empty())
+    EDL_HIP(hipMemcpyAsync(csr.data(), t.csr, sizeof(uint32_t) * csr.size(), hipMemcpyDeviceToHost, st));
+  EDL_HIP(hipStreamSynchronize(st));
+  // Cuthill-McKee: breadth first over the hop graph, each component from a
+  // string of least degree, a string's unvisited neighbours by increasing
+  // degree (then index)
+  auto deg = [&](int j) { return off[j + 1] - off[j]; };
+  std::vector<int32_t> ord;
+  ord.reserve(dim);
+  std::vector<char> seen(dim, 0);
+  std::vector<int32_t> starts(dim);
+  std::iota(starts.begin(), starts.end(), 0);
+  std::stable_sort(starts.begin(), starts.end(), [&](int a, int b) { return deg(a) < deg(b); });
+  std::vector<int32_t> nb;
+  for (int st0 : starts) {
+    if (seen[st0]) continue;
+    seen[st0] = 1;
+    size_t head = ord.size();
+    ord.push_back(st0);
+    while (head < ord.size()) {
+      const int u = ord[head++];
+      nb.clear();
+      for (int k = off[u]; k < off[u + 1]; ++k) {
+        const int v = (int)(csr[k] & 0xFFFFFFu);
+        if (!seen[v]) {
+          seen[v] = 1;
+          nb.push_back(v);
+        }
+      }
+      std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) != deg(b) ? deg(a) < deg(b) : a < b; });
+      ord.insert(ord.end(), nb.begin(), nb.end());
+    }
+  }
+  EDL_CHECK((int)ord.size() == dim, "build_dw_order: the order does not cover the strings");
+  std::vector<int32_t> offo(dim + 1, 0);
+  std::vector<uint32_t> csro(csr.size());
+  for (int k = 0; k < dim; ++k) {
+    const int j = ord[k];
+    offo[k + 1] = offo[k] + deg(j);
+    std::copy(csr.begin() + off[j], csr.begin() + off[j + 1], csro.begin() + offo[k]);
+  }
+  int32_t* d_ord = nullptr;
+  uint32_t* d_csro = nullptr;
+  int32_t* d_offo = nullptr;
+  try {
+    EDL_HIP(hipMalloc(&d_ord, sizeof(int32_t) * dim));
+    EDL_HIP(hipMalloc(&d_csro, sizeof(uint32_t) * std::max<size_t>(csro.size(), 1)));
+    EDL_HIP(hipMalloc(&d_offo, sizeof(int32_t) * (dim + 1)));
+    EDL_HIP(hipMemcpyAsync(d_ord, ord.data(), sizeof(int32_t) * dim, hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(d_csro, csro.data(), sizeof(uint32_t) * csro.size(), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipMemcpyAsync(d_offo, offo.data(), sizeof(int32_t) * (dim + 1), hipMemcpyHostToDevice, st));
+    EDL_HIP(hipStreamSynchronize(st));  // the host vectors must outlive the copies
+  } catch (...) {
+    if (d_ord) (void)hipFree(d_ord);
+    if (d_csro) (void)hipFree(d_csro);
+    if (d_offo) (void)hipFree(d_offo);
+    throw;
+  }
+  t.ord = d_ord;
+  t.csro = d_csro;
+  t.offo = d_offo;
+}
+
 static size_t up_lds_bytes(int dimup, int C, size_t sz) {
   const size_t head = 256 * sz;
   return head + (16 - head % 16) % 16 + (size_t)dimup * C * sz;
@@ -1368,6 +1452,9 @@ TwoPassView two_pass_view(const Sector& s, int ld) {
   v.dg_dw = s.sp[1].diag;
   v.csr_dw = s.sp[1].csr;
   v.off_dw = s.sp[1].off;
+  v.ord_dw = s.sp[1].ord;
+  v.csro_dw = s.sp[1].csro;
+  v.offo_dw = s.sp[1].offo;
   for (int W = 0; W < 4; ++W) {
     v.pkc_up[W] = s.sp[0].pkc[W].words;
     v.pkc_off[W] = s.sp[0].pkc[W].off;
@@ -1587,12 +1674,13 @@ static void launch_dw_n(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
   EDL_HIP(hipGetLastError());
 }
 
-// Dynamic LDS of a down-pass block: the chunk's hop words and their
-// offsets, bounded by ncmax * (kdw + 1) words, ncmax = ceil(dimdw / nchunk) + 1.
+// Dynamic LDS of a down-pass block: the chunk's hop words, their offsets
+// and (vector-gather form) its columns, bounded by ncmax * (kdw + 2) words,
+// ncmax = ceil(dimdw / nchunk) + 1 (a chunk has at most ncmax - 1 columns).
 size_t pass_dw_lds_bytes(const TwoPassView& V, const TwoPassCfg& cfg) {
   if (cfg.dw_rng > 0) return (size_t)-1;  // depends on the element size: see pass_dwr_ok
   const size_t ncmax = (size_t)((V.dimdw + cfg.nchunk - 1) / cfg.nchunk + 1);
-  return sizeof(uint32_t) * ncmax * (size_t)(V.kdw + 1);
+  return sizeof(uint32_t) * ncmax * (size_t)(V.kdw + 2);
 }
 
 template <typename T, int R, int MODE, int NCOL, int KCH>
@@ -1663,12 +1751,21 @@ static void launch_dw_r(const TwoPassView& V, const T* lut, T* qa, T* qb, const 
   EDL_CHECK(lds <= (size_t)lds_limit_bytes() - 4096, "launch_pass_dw: column table exceeds LDS");
   const int grid = pass_dw_grid(V, cfg);
   if (cfg.dw_vec) {
+    // the locality order of columns where asked for, else the natural one
+    TwoPassView Vo = V;
+    if (cfg.dw_ord) {
+      EDL_CHECK(V.ord_dw && V.csro_dw && V.offo_dw, "launch_pass_dw: no column order built for this sector");
+    } else {
+      Vo.ord_dw = nullptr;
+      Vo.csro_dw = nullptr;
+      Vo.offo_dw = nullptr;
+    }
     switch (cfg.ncol * 100 + cfg.kch) {
-      case 102: return launch_dwv_n<T, MODE, 1, 2>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
-      case 104: return launch_dwv_n<T, MODE, 1, 4>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
-      case 201: return launch_dwv_n<T, MODE, 2, 1>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
-      case 202: return launch_dwv_n<T, MODE, 2, 2>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
-      case 204: return launch_dwv_n<T, MODE, 2, 4>(V, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 102: return launch_dwv_n<T, MODE, 1, 2>(Vo, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 104: return launch_dwv_n<T, MODE, 1, 4>(Vo, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 201: return launch_dwv_n<T, MODE, 2, 1>(Vo, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 202: return launch_dwv_n<T, MODE, 2, 2>(Vo, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
+      case 204: return launch_dwv_n<T, MODE, 2, 4>(Vo, lut, qa, qb, tin, out, A, cfg, lds, grid, st);
       default: throw Error("launch_pass_dw: unsupported columns per wave / hop chunk of the vector-gather form");
     }
   }

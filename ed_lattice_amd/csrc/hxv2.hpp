@@ -18,6 +18,11 @@ struct TwoPassView {
   const double* dg_dw;
   const uint32_t* csr_dw;  // down-spin packed words grouped by column (no padding)
   const int32_t* off_dw;   // [dimdw+1] their offsets
+  // the down-pass column order (SpinTable::ord, csro, offo): position k of a
+  // chunk is column ord_dw[k], its words csro_dw[offo_dw[k] ..); null: k itself
+  const int32_t* ord_dw;
+  const uint32_t* csro_dw;
+  const int32_t* offo_dw;
   // bank-scheduled up-spin tables by staged row width W (SpinTable::pkc), or null
   const uint32_t* pkc_up[4];
   const int32_t* pkc_off[4];
@@ -91,9 +96,11 @@ struct TwoPassCfg {
                         //    (R 32|64, threads_dw; ncol, kch, nchunk, dw_vec unused)
   int seg_rank = 0;     // segmented sectors, 1: the up pass finds hop targets by rank arithmetic
                         //    (RankHops) instead of reading the hop table
+  int dw_ord = 0;       // 1: the vector-gather down pass takes its columns in the locality order
+                        //    (SpinTable::ord; the layout of the vectors is unchanged)
   // Down-pass shape of the Lanczos chains (the pass with the Lanczos terms
   // fused, MODE 1), tuned separately; ch_R = 0: the operator's shape.
-  int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0, ch_rng = 0;
+  int ch_R = 0, ch_threads = 0, ch_nchunk = 0, ch_ncol = 0, ch_kch = 0, ch_vec = 0, ch_rng = 0, ch_ord = 0;
   // Up-pass shape of the chains (phase B fused), likewise; ch_C = 0: the operator's.
   int ch_C = 0, ch_up_threads = 0, ch_grid_up = 0, ch_up_rpt = 0, ch_up_kch = 0, ch_up_col = 0;
 };
@@ -110,6 +117,7 @@ inline TwoPassCfg chain_cfg(const TwoPassCfg& c) {
     r.kch = c.ch_kch;
     r.dw_vec = c.ch_vec;
     r.dw_rng = c.ch_rng;
+    r.dw_ord = c.ch_ord;
   }
   if (c.ch_C > 0) {
     r.C = c.ch_C;
@@ -132,6 +140,9 @@ bool up_col_ok(const TwoPassView& V, int C, size_t elem, int col = 1);
 void build_up_color_tables(Sector& s, hipStream_t st);
 // Builds the down-spin column-range tables of a two-pass sector (once).
 void build_dw_range_tables(Sector& s, hipStream_t st);
+// Builds the down spin's locality order of columns and its reordered hop
+// table (SpinTable::ord, csro, offo; once).
+void build_dw_order(Sector& s, hipStream_t st);
 // Whether the range tables for b top sites exist and a block of R rows of
 // the largest range fits in LDS (slab, hop words, per-column offsets).
 bool pass_dwr_ok(const TwoPassView& V, int R, int b, size_t elem);

@@ -364,6 +364,9 @@ void free_sector(Sector& s) {
     if (t.cnt) (void)hipFree(t.cnt);
     if (t.csr) (void)hipFree(t.csr);
     if (t.off) (void)hipFree(t.off);
+    if (t.ord) (void)hipFree(t.ord);
+    if (t.csro) (void)hipFree(t.csro);
+    if (t.offo) (void)hipFree(t.offo);
     if (t.d_lut64) (void)hipFree(t.d_lut64);
     if (t.d_lut32) (void)hipFree(t.d_lut32);
     for (auto& ct : t.pkc) {

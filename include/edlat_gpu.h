@@ -200,8 +200,9 @@ int edl_bench_hxv(edl_ctx* ctx, int isector, int prec, int reps, double* ms_per_
  * columns larger than LDS, 0 = single pass), column stride, up columns per
  * block, up threads, up grid, down panel rows, down threads, down column
  * chunks, down columns per lane, down hop chunk, up rows per thread, up hop
- * chunk, down-pass gathers of 16 bytes per lane (1) or one element per lane
- * (0), up-pass hop coefficients from registers (1: fp32, one distinct
+ * chunk, down-pass gathers of 16 bytes per lane (1; 2: the same with the
+ * column chunks taken in the down spin's locality order) or one element per
+ * lane (0), up-pass hop coefficients from registers (1: fp32, one distinct
  * hopping value) or from the LDS table (0), up-pass hops in the bank-scheduled
  * order (1: a hop to a table word, 2: two hops to a word, 4: two hops to a
  * word scheduled separately for each sign of h; 5, 6: forms 2, 4 with each

@@ -329,6 +329,34 @@ def test_hxv_range_blocked_down_pass_vs_oracle(ed, orc, R, b, th, monkeypatch):
     assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
 
 
+@pytest.mark.parametrize("R,th,nch,dn,dk", [(128, 1024, 29, 2, 2), (64, 512, 32, 1, 4), (32, 256, 7, 2, 1)])
+def test_hxv_ordered_down_pass_vs_oracle(ed, orc, R, th, nch, dn, dk, monkeypatch):
+    """The vector-gather down pass taking its column chunks in the down
+    spin's locality order (Cuthill-McKee over the hop graph; the vectors keep
+    their layout), pinned: H x v on four sectors of the 3x4 cluster against
+    the oracle in fp64 and fp32 (padded and natural layouts), and a mixed
+    ground state whose chains run the same form with the Lanczos terms
+    fused (the pin carries over to the chain's down pass)."""
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"3,1024,256,{R},{th},{nch},{dn},{dk},1,8,1,1,0,1")
+    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", f"1,512,512,{R},{th},{nch},{dn},{dk},2,8,1,0,0,1")
+    m = _use(ed, orc, lattice.square(3, 4), [6.0])
+    for sec in ((7, 6), (6, 6), (5, 6), (6, 4)):
+        isec = ed.sector_index(*sec)
+        dim, du, dd = ed.build_sector(isec)
+        print(f"ordered down pass R {R} chunks {nch}: sector {sec} DimUp {du} DimDw {dd}", flush=True)
+        for prec in ("fp32", "fp64"):
+            assert ed.bench_passes(isec, prec, 1)[2]["dw_vector_gather"] == 2
+        v = RNG.standard_normal(dim)
+        ref = orc.matvec(m, *sec, v, nthreads=8).real
+        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
+        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+    isec = ed.sector_index(6, 6)
+    Hc = orc.hamiltonian_csr(m, 6, 6)
+    e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
+    rmx = ed.lanczos_eigh(isec, "mixed", niter=200, refine_tol=1e-13)
+    assert abs(rmx["e0"] - e0) < 1e-10 * abs(e0)
+
+
 @pytest.mark.parametrize("R", [64, 128])
 def test_hxv_vector_gather_degenerate_sectors(ed, orc, R, monkeypatch):
     """The vector-gather down pass on sectors where one spin has a single
