@@ -1361,34 +1361,50 @@ void build_dw_order(Sector& s, hipStream_t st) {
   if (!csr.empty())
     EDL_HIP(hipMemcpyAsync(csr.data(), t.csr, sizeof(uint32_t) * csr.size(), hipMemcpyDeviceToHost, st));
   EDL_HIP(hipStreamSynchronize(st));
-  // Cuthill-McKee: breadth first over the hop graph, each component from a
-  // string of least degree, a string's unvisited neighbours by increasing
-  // degree (then index)
+  // Cuthill-McKee: breadth first over the hop graph, a string's unvisited
+  // neighbours by increasing degree (then index), each component from a
+  // pseudo-peripheral string (the last string reached from the component's
+  // string of least degree, then the last reached from that one), which
+  // keeps the breadth-first levels narrow.
   auto deg = [&](int j) { return off[j + 1] - off[j]; };
   std::vector<int32_t> ord;
   ord.reserve(dim);
-  std::vector<char> seen(dim, 0);
-  std::vector<int32_t> starts(dim);
-  std::iota(starts.begin(), starts.end(), 0);
-  std::stable_sort(starts.begin(), starts.end(), [&](int a, int b) { return deg(a) < deg(b); });
-  std::vector<int32_t> nb;
-  for (int st0 : starts) {
-    if (seen[st0]) continue;
-    seen[st0] = 1;
-    size_t head = ord.size();
-    ord.push_back(st0);
-    while (head < ord.size()) {
-      const int u = ord[head++];
+  std::vector<int32_t> mark(dim, -1);  // pass that last reached a string
+  int pass = 0;
+  std::vector<int32_t> level, nb;
+  // breadth-first pass `pass` from s over the strings not yet in `ord`,
+  // visiting the order into `out`
+  auto bfs = [&](int s, std::vector<int32_t>& out) {
+    out.clear();
+    mark[s] = pass;
+    out.push_back(s);
+    for (size_t head = 0; head < out.size(); ++head) {
+      const int u = out[head];
       nb.clear();
       for (int k = off[u]; k < off[u + 1]; ++k) {
         const int v = (int)(csr[k] & 0xFFFFFFu);
-        if (!seen[v]) {
-          seen[v] = 1;
+        if (mark[v] != pass && mark[v] != -2) {
+          mark[v] = pass;
           nb.push_back(v);
         }
       }
-      std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) != deg(b) ? deg(a) < deg(b) : a < b; });
-      ord.insert(ord.end(), nb.begin(), nb.end());
+      std::sort(nb.begin(), nb.end(), [&](int x, int y) { return deg(x) != deg(y) ? deg(x) < deg(y) : x < y; });
+      out.insert(out.end(), nb.begin(), nb.end());
+    }
+    ++pass;
+  };
+  std::vector<int32_t> starts(dim);
+  std::iota(starts.begin(), starts.end(), 0);
+  std::stable_sort(starts.begin(), starts.end(), [&](int x, int y) { return deg(x) < deg(y); });
+  for (int s0 : starts) {
+    // (one round per component where the hops go both ways; a one-way hop
+    // pattern can leave s0 outside the last pass's strings: then again)
+    while (mark[s0] != -2) {
+      bfs(s0, level);
+      bfs(level.back(), level);
+      bfs(level.back(), level);
+      for (int v : level) mark[v] = -2;
+      ord.insert(ord.end(), level.begin(), level.end());
     }
   }
   EDL_CHECK((int)ord.size() == dim, "build_dw_order: the order does not cover the strings");
