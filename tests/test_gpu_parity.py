@@ -334,11 +334,19 @@ def test_hxv_ordered_down_pass_vs_oracle(ed, orc, R, th, nch, dn, dk, monkeypatc
     """The vector-gather down pass taking its column chunks in the down
     spin's locality order (Cuthill-McKee over the hop graph; the vectors keep
     their layout), pinned: H x v on four sectors of the 3x4 cluster against
-    the oracle in fp64 and fp32 (padded and natural layouts), and a mixed
-    ground state whose chains run the same form with the Lanczos terms
-    fused (the pin carries over to the chain's down pass)."""
-    monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", f"3,1024,256,{R},{th},{nch},{dn},{dk},1,8,1,1,0,1")
-    monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", f"1,512,512,{R},{th},{nch},{dn},{dk},2,8,1,0,0,1")
+    the oracle in fp64 and fp32 (the engine's padded layout), bit-identical
+    to the same shape in the natural order of columns (a column's hops are
+    summed in the same order in both), and a mixed ground state whose chains
+    run the same form with the Lanczos terms fused (the pin carries over to
+    the chain's down pass)."""
+    pins = {o: (f"3,1024,256,{R},{th},{nch},{dn},{dk},1,8,1,1,0,{o}", f"1,512,512,{R},{th},{nch},{dn},{dk},2,8,1,0,0,{o}")
+            for o in (0, 1)}
+
+    def pin(o):
+        monkeypatch.setenv("EDL_TWO_PASS_CFG_F32", pins[o][0])
+        monkeypatch.setenv("EDL_TWO_PASS_CFG_F64", pins[o][1])
+
+    pin(1)
     m = _use(ed, orc, lattice.square(3, 4), [6.0])
     for sec in ((7, 6), (6, 6), (5, 6), (6, 4)):
         isec = ed.sector_index(*sec)
@@ -348,8 +356,14 @@ def test_hxv_ordered_down_pass_vs_oracle(ed, orc, R, th, nch, dn, dk, monkeypatc
             assert ed.bench_passes(isec, prec, 1)[2]["dw_vector_gather"] == 2
         v = RNG.standard_normal(dim)
         ref = orc.matvec(m, *sec, v, nthreads=8).real
-        assert _relerr(ed.direct_matvec(isec, v, "fp64"), ref) < 1e-13
-        assert _relerr(ed.direct_matvec(isec, v, "fp32"), ref) < 2e-6
+        h1 = {p: ed.direct_matvec(isec, v, p) for p in ("fp64", "fp32")}
+        assert _relerr(h1["fp64"], ref) < 1e-13
+        assert _relerr(h1["fp32"], ref) < 2e-6
+        pin(0)
+        for prec in ("fp32", "fp64"):
+            assert ed.bench_passes(isec, prec, 1)[2]["dw_vector_gather"] == 1
+            np.testing.assert_array_equal(ed.direct_matvec(isec, v, prec), h1[prec])
+        pin(1)
     isec = ed.sector_index(6, 6)
     Hc = orc.hamiltonian_csr(m, 6, 6)
     e0, _, _ = orc.lanczos_ground_state(Hc, lattice.splitmix_vector(Hc.shape[0], 0x5EED), 200, want_vector=False)
